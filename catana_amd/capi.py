@@ -1,0 +1,159 @@
+"""ctypes binding of the C-ABI (include/catana_hip.h) for device-resident data.
+
+torch provides device memory and streams; every computation runs in
+libcatana_hip.so.  Loading fails loudly when the library is missing.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libcatana_hip.so")
+
+OK, INVALID_ARGUMENT, DOMAIN_ERROR, HIP_ERROR, OUT_OF_MEMORY, NOT_SUPPORTED = range(6)
+PREC_FP64, PREC_MIXED = 0, 1
+BESSEL_EXACT, BESSEL_REFERENCE, BESSEL_AUTO = 0, 1, 2
+ZEROS_REFERENCE, ZEROS_CONVERGED = 0, 1
+
+EXPORTED = [
+    "sfbhip_default_params", "sfbhip_create", "sfbhip_destroy", "sfbhip_flmn_count", "sfbhip_kln",
+    "sfbhip_direct", "sfbhip_pixelize", "sfbhip_healpix", "sfbhip_last_kernel_ms", "sfbhip_last_error",
+]
+
+
+class Params(ctypes.Structure):
+    _fields_ = [
+        ("lmax", ctypes.c_int32), ("nmax", ctypes.c_int32), ("rmax", ctypes.c_double),
+        ("k_max", ctypes.c_double), ("window_volume", ctypes.c_double), ("precision", ctypes.c_int32),
+        ("bessel", ctypes.c_int32), ("zeros", ctypes.c_int32), ("alias_compat", ctypes.c_int32),
+        ("verbose", ctypes.c_int32), ("reserved", ctypes.c_int32 * 3),
+    ]
+
+
+class SfbError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def lib():
+    """Load libcatana_hip.so (no fallback: a missing library is an error)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is not built; run __graft_entry__.build() or catana_amd/build.py")
+        L = ctypes.CDLL(LIB_PATH)
+        P = ctypes.c_void_p
+        i32, i64 = ctypes.c_int32, ctypes.c_int64
+        L.sfbhip_default_params.argtypes = [ctypes.POINTER(Params)]
+        L.sfbhip_create.argtypes = [ctypes.POINTER(P), ctypes.c_int]
+        L.sfbhip_destroy.argtypes = [P]
+        L.sfbhip_flmn_count.restype = ctypes.c_size_t
+        L.sfbhip_flmn_count.argtypes = [i32, i32]
+        L.sfbhip_kln.argtypes = [P, ctypes.POINTER(Params), P, P]
+        L.sfbhip_direct.argtypes = [P, P, i64, ctypes.POINTER(Params), P, P, P, P]
+        L.sfbhip_pixelize.argtypes = [P, P, i64, i32, P, P, P]
+        L.sfbhip_healpix.argtypes = [P, P, P, i32, ctypes.POINTER(Params), P, P, P, P]
+        L.sfbhip_last_kernel_ms.argtypes = [P, ctypes.POINTER(ctypes.c_float)]
+        L.sfbhip_last_error.restype = ctypes.c_char_p
+        L.sfbhip_last_error.argtypes = []
+        _lib = L
+    return _lib
+
+
+def check(rc, what):
+    if rc == OK:
+        return
+    msg = lib().sfbhip_last_error().decode()
+    if rc in (INVALID_ARGUMENT, DOMAIN_ERROR):
+        raise ValueError(f"{what}: {msg}")
+    raise SfbError(f"{what}: status {rc}: {msg}")
+
+
+def make_params(lmax, nmax, rmax, k_max=0.0, window_volume=0.0, precision=PREC_FP64, bessel=BESSEL_AUTO,
+                zeros=ZEROS_REFERENCE, alias_compat=True, verbose=False):
+    p = Params()
+    lib().sfbhip_default_params(ctypes.byref(p))
+    p.lmax, p.nmax, p.rmax = int(lmax), int(nmax), float(rmax)
+    p.k_max, p.window_volume = float(k_max), float(window_volume)
+    p.precision, p.bessel, p.zeros = int(precision), int(bessel), int(zeros)
+    p.alias_compat, p.verbose = int(bool(alias_compat)), int(bool(verbose))
+    return p
+
+
+def flmn_count(lmax, nmax):
+    return int(lib().sfbhip_flmn_count(int(lmax), int(nmax)))
+
+
+def split_flmn(flat, lmax, nmax):
+    """Flat complex f_lmn -> list of (l+1, nmax) arrays (the reference's KClkk.f_lmn)."""
+    c = np.asarray(flat).view(np.complex128).reshape(-1)
+    out, off = [], 0
+    for l in range(lmax):
+        out.append(c[off:off + (l + 1) * nmax].reshape(l + 1, nmax).copy())
+        off += (l + 1) * nmax
+    return out
+
+
+class Handle:
+    """Owns an sfbhip_handle bound to one device."""
+
+    def __init__(self, device=0):
+        self._h = ctypes.c_void_p()
+        check(lib().sfbhip_create(ctypes.byref(self._h), int(device)), "sfbhip_create")
+        self.device = device
+
+    def close(self):
+        if self._h:
+            lib().sfbhip_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def ptr(self):
+        return self._h
+
+    def kln(self, params):
+        k = np.zeros((params.lmax, params.nmax))
+        nl = np.zeros(params.lmax, dtype=np.int32)
+        check(lib().sfbhip_kln(self._h, ctypes.byref(params), k.ctypes.data_as(ctypes.c_void_p),
+                               nl.ctypes.data_as(ctypes.c_void_p)), "sfbhip_kln")
+        return k, nl
+
+    def last_kernel_ms(self):
+        ms = ctypes.c_float()
+        check(lib().sfbhip_last_kernel_ms(self._h, ctypes.byref(ms)), "sfbhip_last_kernel_ms")
+        return ms.value
+
+
+def _torch():
+    import torch
+    return torch
+
+
+def sfb_direct_torch(handle, points, lmax, nmax, rmax, store_flmn=False, stream=None, **kw):
+    """points: torch float64 tensor (N, 3) on a GPU, rows (r, theta, phi).
+    Returns (k_ln, c_ln, f_lmn|None) as torch tensors on the same device."""
+    torch = _torch()
+    assert points.dtype == torch.float64 and points.dim() == 2 and points.shape[1] == 3
+    pts = points.contiguous()
+    p = make_params(lmax, nmax, rmax, **kw)
+    dev = pts.device
+    k = torch.empty((lmax, nmax), dtype=torch.float64, device=dev)
+    c = torch.empty((lmax, nmax), dtype=torch.float64, device=dev)
+    f = torch.empty(2 * flmn_count(lmax, nmax), dtype=torch.float64, device=dev) if store_flmn else None
+    if stream is None:
+        stream = torch.cuda.current_stream(dev).cuda_stream
+    rc = lib().sfbhip_direct(handle.ptr, ctypes.c_void_p(pts.data_ptr()), ctypes.c_int64(pts.shape[0]),
+                             ctypes.byref(p), ctypes.c_void_p(k.data_ptr()), ctypes.c_void_p(c.data_ptr()),
+                             ctypes.c_void_p(f.data_ptr()) if f is not None else None,
+                             ctypes.c_void_p(stream) if stream else None)
+    check(rc, "sfbhip_direct")
+    return k, c, f

@@ -1,0 +1,541 @@
+// sfb_capi.hip -- the C-ABI of libcatana_hip.so (declared in include/catana_hip.h).
+//
+// Host side of the drop-in boundary: validates arguments the way the
+// reference does (exceptions become status codes), keeps a per-device
+// workspace and cached k_ln / Legendre / interpolation tables in the handle,
+// plans the work items of the direct and HEALPix kernels and launches them on
+// the caller's stream.
+#include "catana_hip.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "sfb_common.hpp"
+#include "sfb_direct_f64.hpp"
+#include "sfb_math.hpp"
+#include "sfb_support.hpp"
+
+using namespace sfb;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+sfbhip_status fail(sfbhip_status st, const std::string &msg) {
+  g_last_error = msg;
+  return st;
+}
+
+#define HIPCHK(expr)                                                                          \
+  do {                                                                                        \
+    hipError_t e_ = (expr);                                                                   \
+    if (e_ != hipSuccess) return fail(SFBHIP_HIP_ERROR, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+constexpr double kPi = 3.14159265358979323846;
+
+int64_t cdiv(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+}  // namespace
+
+struct DevBuf {
+  void *p = nullptr;
+  size_t bytes = 0;
+};
+
+struct KlnCache {
+  bool valid = false;
+  int lmax = -1, nmax = -1, mode = -1;
+  double rmax = 0, kmax = 0;
+  std::vector<double> zeros;  // [lmax][nmax+1]
+  std::vector<double> kln;    // [lmax][nmax]
+  std::vector<int32_t> nl;
+  bool table_valid = false;   // reference interpolation table on the device
+};
+
+struct sfbhip_handle {
+  int device = 0;
+  std::map<std::string, DevBuf> bufs;
+  int *h_err = nullptr;  // pinned
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  float last_ms = 0.f;
+  KlnCache kc;
+  int leg_lmax = -1;
+};
+
+namespace {
+
+template <class T>
+sfbhip_status get_buf(sfbhip_handle *h, const std::string &name, size_t count, T **out) {
+  DevBuf &b = h->bufs[name];
+  const size_t need = std::max<size_t>(count * sizeof(T), 256);
+  if (b.bytes < need) {
+    if (b.p) HIPCHK(hipFree(b.p));
+    b.p = nullptr;
+    b.bytes = 0;
+    hipError_t e = hipMalloc(&b.p, need);
+    if (e != hipSuccess) {
+      b.p = nullptr;
+      return fail(SFBHIP_OUT_OF_MEMORY, "hipMalloc(" + std::to_string(need) + " bytes) for " + name + ": " +
+                                            hipGetErrorString(e));
+    }
+    b.bytes = need;
+  }
+  *out = static_cast<T *>(b.p);
+  return SFBHIP_OK;
+}
+
+#define CHK(expr)                           \
+  do {                                      \
+    sfbhip_status s_ = (expr);              \
+    if (s_ != SFBHIP_OK) return s_;         \
+  } while (0)
+
+sfbhip_status validate(const sfbhip_params *p) {
+  if (!p) return fail(SFBHIP_INVALID_ARGUMENT, "params is NULL");
+  if (p->lmax < 0 || p->lmax > 4096) return fail(SFBHIP_INVALID_ARGUMENT, "lmax must be in [0, 4096]");
+  if (p->nmax < 1 || p->nmax > 65535) return fail(SFBHIP_INVALID_ARGUMENT, "nmax must be in [1, 65535]");
+  if (!(p->rmax > 0.0) || !std::isfinite(p->rmax)) return fail(SFBHIP_INVALID_ARGUMENT, "rmax must be positive");
+  if (p->precision != SFBHIP_PREC_FP64 && p->precision != SFBHIP_PREC_MIXED)
+    return fail(SFBHIP_INVALID_ARGUMENT, "unknown precision mode");
+  if (p->bessel < 0 || p->bessel > 2) return fail(SFBHIP_INVALID_ARGUMENT, "unknown bessel mode");
+  if (p->zeros != SFBHIP_ZEROS_REFERENCE && p->zeros != SFBHIP_ZEROS_CONVERGED)
+    return fail(SFBHIP_INVALID_ARGUMENT, "unknown zeros mode");
+  return SFBHIP_OK;
+}
+
+// Bessel zeros on the device (one thread per l), then k_ln and n_l on the host.
+sfbhip_status ensure_kln(sfbhip_handle *h, const sfbhip_params *p, hipStream_t st) {
+  KlnCache &c = h->kc;
+  const double kmax = p->k_max > 0 ? p->k_max : 0.0;
+  if (c.valid && c.lmax == p->lmax && c.nmax == p->nmax && c.mode == p->zeros && c.rmax == p->rmax &&
+      c.kmax == kmax)
+    return SFBHIP_OK;
+  c.valid = false;
+  c.table_valid = false;
+  const int L = p->lmax, nz = p->nmax + 1;
+  c.zeros.assign((size_t)std::max(L, 0) * nz, 0.0);
+  c.kln.assign((size_t)std::max(L, 0) * p->nmax, 0.0);
+  c.nl.assign(std::max(L, 0), 0);
+  if (L > 0) {
+    double *dz;
+    CHK(get_buf(h, "zeros", (size_t)L * nz, &dz));
+    bessel_zeros_kernel<<<cdiv(L, 64), 64, 0, st>>>(L, nz, p->zeros, dz);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(c.zeros.data(), dz, sizeof(double) * L * nz, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    for (int l = 0; l < L; ++l) {
+      int cnt = 0;
+      for (int n = 0; n < p->nmax; ++n) {
+        const double z = c.zeros[(size_t)l * nz + n];
+        c.kln[(size_t)l * p->nmax + n] = z / p->rmax;
+        if (kmax <= 0.0 || z < kmax * p->rmax) cnt = n + 1;
+      }
+      // zeros increase with n, so the k_max cut keeps a prefix
+      c.nl[l] = cnt;
+    }
+    double *dk;
+    int32_t *dnl;
+    CHK(get_buf(h, "kln", (size_t)L * p->nmax, &dk));
+    CHK(get_buf(h, "nl", (size_t)L, &dnl));
+    HIPCHK(hipMemcpyAsync(dk, c.kln.data(), sizeof(double) * L * p->nmax, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(dnl, c.nl.data(), sizeof(int32_t) * L, hipMemcpyHostToDevice, st));
+  }
+  c.lmax = L;
+  c.nmax = p->nmax;
+  c.mode = p->zeros;
+  c.rmax = p->rmax;
+  c.kmax = kmax;
+  c.valid = true;
+  return SFBHIP_OK;
+}
+
+// Reference interpolation table: P = 1000*nmax nodes per l on [0, z_{l,nmax}].
+sfbhip_status ensure_table(sfbhip_handle *h, const sfbhip_params *p, hipStream_t st, BesselTableRef *t) {
+  KlnCache &c = h->kc;
+  const int L = p->lmax, P = 1000 * p->nmax, nz = p->nmax + 1;
+  double *y, *zm, *dxi;
+  CHK(get_buf(h, "tab_y", (size_t)std::max(L, 1) * P, &y));
+  CHK(get_buf(h, "tab_zmax", (size_t)std::max(L, 1), &zm));
+  CHK(get_buf(h, "tab_dxinv", (size_t)std::max(L, 1), &dxi));
+  if (!c.table_valid && L > 0) {
+    std::vector<double> hz(L), hd(L);
+    for (int l = 0; l < L; ++l) {
+      hz[l] = c.zeros[(size_t)l * nz + p->nmax];
+      hd[l] = 1.0 / (hz[l] / (P - 1));
+    }
+    HIPCHK(hipMemcpyAsync(zm, hz.data(), sizeof(double) * L, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(dxi, hd.data(), sizeof(double) * L, hipMemcpyHostToDevice, st));
+    const int64_t tot = (int64_t)L * P;
+    bessel_table_kernel<<<cdiv(tot, 256), 256, 0, st>>>(L, P, zm, y);
+    HIPCHK(hipGetLastError());
+    c.table_valid = true;
+  }
+  t->y = y;
+  t->zmax = zm;
+  t->dx_inv = dxi;
+  t->P = P;
+  return SFBHIP_OK;
+}
+
+// Legendre constants for l < lmax (see LegendreTables).
+sfbhip_status ensure_legendre(sfbhip_handle *h, int lmax, hipStream_t st, LegendreTables *lt) {
+  double2 *sect, *coef;
+  int32_t *off;
+  const size_t ncoef = (size_t)lmax * (lmax + 1) / 2 + 1;
+  CHK(get_buf(h, "leg_sect", (size_t)std::max(lmax, 1), &sect));
+  CHK(get_buf(h, "leg_coef", ncoef, &coef));
+  CHK(get_buf(h, "leg_off", (size_t)std::max(lmax, 1), &off));
+  if (h->leg_lmax < lmax) {
+    std::vector<double2> hs(std::max(lmax, 1)), hc(ncoef);
+    std::vector<int32_t> ho(std::max(lmax, 1));
+    double prod = 1.0;
+    int32_t o = 0;
+    for (int l = 0; l < lmax; ++l) {
+      if (l > 0) prod *= (2.0 * l - 1.0) / (2.0 * l);
+      double c = std::sqrt((2.0 * l + 1.0) / (4.0 * kPi) * prod);
+      if (l & 1) c = -c;
+      int e;
+      const double mnt = std::frexp(c, &e);
+      hs[l] = make_double2(mnt, (double)e);
+      ho[l] = o;
+      hc[o] = make_double2(0.0, 0.0);
+      for (int m = 1; m <= l; ++m) {
+        const double den = std::sqrt((double)(l + m) * (double)(l - m + 1));
+        hc[o + m] = make_double2(2.0 * m / den, std::sqrt((double)(l - m) * (double)(l + m + 1)) / den);
+      }
+      o += l + 1;
+    }
+    HIPCHK(hipMemcpyAsync(sect, hs.data(), sizeof(double2) * hs.size(), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(coef, hc.data(), sizeof(double2) * hc.size(), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(off, ho.data(), sizeof(int32_t) * ho.size(), hipMemcpyHostToDevice, st));
+    HIPCHK(hipStreamSynchronize(st));  // host vectors go out of scope
+    h->leg_lmax = lmax;
+  }
+  lt->sect = sect;
+  lt->coef = coef;
+  lt->coef_off = off;
+  return SFBHIP_OK;
+}
+
+double norm_factor(const sfbhip_params *p, double npts) {
+  if (p->window_volume > 0.0) return std::sqrt(2. / kPi) * p->window_volume / npts;
+  double nf = std::sqrt(2. / kPi) * (4. / 3.) * kPi;  // sfb_raw.cpp:29-30
+  nf *= p->rmax * p->rmax * p->rmax / npts;
+  return nf;
+}
+
+std::vector<int64_t> flmn_offsets(int lmax, int nmax) {
+  std::vector<int64_t> off(std::max(lmax, 1));
+  int64_t o = 0;
+  for (int l = 0; l < lmax; ++l) {
+    off[l] = o;
+    o += (int64_t)(l + 1) * nmax;
+  }
+  return off;
+}
+
+sfbhip_status zero_err(sfbhip_handle *h, hipStream_t st, int **derr) {
+  CHK(get_buf(h, "err", 4, derr));
+  fill_int_kernel<<<1, 64, 0, st>>>(*derr, 4, 0);
+  HIPCHK(hipGetLastError());
+  return SFBHIP_OK;
+}
+
+sfbhip_status read_err(sfbhip_handle *h, hipStream_t st, int *derr, int *out) {
+  HIPCHK(hipMemcpyAsync(h->h_err, derr, sizeof(int), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  *out = *h->h_err;
+  return SFBHIP_OK;
+}
+
+int qclass(int q) {
+  int c = 1;
+  while (c < q) c <<= 1;
+  return c;
+}
+
+// Items of a row-tiled direct kernel: groups (l, n-tile), split over point
+// ranges, with m chunks split into passes of at most qmax_cap chunks.
+struct DirectPlan {
+  std::vector<DirectItem> items[6];  // by QMAX class 1,2,4,8,16,32
+  std::vector<ReduceGroup> groups;
+  int64_t partial_size = 0;
+};
+
+void plan_direct(const std::vector<int32_t> &nl, int lmax, int64_t npts, int nt, int tp, int mc, int qmax_cap,
+                 int64_t target_items, DirectPlan *pl) {
+  int64_t ngroups = 0, passes_total = 0;
+  for (int l = 0; l < lmax; ++l) {
+    const int64_t g = cdiv(nl[l], nt);
+    ngroups += g;
+    passes_total += g * cdiv(l / mc + 1, qmax_cap);
+  }
+  if (ngroups == 0) return;
+  int64_t S = std::max<int64_t>(1, cdiv(target_items, passes_total));
+  S = std::min<int64_t>(S, cdiv(npts, tp));
+  const int64_t chunk = cdiv(cdiv(npts, S), tp) * tp;
+  S = cdiv(npts, chunk);
+  for (int l = lmax - 1; l >= 0; --l) {
+    const int Q = l / mc + 1;
+    const int ncol = 2 * (l + 1);
+    for (int n0 = 0; n0 < nl[l]; n0 += nt) {
+      ReduceGroup g;
+      g.l = l;
+      g.n0 = n0;
+      g.nrows = std::min(nt, nl[l] - n0);
+      g.nsplit = (int32_t)S;
+      g.base = pl->partial_size;
+      g.stride = (int64_t)nt * ncol;
+      pl->partial_size += S * g.stride;
+      pl->groups.push_back(g);
+      for (int64_t s = 0; s < S; ++s) {
+        for (int qlo = 0; qlo < Q; qlo += qmax_cap) {
+          DirectItem it;
+          it.l = l;
+          it.n0 = n0;
+          it.nrows = g.nrows;
+          it.qlo = qlo;
+          it.qhi = std::min(Q, qlo + qmax_cap);
+          it.pad = 0;
+          it.p0 = s * chunk;
+          it.p1 = std::min(npts, (s + 1) * chunk);
+          it.out = g.base + s * g.stride;
+          const int qc = qclass(it.qhi - it.qlo);
+          int ci = 0;
+          while ((1 << ci) < qc) ++ci;
+          pl->items[ci].push_back(it);
+        }
+      }
+    }
+  }
+}
+
+template <int QMAX>
+void launch_f64(int nitems, const DirectItem *items, hipStream_t st, const PointsSoA &pts, const double *kln,
+                int nmax, const LegendreTables &lt, int use_table, const BesselTableRef &tab, double *partial,
+                int *err) {
+  sfb_direct_f64_kernel<QMAX><<<nitems, 256, 0, st>>>(pts, items, kln, nmax, lt, use_table, tab, partial, err);
+}
+
+sfbhip_status prep_points(sfbhip_handle *h, const double *d_points, int64_t n, hipStream_t st, int *err,
+                          PointsSoA *pts) {
+  double *buf;
+  CHK(get_buf(h, "soa", (size_t)6 * n, &buf));
+  double *r = buf, *x = buf + n, *s = buf + 2 * n, *ph = buf + 3 * n, *cp = buf + 4 * n, *sp = buf + 5 * n;
+  prep_points_kernel<<<cdiv(n, 256), 256, 0, st>>>(d_points, n, r, x, s, ph, cp, sp, err);
+  HIPCHK(hipGetLastError());
+  pts->r = r;
+  pts->x = x;
+  pts->s = s;
+  pts->phi = ph;
+  pts->cphi = cp;
+  pts->sphi = sp;
+  pts->n = n;
+  return SFBHIP_OK;
+}
+
+// f_lmn buffer (the caller's or the workspace), zeroed; reduce; c_ln; k_ln copy.
+sfbhip_status finalize(sfbhip_handle *h, const sfbhip_params *p, const std::vector<ReduceGroup> &groups,
+                       const double *partial, double norm, double *d_k_ln, double *d_c_ln, double *d_f_lmn,
+                       hipStream_t st) {
+  const int L = p->lmax, nmax = p->nmax;
+  const size_t nf = sfbhip_flmn_count(L, nmax);
+  double *f = d_f_lmn;
+  if (!f) CHK(get_buf(h, "flmn", 2 * nf, &f));
+  if (nf) {
+    fill_kernel<<<cdiv(2 * nf, 256), 256, 0, st>>>(f, (int64_t)(2 * nf), 0.0);
+    HIPCHK(hipGetLastError());
+  }
+  std::vector<int64_t> hoff = flmn_offsets(L, nmax);
+  int64_t *doff;
+  ReduceGroup *dg;
+  CHK(get_buf(h, "flmn_off", hoff.size(), &doff));
+  CHK(get_buf(h, "groups", std::max<size_t>(groups.size(), 1), &dg));
+  HIPCHK(hipMemcpyAsync(doff, hoff.data(), sizeof(int64_t) * hoff.size(), hipMemcpyHostToDevice, st));
+  if (!groups.empty()) {
+    HIPCHK(hipMemcpyAsync(dg, groups.data(), sizeof(ReduceGroup) * groups.size(), hipMemcpyHostToDevice, st));
+    double *dk;
+    CHK(get_buf(h, "kln", (size_t)L * nmax, &dk));
+    reduce_partials_kernel<<<(unsigned)groups.size(), 256, 0, st>>>(dg, partial, 0, dk, nmax, doff, norm, f);
+    HIPCHK(hipGetLastError());
+  }
+  if (L > 0) {
+    int32_t *dnl;
+    double *dk;
+    CHK(get_buf(h, "nl", (size_t)L, &dnl));
+    CHK(get_buf(h, "kln", (size_t)L * nmax, &dk));
+    cln_kernel<<<cdiv((int64_t)L * nmax, 256), 256, 0, st>>>(L, nmax, dnl, doff, f, d_c_ln);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(d_k_ln, dk, sizeof(double) * L * nmax, hipMemcpyDeviceToDevice, st));
+  }
+  return SFBHIP_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+sfbhip_status sfbhip_default_params(sfbhip_params *p) {
+  if (!p) return fail(SFBHIP_INVALID_ARGUMENT, "params is NULL");
+  std::memset(p, 0, sizeof(*p));
+  p->lmax = 0;
+  p->nmax = 1;
+  p->rmax = 1.0;
+  p->k_max = 0.0;
+  p->window_volume = 0.0;
+  p->precision = SFBHIP_PREC_FP64;
+  p->bessel = SFBHIP_BESSEL_AUTO;
+  p->zeros = SFBHIP_ZEROS_REFERENCE;
+  p->alias_compat = 1;
+  p->verbose = 0;
+  return SFBHIP_OK;
+}
+
+sfbhip_status sfbhip_create(sfbhip_handle **out, int device) {
+  if (!out) return fail(SFBHIP_INVALID_ARGUMENT, "out is NULL");
+  int ndev = 0;
+  HIPCHK(hipGetDeviceCount(&ndev));
+  if (device < 0 || device >= ndev) return fail(SFBHIP_INVALID_ARGUMENT, "no such device");
+  HIPCHK(hipSetDevice(device));
+  sfbhip_handle *h = new sfbhip_handle();
+  h->device = device;
+  hipError_t e = hipHostMalloc((void **)&h->h_err, sizeof(int) * 4, hipHostMallocDefault);
+  if (e != hipSuccess) {
+    delete h;
+    return fail(SFBHIP_HIP_ERROR, std::string("hipHostMalloc: ") + hipGetErrorString(e));
+  }
+  HIPCHK(hipEventCreate(&h->ev0));
+  HIPCHK(hipEventCreate(&h->ev1));
+  *out = h;
+  return SFBHIP_OK;
+}
+
+sfbhip_status sfbhip_destroy(sfbhip_handle *h) {
+  if (!h) return SFBHIP_OK;
+  (void)hipSetDevice(h->device);
+  (void)hipDeviceSynchronize();
+  for (auto &kv : h->bufs)
+    if (kv.second.p) (void)hipFree(kv.second.p);
+  if (h->h_err) (void)hipHostFree(h->h_err);
+  if (h->ev0) (void)hipEventDestroy(h->ev0);
+  if (h->ev1) (void)hipEventDestroy(h->ev1);
+  delete h;
+  return SFBHIP_OK;
+}
+
+size_t sfbhip_flmn_count(int32_t lmax, int32_t nmax) {
+  if (lmax <= 0 || nmax <= 0) return 0;
+  return (size_t)lmax * (lmax + 1) / 2 * (size_t)nmax;
+}
+
+sfbhip_status sfbhip_kln(sfbhip_handle *h, const sfbhip_params *p, double *k_ln_host, int32_t *nl_host) {
+  if (!h) return fail(SFBHIP_INVALID_ARGUMENT, "handle is NULL");
+  CHK(validate(p));
+  HIPCHK(hipSetDevice(h->device));
+  CHK(ensure_kln(h, p, nullptr));
+  HIPCHK(hipStreamSynchronize(nullptr));
+  if (k_ln_host) std::memcpy(k_ln_host, h->kc.kln.data(), sizeof(double) * h->kc.kln.size());
+  if (nl_host) std::memcpy(nl_host, h->kc.nl.data(), sizeof(int32_t) * h->kc.nl.size());
+  return SFBHIP_OK;
+}
+
+sfbhip_status sfbhip_last_kernel_ms(sfbhip_handle *h, float *ms) {
+  if (!h || !ms) return fail(SFBHIP_INVALID_ARGUMENT, "NULL argument");
+  *ms = h->last_ms;
+  return SFBHIP_OK;
+}
+
+const char *sfbhip_last_error(void) { return g_last_error.c_str(); }
+
+sfbhip_status sfbhip_direct(sfbhip_handle *h, const double *d_points, int64_t n_points, const sfbhip_params *p,
+                            double *d_k_ln, double *d_c_ln, double *d_f_lmn, void *stream) {
+  g_last_error.clear();
+  if (!h) return fail(SFBHIP_INVALID_ARGUMENT, "handle is NULL");
+  CHK(validate(p));
+  if (n_points <= 0 || !d_points)
+    return fail(SFBHIP_INVALID_ARGUMENT, "empty catalogue: the normalisation divides by the number of points");
+  if (!d_k_ln || !d_c_ln) return fail(SFBHIP_INVALID_ARGUMENT, "k_ln / c_ln output is NULL");
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  HIPCHK(hipSetDevice(h->device));
+  if (p->verbose) {
+    std::printf("SFB decomposition (raw method): lmax=%d nmax=%d Rmax=%g\n", p->lmax, p->nmax, p->rmax);
+    std::printf("Catalog consists of %lld points.\n", (long long)n_points);
+  }
+  int *derr;
+  CHK(zero_err(h, st, &derr));
+  CHK(ensure_kln(h, p, st));
+  LegendreTables lt;
+  CHK(ensure_legendre(h, std::max(p->lmax, 1), st, &lt));
+  PointsSoA pts;
+  CHK(prep_points(h, d_points, n_points, st, derr, &pts));
+  const double norm = norm_factor(p, (double)n_points);
+  double *dk;
+  CHK(get_buf(h, "kln", (size_t)std::max(p->lmax, 1) * p->nmax, &dk));
+
+  DirectPlan pl;
+  double *partial = nullptr;
+  h->last_ms = 0.f;
+  if (p->precision == SFBHIP_PREC_FP64) {
+    const bool use_table = p->bessel == SFBHIP_BESSEL_REFERENCE || (p->bessel == SFBHIP_BESSEL_AUTO && n_points > 1000);
+    BesselTableRef tab{nullptr, nullptr, nullptr, 0};
+    if (use_table) CHK(ensure_table(h, p, st, &tab));
+    plan_direct(h->kc.nl, p->lmax, n_points, F64_NT, F64_TP, F64_MC, 16, 4096, &pl);
+    CHK(get_buf(h, "partial", (size_t)std::max<int64_t>(pl.partial_size, 1), &partial));
+    size_t nitems = 0;
+    for (auto &v : pl.items) nitems += v.size();
+    DirectItem *ditems;
+    CHK(get_buf(h, "items", std::max<size_t>(nitems, 1), &ditems));
+    std::vector<DirectItem> all;
+    all.reserve(nitems);
+    for (auto &v : pl.items) all.insert(all.end(), v.begin(), v.end());
+    if (nitems)
+      HIPCHK(hipMemcpyAsync(ditems, all.data(), sizeof(DirectItem) * nitems, hipMemcpyHostToDevice, st));
+    HIPCHK(hipEventRecord(h->ev0, st));
+    size_t off = 0;
+    for (int ci = 0; ci < 5; ++ci) {
+      const int n = (int)pl.items[ci].size();
+      if (!n) continue;
+      const DirectItem *di = ditems + off;
+      switch (ci) {
+        case 0: launch_f64<1>(n, di, st, pts, dk, p->nmax, lt, use_table, tab, partial, derr); break;
+        case 1: launch_f64<2>(n, di, st, pts, dk, p->nmax, lt, use_table, tab, partial, derr); break;
+        case 2: launch_f64<4>(n, di, st, pts, dk, p->nmax, lt, use_table, tab, partial, derr); break;
+        case 3: launch_f64<8>(n, di, st, pts, dk, p->nmax, lt, use_table, tab, partial, derr); break;
+        default: launch_f64<16>(n, di, st, pts, dk, p->nmax, lt, use_table, tab, partial, derr); break;
+      }
+      HIPCHK(hipGetLastError());
+      off += n;
+    }
+    HIPCHK(hipEventRecord(h->ev1, st));
+  } else {
+    return fail(SFBHIP_NOT_SUPPORTED, "mixed precision path not built yet");
+  }
+  CHK(finalize(h, p, pl.groups, partial, norm, d_k_ln, d_c_ln, d_f_lmn, st));
+  int e = 0;
+  CHK(read_err(h, st, derr, &e));
+  if (!pl.groups.empty()) HIPCHK(hipEventElapsedTime(&h->last_ms, h->ev0, h->ev1));
+  if (e & ERR_BADPOINT) return fail(SFBHIP_DOMAIN_ERROR, "point with r < 0 or a non-finite coordinate");
+  if (e & ERR_DOMAIN) return fail(SFBHIP_DOMAIN_ERROR, "argument (z) outside of defined range.");
+  return SFBHIP_OK;
+}
+
+sfbhip_status sfbhip_pixelize(sfbhip_handle *h, const double *d_points, int64_t n_points, int32_t nside,
+                              int64_t *d_offsets, double *d_radii, void *stream) {
+  return fail(SFBHIP_NOT_SUPPORTED, "HEALPix path not built yet");
+}
+
+sfbhip_status sfbhip_healpix(sfbhip_handle *h, const int64_t *d_offsets, const double *d_radii, int32_t nside,
+                             const sfbhip_params *p, double *d_k_ln, double *d_c_ln, double *d_f_lmn,
+                             void *stream) {
+  return fail(SFBHIP_NOT_SUPPORTED, "HEALPix path not built yet");
+}
+
+}  // extern "C"

@@ -1,0 +1,59 @@
+// sfb_common.hpp -- types shared by the kernels and the C-ABI layer.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace sfb {
+
+typedef double v4d __attribute__((ext_vector_type(4)));
+typedef float v4f __attribute__((ext_vector_type(4)));
+typedef float v16f __attribute__((ext_vector_type(16)));
+
+// Device-side error bits (OR-ed into one int by any thread that hits them).
+enum : int {
+  ERR_DOMAIN = 1,   // Bessel argument outside the reference table range
+  ERR_BADPOINT = 2  // r < 0 or a non-finite coordinate
+};
+
+// Per-point data derived once from the AoS (r, theta, phi) input; SoA, FP64.
+struct PointsSoA {
+  const double *r, *x, *s, *phi, *cphi, *sphi;
+  int64_t n;
+};
+
+// One work item of the direct kernels: multipole l, rows [n0, n0+nrows) of
+// the mode index n, points [p0, p1), and where its partial sums go.
+struct DirectItem {
+  int32_t l;
+  int32_t n0;
+  int32_t nrows;     // valid rows (<= tile height)
+  int32_t qlo, qhi;  // m chunks [qlo, qhi) handled by this item
+  int32_t pad;
+  int64_t p0, p1;
+  int64_t out;       // offset (in doubles) of this item's partial block
+};
+
+// One (l, row-tile) group: its items are `nsplit` consecutive point ranges
+// whose partial blocks sit `stride` doubles apart starting at `base`.
+struct ReduceGroup {
+  int32_t l;
+  int32_t n0;
+  int32_t nrows;
+  int32_t nsplit;
+  int64_t base;
+  int64_t stride;
+};
+
+// Per-l constants for the Legendre rows:
+//   sect[l] = (mantissa, exponent) of (-1)^l sqrt((2l+1)/(4 pi) prod_{k<=l} (2k-1)/(2k)),
+//   so that P_l^l(cos t) = sect * sin^l t;
+//   coef[coef_off[l] + m] = (a_m, b_m) of the downward recurrence
+//   P_l^{m-1} = -(a_m cot t P_l^m + b_m P_l^{m+1}).
+struct LegendreTables {
+  const double2 *sect;
+  const double2 *coef;
+  const int32_t *coef_off;
+};
+
+}  // namespace sfb

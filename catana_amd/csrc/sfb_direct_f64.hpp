@@ -1,0 +1,166 @@
+// sfb_direct_f64.hpp -- FP64 direct point-mode SFB kernel (the drop-in path
+// and the GPU oracle).
+//
+// Per l the reference computes f_lmn = sum_i k_ln j_l(k_ln r_i) Y*_lm(t_i, p_i)
+// (sfb_raw.cpp:61-80), i.e. a product of the (n x points) Bessel matrix with
+// the (points x 2(l+1)) matrix of real/imaginary Y* columns.  A work item is
+// (l, 32 rows of n, a range of points).  Per 128-point tile the workgroup
+//   1. evaluates the 32 x 128 Bessel tile into LDS (all 256 threads),
+//   2. walks m from l down to 0 in chunks of 16 (32 columns): threads 0..127
+//      advance their point's Legendre/azimuth recurrence and write the chunk
+//      into LDS, then the four waves contract it on the FP64 matrix cores
+//      (v_mfma_f64_16x16x4f64), each wave owning one 16 x 16 output tile of
+//      the 32 x 32 chunk block.
+// Accumulators stay in registers for the whole point range (QMAX chunks of
+// double4 per lane), and the item's 32 x 2(l+1) block is written once.  An
+// item covers m chunks [qlo, qhi); for l >= 16*QMAX the columns are split over
+// several items, each of which first advances the recurrence from m = l down
+// to its own top chunk.
+#pragma once
+
+#include "sfb_common.hpp"
+#include "sfb_math.hpp"
+#include "sfb_ylm.hpp"
+
+namespace sfb {
+
+constexpr int F64_NT = 32;    // rows (n) per item
+constexpr int F64_TP = 128;   // points per tile
+constexpr int F64_MC = 16;    // m values per chunk (32 real columns)
+constexpr int F64_LDP = F64_TP + 1;  // LDS row pitch (doubles)
+
+struct BesselTableRef {
+  const double *y;       // [lmax][P] values of j_l at i*dx
+  const double *zmax;    // [lmax]
+  const double *dx_inv;  // [lmax]
+  int32_t P;             // nodes per l
+};
+
+template <int QMAX>
+__global__ __launch_bounds__(256) void sfb_direct_f64_kernel(
+    PointsSoA pts, const DirectItem *__restrict__ items, const double *__restrict__ kln, int nmax,
+    LegendreTables leg, int use_table, BesselTableRef tab, double *__restrict__ partial,
+    int *__restrict__ err) {
+  __shared__ double Js[F64_NT * F64_LDP];
+  __shared__ double Ys[2 * F64_MC * F64_LDP];
+
+  const DirectItem it = items[blockIdx.x];
+  const int l = it.l;
+  const int ncol = 2 * (l + 1);
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
+  const int rt = wave >> 1, ct = wave & 1;
+
+  const double *kl = kln + (size_t)l * nmax + it.n0;
+  const double2 sect = leg.sect[l];
+  const double2 *coef = leg.coef + leg.coef_off[l];
+
+  v4d acc[QMAX];
+#pragma unroll
+  for (int q = 0; q < QMAX; ++q) acc[q] = v4d{0.0, 0.0, 0.0, 0.0};
+
+  int myerr = 0;
+  for (int64_t base = it.p0; base < it.p1; base += F64_TP) {
+    // ---- 1. Bessel tile: thread -> point (tid & 127), rows (tid >> 7) + 2j
+    {
+      const int p = tid & (F64_TP - 1);
+      const int64_t gp = base + p;
+      const bool ok = gp < it.p1;
+      const double r = ok ? pts.r[gp] : 0.0;
+      for (int j = 0; j < F64_NT / 2; ++j) {
+        const int row = (tid >> 7) + 2 * j;
+        double val = 0.0;
+        if (ok && row < it.nrows) {
+          const double z = kl[row] * r;
+          if (use_table) {
+            const double *y = tab.y + (size_t)l * tab.P;
+            if (!(z < tab.zmax[l])) {
+              myerr |= ERR_DOMAIN;
+            } else {
+              const double ia = z * tab.dx_inv[l];
+              int i = (int)ia;
+              i = i > tab.P - 2 ? tab.P - 2 : i;
+              val = y[i] + (y[i + 1] - y[i]) * (ia - i);
+            }
+          } else {
+            val = sbessel_jl(l, z);
+          }
+        }
+        Js[row * F64_LDP + p] = val;
+      }
+    }
+    // ---- 2. Legendre/azimuth rows, chunk by chunk from high m to low m
+    YlmDown g;
+    const bool ygen = tid < F64_TP;
+    const int64_t gp = base + tid;
+    const bool yok = ygen && gp < it.p1;
+    if (ygen) {
+      if (yok) {
+        g.init(l, pts.x[gp], pts.s[gp], pts.phi[gp], pts.cphi[gp], pts.sphi[gp], sect);
+        // advance to the top m of the first chunk this item handles
+        for (int m = l; m >= F64_MC * it.qhi; --m) {
+          g.step(coef[m]);
+          g.rescale();
+        }
+      } else {
+        g.v = g.vp = 0.0;
+        g.E = 0;
+        g.cm = g.sm = g.cot = g.cph = g.sph = 0.0;
+      }
+    }
+#pragma unroll
+    for (int qq = 0; qq < QMAX; ++qq) {
+      const int q = it.qhi - 1 - qq;
+      if (q >= it.qlo) {
+        __syncthreads();  // previous readers of Ys (and Js writers) are done
+        if (ygen) {
+          const int mlo = q * F64_MC;
+          for (int mm = F64_MC - 1; mm >= 0; --mm) {
+            const int m = mlo + mm;
+            double re = 0.0, im = 0.0;
+            if (m <= l) {
+              g.value(re, im);
+              if (m > 0) {
+                g.step(coef[m]);
+                g.rescale();
+              }
+            }
+            Ys[(2 * mm) * F64_LDP + tid] = re;
+            Ys[(2 * mm + 1) * F64_LDP + tid] = im;
+          }
+        }
+        __syncthreads();
+        // contraction of the 32 x 32 chunk block: wave (rt, ct) owns a 16 x 16 tile
+        const double *Arow = Js + (16 * rt + (lane & 15)) * F64_LDP + (lane >> 4);
+        const double *Brow = Ys + (16 * ct + (lane & 15)) * F64_LDP + (lane >> 4);
+        v4d c = acc[qq];
+#pragma unroll 8
+        for (int s = 0; s < F64_TP / 4; ++s) {
+          c = __builtin_amdgcn_mfma_f64_16x16x4f64(Arow[4 * s], Brow[4 * s], c, 0, 0, 0);
+        }
+        acc[qq] = c;
+      }
+    }
+    __syncthreads();  // Js is rewritten by the next tile
+  }
+  if (myerr) atomicOr(err, myerr);
+
+  // ---- write the item's partial block (rows x 2(l+1), row-major)
+  double *out = partial + it.out;
+#pragma unroll
+  for (int qq = 0; qq < QMAX; ++qq) {
+    const int q = it.qhi - 1 - qq;
+    if (q >= it.qlo) {
+      const int col = q * 2 * F64_MC + 16 * ct + (lane & 15);
+      // v_mfma_f64_16x16x4f64 result: element j of lane i is row i/16 + 4j, column i%16
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int row = 16 * rt + (lane >> 4) + 4 * j;
+        if (col < ncol) out[(size_t)row * ncol + col] = acc[qq][j];
+      }
+    }
+  }
+}
+
+}  // namespace sfb

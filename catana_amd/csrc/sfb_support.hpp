@@ -1,0 +1,101 @@
+// sfb_support.hpp -- the small kernels around the contraction: point
+// preparation, Bessel zeros, the reference interpolation table, the
+// deterministic reduction of partial blocks and the C_l assembly.
+#pragma once
+
+#include "sfb_common.hpp"
+#include "sfb_math.hpp"
+
+namespace sfb {
+
+// AoS (r, theta, phi) -> SoA (r, cos t, sin t, phi, cos phi, sin phi); flags
+// r < 0 and non-finite input.
+__global__ void prep_points_kernel(const double *__restrict__ aos, int64_t n, double *__restrict__ r,
+                                   double *__restrict__ x, double *__restrict__ s, double *__restrict__ phi,
+                                   double *__restrict__ cphi, double *__restrict__ sphi, int *__restrict__ err) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double rr = aos[3 * i], th = aos[3 * i + 1], ph = aos[3 * i + 2];
+  double st, ctt, sp, cp;
+  sincos(th, &st, &ctt);
+  sincos(ph, &sp, &cp);
+  r[i] = rr;
+  x[i] = ctt;
+  s[i] = fabs(st);
+  phi[i] = ph;
+  cphi[i] = cp;
+  sphi[i] = sp;
+  if (!(rr >= 0.0) || !isfinite(rr) || !isfinite(th) || !isfinite(ph)) atomicOr(err, ERR_BADPOINT);
+}
+
+// One thread per l: zeros z_{l,0..nz-1} of j_l, row-major [l][nz].
+__global__ void bessel_zeros_kernel(int lmax, int nz, int mode, double *__restrict__ z) {
+  const int l = blockIdx.x * blockDim.x + threadIdx.x;
+  if (l >= lmax) return;
+  bessel_zeros(l, nz, mode, z + (size_t)l * nz);
+}
+
+// Reference linear table (SBesselInterpolator.cpp:10-17, FunctionInterpolator.cpp:10-17):
+// P = 1000*nmax nodes x_i = i*dx on [0, z_{l,nmax}], dx = z_{l,nmax}/(P-1).
+__global__ void bessel_table_kernel(int lmax, int P, const double *__restrict__ zmax, double *__restrict__ y) {
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (int64_t)lmax * P) return;
+  const int l = (int)(idx / P);
+  const int i = (int)(idx - (int64_t)l * P);
+  const double dx = zmax[l] / (P - 1);
+  y[idx] = sbessel_jl(l, 0.0 + i * dx);
+}
+
+// Sum the partial blocks of a group in split order and scale by norm * k_ln:
+// f[l][m][n] (complex, m >= 0, n < nmax).  One block per group.
+__global__ void reduce_partials_kernel(const ReduceGroup *__restrict__ groups, const double *__restrict__ partial,
+                                       int row_pitch_rows, const double *__restrict__ kln, int nmax,
+                                       const int64_t *__restrict__ flmn_off, double norm,
+                                       double *__restrict__ flmn) {
+  const ReduceGroup g = groups[blockIdx.x];
+  const int ncol = 2 * (g.l + 1);
+  const int total = g.nrows * ncol;
+  for (int e = threadIdx.x; e < total; e += blockDim.x) {
+    const int row = e / ncol, col = e - row * ncol;
+    const double *src = partial + g.base + (size_t)row * ncol + col;
+    double acc = 0.0;
+    for (int s = 0; s < g.nsplit; ++s) acc += src[(size_t)s * g.stride];
+    const int n = g.n0 + row;
+    const int m = col >> 1;
+    const double k = kln[(size_t)g.l * nmax + n];
+    flmn[(flmn_off[g.l] + (size_t)m * nmax + n) * 2 + (col & 1)] = acc * norm * k;
+  }
+  (void)row_pitch_rows;
+}
+
+// c_ln = (|f_l0n|^2 + 2 sum_{m>=1} |f_lmn|^2) / (2l+1)   (sfb_raw.cpp:91-96);
+// columns n >= n_l are zero.
+__global__ void cln_kernel(int lmax, int nmax, const int32_t *__restrict__ nl, const int64_t *__restrict__ flmn_off,
+                           const double *__restrict__ flmn, double *__restrict__ cln) {
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= lmax * nmax) return;
+  const int l = idx / nmax, n = idx - l * nmax;
+  double c = 0.0;
+  if (n < nl[l]) {
+    const double *f = flmn + flmn_off[l] * 2;
+    for (int m = -l; m <= l; ++m) {
+      const int am = m < 0 ? -m : m;
+      const double re = f[((size_t)am * nmax + n) * 2], im = f[((size_t)am * nmax + n) * 2 + 1];
+      c += re * re + im * im;
+    }
+    c /= (2 * l + 1);
+  }
+  cln[idx] = c;
+}
+
+__global__ void fill_kernel(double *__restrict__ p, int64_t n, double v) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+__global__ void fill_int_kernel(int *__restrict__ p, int64_t n, int v) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+}  // namespace sfb

@@ -1,0 +1,88 @@
+// sfb_ylm.hpp -- Legendre x azimuth rows for one (point, l), FP64, O(1) per m.
+//
+// The reference evaluates gsl_sf_legendre_sphPlm(l, m, cos t) for every m
+// (sfb_raw.cpp:72-74), an O(l-m) recurrence each, i.e. O(l^2) per point and l.
+// Here the rows are generated in m from m = l down to 0 with the three-term
+// recurrence of the orthonormal functions at fixed l,
+//   sqrt((l-m)(l+m+1)) P^{m+1} + 2m cot(t) P^m + sqrt((l+m)(l-m+1)) P^{m-1} = 0,
+// started from the sectoral value P_l^l = c_l sin^l t and P_l^{l+1} = 0.
+// Downward in m the wanted solution dominates in the evanescent region
+// (m > l sin t) and is oscillatory below it, so the recurrence is stable.
+// sin^l t underflows FP64 near the poles, so the state carries a binary
+// exponent E (value = v * 2^E) and is renormalised when |v| exceeds 2^600.
+// sin t is clamped to >= 2^-80: the rows of such points differ from the
+// pole values by O(l^2 2^-80).
+#pragma once
+
+#include "sfb_common.hpp"
+
+namespace sfb {
+
+struct YlmDown {
+  double v, vp;  // P^m, P^{m+1} without the 2^E factor
+  int E;
+  double cm, sm;  // cos(m phi), sin(m phi)
+  double cot, cph, sph;
+
+  __device__ __forceinline__ void init(int l, double x, double s, double phi, double cph_, double sph_,
+                                       double2 sect) {
+    const double se = fmax(s, 0x1p-80);
+    cot = x / se;
+    cph = cph_;
+    sph = sph_;
+    // se^l by binary powering, kept normalised with frexp
+    int e = 0, be, t;
+    double m = 1.0;
+    double b = frexp(se, &be);
+    for (int k = l; k; k >>= 1) {
+      if (k & 1) {
+        m *= b;
+        e += be;
+        m = frexp(m, &t);
+        e += t;
+      }
+      b *= b;
+      be *= 2;
+      b = frexp(b, &t);
+      be += t;
+    }
+    v = m * sect.x;
+    E = e + (int)sect.y;
+    vp = 0.0;
+    // e^{i l phi}; l*phi is split exactly into hi + lo
+    const double hi = (double)l * phi;
+    const double lo = fma((double)l, phi, -hi);
+    double S, C;
+    sincos(hi, &S, &C);
+    cm = C - lo * S;
+    sm = S + lo * C;
+  }
+
+  // Y*_lm = P_l^m (cos m phi - i sin m phi)
+  __device__ __forceinline__ void value(double &re, double &im) const {
+    const double P = ldexp(v, E);
+    re = P * cm;
+    im = -P * sm;
+  }
+
+  // m -> m-1 with ab = (a_m, b_m)
+  __device__ __forceinline__ void step(double2 ab) {
+    const double vn = -fma(ab.x * cot, v, ab.y * vp);
+    vp = v;
+    v = vn;
+    const double cn = fma(cm, cph, sm * sph);
+    const double sn = fma(sm, cph, -cm * sph);
+    cm = cn;
+    sm = sn;
+  }
+
+  __device__ __forceinline__ void rescale() {
+    if (fabs(v) > 0x1p+600) {
+      v = ldexp(v, -600);
+      vp = ldexp(vp, -600);
+      E += 600;
+    }
+  }
+};
+
+}  // namespace sfb

@@ -1,0 +1,120 @@
+/*
+ * catana_hip.h -- C-ABI of the MI355X (gfx950) SFB decomposition library
+ * (libcatana_hip.so).  Plain C types only: device pointers, sizes, a stream
+ * passed as void* (a hipStream_t, NULL = the default stream).
+ *
+ * Conventions (those of the reference, catana/include/catana/decomposition/):
+ *   - lmax is exclusive: l in [0, lmax)                       (sfb_raw.cpp:38)
+ *   - k_ln, c_ln: lmax x nmax doubles, row-major, row = l     (return_types.hpp:19-38)
+ *   - f_lmn: m >= 0 only, one (l+1) x nmax complex128 block per l, row-major
+ *     (m, n), blocks consecutive in l; sfbhip_flmn_count() gives the length
+ *   - points: AoS (r, theta, phi) float64, 24 bytes each -- the memory of a
+ *     catana::PointContainer (Point.hpp:17-18)
+ *   - f_lmn = sqrt(2/pi) (V/N) k_ln sum_i j_l(k_ln r_i) Y*_lm(theta_i, phi_i),
+ *     c_ln = sum_{m=-l..l} |f_lmn|^2 / (2l+1), V = (4/3) pi rmax^3 unless a
+ *     window volume is given.
+ * Every entry point is synchronous with respect to the stream it is given:
+ * it returns after its work on that stream has finished, so that domain
+ * errors found on the device can be reported.
+ */
+#ifndef CATANA_HIP_H
+#define CATANA_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum sfbhip_status {
+  SFBHIP_OK = 0,
+  SFBHIP_INVALID_ARGUMENT = 1, /* std::invalid_argument / std::runtime_error in the reference */
+  SFBHIP_DOMAIN_ERROR = 2,     /* std::domain_error (SBesselInterpolator.cpp:20-22) */
+  SFBHIP_HIP_ERROR = 3,
+  SFBHIP_OUT_OF_MEMORY = 4,
+  SFBHIP_NOT_SUPPORTED = 5
+} sfbhip_status;
+
+/* arithmetic of the point contraction */
+enum {
+  SFBHIP_PREC_FP64 = 0,   /* FP64 everywhere: the drop-in and the GPU oracle */
+  SFBHIP_PREC_MIXED = 1   /* FP32 Bessel table + FP64 Legendre rows, split-FP16 MFMA, FP64 tile sums */
+};
+/* how j_l(k r) is evaluated in the FP64 path */
+enum {
+  SFBHIP_BESSEL_EXACT = 0,     /* gsl_sf_bessel_jl semantics (interpolated = false) */
+  SFBHIP_BESSEL_REFERENCE = 1, /* the reference's linear table, 1000*nmax nodes on [0, z_{l,nmax}] */
+  SFBHIP_BESSEL_AUTO = 2       /* the reference's default: table when N > 1000 (sfb_decomposition.cpp:17) */
+};
+/* Bessel-zero finder */
+enum {
+  SFBHIP_ZEROS_REFERENCE = 0, /* SphericalBesselZeros as written (early-stopping Newton) */
+  SFBHIP_ZEROS_CONVERGED = 1  /* Newton run to |delta|/z <= 1e-15 */
+};
+
+typedef struct sfbhip_params {
+  int32_t lmax;          /* exclusive */
+  int32_t nmax;          /* columns of k_ln / c_ln */
+  double rmax;
+  double k_max;          /* > 0: l keeps n_l = #{n < nmax : z_ln < k_max rmax} modes; <= 0: nmax */
+  double window_volume;  /* > 0 replaces (4/3) pi rmax^3 in the normalisation */
+  int32_t precision;     /* SFBHIP_PREC_* */
+  int32_t bessel;        /* SFBHIP_BESSEL_* (FP64 path only) */
+  int32_t zeros;         /* SFBHIP_ZEROS_* */
+  int32_t alias_compat;  /* HEALPix path: 1 keeps the reference's unconjugated fold (sfb_reverse_fft.cpp:202-206) */
+  int32_t verbose;       /* print a one-line summary per call to stdout */
+  int32_t reserved[3];
+} sfbhip_params;
+
+typedef struct sfbhip_handle sfbhip_handle;
+
+/* Fills defaults: precision FP64, bessel AUTO, zeros REFERENCE, alias_compat 1. */
+sfbhip_status sfbhip_default_params(sfbhip_params *p);
+
+/* Library state bound to one device: workspace and cached k_ln tables.
+ * No counterpart in the reference (it keeps no state between calls). */
+sfbhip_status sfbhip_create(sfbhip_handle **out, int device);
+sfbhip_status sfbhip_destroy(sfbhip_handle *h);
+
+/* Number of complex entries in f_lmn for (lmax, nmax): sum_l (l+1)*nmax. */
+size_t sfbhip_flmn_count(int32_t lmax, int32_t nmax);
+
+/* k_ln and the per-l mode counts n_l on the host.
+ * Replaces catana::compute_kln (return_types.cpp:10-19) / KClkk::KClkk (:22-23). */
+sfbhip_status sfbhip_kln(sfbhip_handle *h, const sfbhip_params *p, double *k_ln_host, int32_t *nl_host);
+
+/* Direct point-mode SFB on device-resident points.
+ * Replaces catana::_sfb_raw (sfb_raw.cpp:13-108) and, with bessel = AUTO,
+ * catana::sfb_decomposition(const PointContainer&, ...) (sfb_decomposition.cpp:15-19).
+ * d_k_ln, d_c_ln: lmax*nmax doubles (device); d_f_lmn: NULL or 2*flmn_count doubles (device). */
+sfbhip_status sfbhip_direct(sfbhip_handle *h, const double *d_points, int64_t n_points,
+                            const sfbhip_params *p, double *d_k_ln, double *d_c_ln, double *d_f_lmn,
+                            void *stream);
+
+/* HEALPix RING binning of device points into CSR order by pixel: d_offsets
+ * (12 nside^2 + 1 int64), d_radii (n_points doubles), radii of a pixel kept in
+ * input order.  Replaces PixelizedPointContainer(nside, pc)
+ * (PixelizedPointContainer.cpp:11-30); nside must be a power of 2. */
+sfbhip_status sfbhip_pixelize(sfbhip_handle *h, const double *d_points, int64_t n_points, int32_t nside,
+                              int64_t *d_offsets, double *d_radii, void *stream);
+
+/* HEALPix SFB on a pixelised catalogue (CSR as produced by sfbhip_pixelize).
+ * Replaces catana::_sfb_reverse_fft (sfb_reverse_fft.cpp:43-209) and
+ * catana::sfb_decomposition(const PixelizedPointContainer&, ...) (sfb_decomposition.cpp:22-26);
+ * with alias_compat = 0 it equals catana::_sfb_reverse (sfb_reverse.cpp:13-113). */
+sfbhip_status sfbhip_healpix(sfbhip_handle *h, const int64_t *d_offsets, const double *d_radii,
+                             int32_t nside, const sfbhip_params *p, double *d_k_ln, double *d_c_ln,
+                             double *d_f_lmn, void *stream);
+
+/* Duration in ms of the dominant kernel of the last call on this handle,
+ * measured with HIP events on the call's stream (0 if none). */
+sfbhip_status sfbhip_last_kernel_ms(sfbhip_handle *h, float *ms);
+
+/* Message of the last error on the calling thread ("" if none). */
+const char *sfbhip_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

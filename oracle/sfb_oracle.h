@@ -1,0 +1,78 @@
+/*
+ * sfb_oracle.h -- CPU restatement of CatAna's spherical Fourier-Bessel (SFB)
+ * decomposition hot path, in plain C99 + OpenMP, FP64 throughout.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the parity checker for the HIP
+ * product path and the CPU timing baseline.  It is linked or loaded only by
+ * tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg; nothing in
+ * the shipped library (catana_amd/) may call it.
+ *
+ * Conventions (all follow the reference):
+ *   - lmax is exclusive: l in [0, lmax)            (sfb_raw.cpp:38)
+ *   - k_ln and c_ln are (lmax x nmax) row-major, row = l, column = n
+ *   - f_lmn is stored only for m >= 0, as one (l+1) x nmax complex block per l,
+ *     row-major (m, n), interleaved (re, im); blocks are consecutive in l.
+ *   - points are AoS (r, theta, phi) doubles, 24 bytes each, the memory layout
+ *     of catana::Point / PointContainer (Point.hpp:17-18)
+ */
+#ifndef SFB_ORACLE_H
+#define SFB_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { ORC_OK = 0, ORC_EINVAL = 1, ORC_EDOMAIN = 2, ORC_ENOMEM = 3 };
+
+/* zero finder modes */
+enum { ORC_ZEROS_REFERENCE = 0, ORC_ZEROS_CONVERGED = 1 };
+
+/* ---- special functions ---- */
+double orc_sbessel_jl(int l, double x);                 /* j_l(x), FP64 */
+double orc_sph_plm(int l, int m, double x);             /* orthonormal P_l^m(x), Condon-Shortley phase */
+int    orc_bessel_zeros(int l, int n, int mode, double *z); /* first n positive zeros of j_l */
+int    orc_kln(int lmax, int nmax, double rmax, int mode, double *k_ln);
+int    orc_count_nl(int lmax, int nmax, double rmax, double k_max, int mode, int *nl);
+
+/* linear interpolation table of j_l on [0, z_{l,nmax}] with 1000*nmax nodes
+ * (SBesselInterpolator.cpp:10-26, FunctionInterpolator.cpp:7-30). */
+int    orc_interp_table(int l, int nmax, int mode, double *y, int *npts, double *zmax, double *dx_inv);
+
+/* ---- direct point-mode SFB (restates sfb_raw.cpp:13-108) ----
+ * nl:            per-l count of n modes (NULL -> nmax for every l); n >= nl[l] left zero
+ * volume:        window volume V in the normalisation (<= 0 -> (4/3) pi rmax^3)
+ * interpolated:  0 exact j_l, 1 reference linear table
+ * nthreads:      OpenMP threads (<= 0 -> default)
+ * f_lmn:         NULL or sum_l (l+1)*nmax complex (2 doubles each)
+ */
+int orc_sfb_raw(const double *pts, long long npts, int lmax, int nmax, double rmax,
+                const int *nl, double volume, int interpolated, int zeros_mode, int nthreads,
+                double *k_ln, double *c_ln, double *f_lmn);
+
+/* ---- HEALPix RING geometry (Gorski et al. 2005) ---- */
+long long orc_ang2pix_ring(int nside, double theta, double phi);
+void orc_pix2ang_ring(int nside, long long pix, double *theta, double *phi);
+void orc_ring_info(int nside, int ring, long long *startpix, int *ringpix, int *shifted,
+                   double *z, double *phi0);
+
+/* bin points by RING pixel: counting sort into CSR (offsets npix+1, radii npts) */
+int orc_pixelize(const double *pts, long long npts, int nside, long long *offsets, double *radii);
+
+/* ---- pixelised SFB ----
+ * restates sfb_reverse_fft.cpp:43-209 (alias_compat = 1 keeps the missing
+ * conjugate when m folds past npix_ring/2, :202-206; 0 conjugates) */
+int orc_sfb_reverse_fft(const long long *offsets, const double *radii, int nside, int lmax, int nmax,
+                        double rmax, const int *nl, double volume, int interpolated, int zeros_mode,
+                        int alias_compat, int nthreads, double *k_ln, double *c_ln, double *f_lmn);
+/* restates sfb_reverse.cpp:13-113 (exact pixelised transform, no FFT) */
+int orc_sfb_reverse(const long long *offsets, const double *radii, int nside, int lmax, int nmax,
+                    double rmax, const int *nl, double volume, int interpolated, int zeros_mode,
+                    int nthreads, double *k_ln, double *c_ln, double *f_lmn);
+
+/* KClkk::savetxt text format (return_types.cpp:25-64): precision 18, scientific */
+int orc_savetxt(const char *path, const double *a, int rows, int cols);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
