@@ -1,0 +1,150 @@
+"""Benchmark of the direct (point-mode) SFB decomposition on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload clustered1m|lognormal2p25|uniform10k]
+                    [--precision mixed|fp64] [--cpu-sample N | --no-cpu-baseline]
+
+A step is one full sfbhip_direct call (k_ln lookup, point preparation, the
+contraction kernels, the FP64 reduction and C_l assembly) on points already
+resident in HBM.  The metric is point-mode pairs/s = N * sum_l (2l+1) n_l /
+step time.  Prints one JSON line.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "SFB point-mode pairs/s (N·Σ_l(2l+1)n_l); mixed vs FP64 GPU at ≤1e-6 rel C_l"
+
+# peaks (MI355X datasheet, dense)
+PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3, "fp16": 2516.6}
+
+WORKLOADS = {
+    # 1,000,000 clustered points (20,000 parents x Poisson(50), sigma 5, clipped), lmax 64, k_max 0.2, rmax 1000
+    "clustered1m": dict(lmax=64, nmax=64, rmax=1000.0, k_max=0.2),
+    # 2^25 lognormal points, lmax 256, k_max 0.5, rmax 1000
+    "lognormal2p25": dict(lmax=256, nmax=160, rmax=1000.0, k_max=0.5),
+    # 10,000 uniform points, lmax 16, k_max 0.1, rmax 500
+    "uniform10k": dict(lmax=16, nmax=16, rmax=500.0, k_max=0.1),
+}
+
+
+def make_points(name):
+    from catana_amd import workloads as W
+    if name == "clustered1m":
+        return W.clustered_ball(20000, 1000.0, mean_children=50, sigma=5.0, seed=2)
+    if name == "lognormal2p25":
+        return W.lognormal_ball(2 ** 25, rmax=1000.0, ngrid=256, k0=0.1, seed=5)
+    if name == "uniform10k":
+        return W.uniform_ball(10000, 500.0, seed=1)
+    raise ValueError(name)
+
+
+def cpu_baseline(points, cfg, nl, sample, threads):
+    """The oracle (C/OpenMP restatement of sfb_raw.cpp:13-108 with the public
+    wrapper's interpolated = N > 1000, sfb_decomposition.cpp:17) timed on a
+    seeded subsample of the same catalogue; pairs/s scale linearly in N."""
+    from oracle import oracle as O
+    rng = np.random.default_rng(123)
+    n = min(sample, points.shape[0])
+    sub = points[np.sort(rng.choice(points.shape[0], n, replace=False))]
+    t0 = time.perf_counter()
+    O.sfb_raw(sub, cfg["lmax"], cfg["nmax"], cfg["rmax"], nl=nl, interpolated=n > 1000, nthreads=threads)
+    dt = time.perf_counter() - t0
+    l = np.arange(cfg["lmax"])
+    pairs = n * int(((2 * l + 1) * nl).sum())
+    return {"value": pairs / dt, "unit": "pairs/s", "cores": threads, "kind": "port",
+            "sample": f"{n} of {points.shape[0]} points (seeded subset), same lmax/k_max, "
+                      f"reference-default linear Bessel table, {dt:.2f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="clustered1m", choices=sorted(WORKLOADS))
+    ap.add_argument("--precision", default="mixed", choices=["mixed", "fp64"])
+    ap.add_argument("--cpu-sample", type=int, default=65536)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-accuracy", action="store_true")
+    args = ap.parse_args()
+
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print("bench.py: this build runs on one GPU only (the SFB path shards by points with one "
+              "all-reduce of f_lmn; see DESIGN.md); --gpus must be 1", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from catana_amd import capi
+
+    cfg = WORKLOADS[args.workload]
+    pts = make_points(args.workload)
+    N = pts.shape[0]
+    dev = torch.device("cuda:0")
+    d_pts = torch.from_numpy(np.ascontiguousarray(pts)).to(dev)
+    h = capi.Handle(0)
+    prec = capi.PREC_MIXED if args.precision == "mixed" else capi.PREC_FP64
+    kw = dict(k_max=cfg["k_max"], precision=prec, bessel=capi.BESSEL_EXACT)
+    p = capi.make_params(cfg["lmax"], cfg["nmax"], cfg["rmax"], **kw)
+    _, nl = h.kln(p)
+    l = np.arange(cfg["lmax"])
+    pairs = N * int(((2 * l + 1) * nl).sum())
+    flops = 4.0 * N * int(((l + 1) * nl).sum())
+
+    stream = torch.cuda.current_stream(dev)
+    for _ in range(args.warmup):
+        capi.sfb_direct_torch(h, d_pts, cfg["lmax"], cfg["nmax"], cfg["rmax"], **kw)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    kern_ms = []
+    t0 = time.perf_counter()
+    e0.record(stream)
+    for _ in range(args.steps):
+        k, c, _ = capi.sfb_direct_torch(h, d_pts, cfg["lmax"], cfg["nmax"], cfg["rmax"], **kw)
+        kern_ms.append(h.last_kernel_ms())
+    e1.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms = e0.elapsed_time(e1) / args.steps
+    kms = float(np.mean(kern_ms))
+
+    dtype = "f64" if prec == capi.PREC_FP64 else "f32"
+    peak = PEAK_TFLOPS["fp64"] if prec == capi.PREC_FP64 else PEAK_TFLOPS["fp32"]
+    achieved = flops / (kms * 1e-3) / 1e12
+    out = {
+        "metric": METRIC, "value": pairs / (ms * 1e-3), "unit": "pairs/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": dtype,
+        "data": "synthetic, seeded (catana_amd/workloads.py), inputs resident in HBM",
+        "config": {"workload": args.workload, "N": int(N), "lmax": cfg["lmax"], "k_max": cfg["k_max"],
+                   "rmax": cfg["rmax"], "precision": args.precision, "pairs_per_step": pairs,
+                   "wall_s_timed_region": wall},
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
+                     "frac": achieved / peak, "traffic": None,
+                     "kernel": "sfb_direct_%s_kernel" % ("f64" if prec == capi.PREC_FP64 else "mixed"),
+                     "kernel_ms": kms, "algorithmic_flops": flops},
+    }
+    if prec != capi.PREC_FP64 and not args.no_accuracy:
+        k64, c64, _ = capi.sfb_direct_torch(h, d_pts, cfg["lmax"], cfg["nmax"], cfg["rmax"],
+                                            k_max=cfg["k_max"], precision=capi.PREC_FP64,
+                                            bessel=capi.BESSEL_EXACT)
+        a, b = c.cpu().numpy(), c64.cpu().numpy()
+        mask = b != 0
+        rel = np.abs(a[mask] / b[mask] - 1)
+        out["accuracy"] = {"max_rel_cln_vs_fp64": float(rel.max()), "median_rel_cln_vs_fp64": float(np.median(rel))}
+    if not args.no_cpu_baseline:
+        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        out["cpu_baseline"] = cpu_baseline(pts, cfg, nl, args.cpu_sample, threads)
+    h.close()
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -27,7 +27,7 @@ class Params(ctypes.Structure):
         ("lmax", ctypes.c_int32), ("nmax", ctypes.c_int32), ("rmax", ctypes.c_double),
         ("k_max", ctypes.c_double), ("window_volume", ctypes.c_double), ("precision", ctypes.c_int32),
         ("bessel", ctypes.c_int32), ("zeros", ctypes.c_int32), ("alias_compat", ctypes.c_int32),
-        ("verbose", ctypes.c_int32), ("reserved", ctypes.c_int32 * 3),
+        ("verbose", ctypes.c_int32), ("mixed_fp64_below", ctypes.c_int32), ("reserved", ctypes.c_int32 * 2),
     ]
 
 
@@ -73,13 +73,15 @@ def check(rc, what):
 
 
 def make_params(lmax, nmax, rmax, k_max=0.0, window_volume=0.0, precision=PREC_FP64, bessel=BESSEL_AUTO,
-                zeros=ZEROS_REFERENCE, alias_compat=True, verbose=False):
+                zeros=ZEROS_REFERENCE, alias_compat=True, verbose=False, mixed_fp64_below=None):
     p = Params()
     lib().sfbhip_default_params(ctypes.byref(p))
     p.lmax, p.nmax, p.rmax = int(lmax), int(nmax), float(rmax)
     p.k_max, p.window_volume = float(k_max), float(window_volume)
     p.precision, p.bessel, p.zeros = int(precision), int(bessel), int(zeros)
     p.alias_compat, p.verbose = int(bool(alias_compat)), int(bool(verbose))
+    if mixed_fp64_below is not None:
+        p.mixed_fp64_below = int(mixed_fp64_below)
     return p
 
 

@@ -19,6 +19,7 @@
 
 #include "sfb_common.hpp"
 #include "sfb_direct_f64.hpp"
+#include "sfb_direct_mixed.hpp"
 #include "sfb_math.hpp"
 #include "sfb_support.hpp"
 
@@ -58,6 +59,9 @@ struct KlnCache {
   std::vector<double> kln;    // [lmax][nmax]
   std::vector<int32_t> nl;
   bool table_valid = false;   // reference interpolation table on the device
+  bool herm_valid = false;    // Hermite tables of the mixed path on the device
+  std::vector<int32_t> herm_off, herm_nn;
+  int32_t herm_max = 0;
 };
 
 struct sfbhip_handle {
@@ -120,6 +124,7 @@ sfbhip_status ensure_kln(sfbhip_handle *h, const sfbhip_params *p, hipStream_t s
     return SFBHIP_OK;
   c.valid = false;
   c.table_valid = false;
+  c.herm_valid = false;
   const int L = p->lmax, nz = p->nmax + 1;
   c.zeros.assign((size_t)std::max(L, 0) * nz, 0.0);
   c.kln.assign((size_t)std::max(L, 0) * p->nmax, 0.0);
@@ -270,10 +275,10 @@ struct DirectPlan {
   int64_t partial_size = 0;
 };
 
-void plan_direct(const std::vector<int32_t> &nl, int lmax, int64_t npts, int nt, int tp, int mc, int qmax_cap,
-                 int64_t target_items, DirectPlan *pl) {
+void plan_direct(const std::vector<int32_t> &nl, int lmin, int lmax, int64_t npts, int nt, int tp, int mc,
+                 int qmax_cap, int64_t target_items, DirectPlan *pl) {
   int64_t ngroups = 0, passes_total = 0;
-  for (int l = 0; l < lmax; ++l) {
+  for (int l = lmin; l < lmax; ++l) {
     const int64_t g = cdiv(nl[l], nt);
     ngroups += g;
     passes_total += g * cdiv(l / mc + 1, qmax_cap);
@@ -283,7 +288,7 @@ void plan_direct(const std::vector<int32_t> &nl, int lmax, int64_t npts, int nt,
   S = std::min<int64_t>(S, cdiv(npts, tp));
   const int64_t chunk = cdiv(cdiv(npts, S), tp) * tp;
   S = cdiv(npts, chunk);
-  for (int l = lmax - 1; l >= 0; --l) {
+  for (int l = lmax - 1; l >= lmin; --l) {
     const int Q = l / mc + 1;
     const int ncol = 2 * (l + 1);
     for (int n0 = 0; n0 < nl[l]; n0 += nt) {
@@ -379,6 +384,108 @@ sfbhip_status finalize(sfbhip_handle *h, const sfbhip_params *p, const std::vect
   return SFBHIP_OK;
 }
 
+// Hermite tables of the mixed path: nodes i*h, i < nn_l = floor(k_{l,n_l-1} rmax / h) + 2.
+sfbhip_status ensure_hermite(sfbhip_handle *h, const sfbhip_params *p, hipStream_t st, HermiteTables *ht) {
+  KlnCache &c = h->kc;
+  const int L = p->lmax;
+  if (!c.herm_valid) {
+    c.herm_off.assign(std::max(L, 1), 0);
+    c.herm_nn.assign(std::max(L, 1), 2);
+    int32_t o = 0, mx = 2;
+    for (int l = 0; l < L; ++l) {
+      int32_t nn = 2;
+      if (c.nl[l] > 0) {
+        const double X = c.kln[(size_t)l * p->nmax + c.nl[l] - 1] * p->rmax;
+        nn = (int32_t)std::floor(X / MX_H) + 2;
+      }
+      if (nn > MX_TAB_MAX)
+        return fail(SFBHIP_NOT_SUPPORTED, "mixed path: k_max * rmax too large for the LDS Bessel table (max " +
+                                              std::to_string(MX_TAB_MAX * MX_H) + "); use precision FP64");
+      c.herm_off[l] = o;
+      c.herm_nn[l] = nn;
+      o += nn;
+      mx = std::max(mx, nn);
+    }
+    float2 *tab;
+    int32_t *doff, *dnn;
+    CHK(get_buf(h, "herm_tab", (size_t)std::max(o, 1), &tab));
+    CHK(get_buf(h, "herm_off", c.herm_off.size(), &doff));
+    CHK(get_buf(h, "herm_nn", c.herm_nn.size(), &dnn));
+    HIPCHK(hipMemcpyAsync(doff, c.herm_off.data(), sizeof(int32_t) * c.herm_off.size(), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(dnn, c.herm_nn.data(), sizeof(int32_t) * c.herm_nn.size(), hipMemcpyHostToDevice, st));
+    if (L > 0) {
+      dim3 grid((unsigned)cdiv(mx, 256), (unsigned)L);
+      hermite_table_kernel<<<grid, 256, 0, st>>>(L, doff, dnn, tab);
+      HIPCHK(hipGetLastError());
+    }
+    HIPCHK(hipStreamSynchronize(st));
+    c.herm_max = mx;
+    c.herm_valid = true;
+  }
+  float2 *tab;
+  int32_t *doff, *dnn;
+  CHK(get_buf(h, "herm_tab", 1, &tab));
+  CHK(get_buf(h, "herm_off", 1, &doff));
+  CHK(get_buf(h, "herm_nn", 1, &dnn));
+  ht->tab = tab;
+  ht->off = doff;
+  ht->nnodes = dnn;
+  return SFBHIP_OK;
+}
+
+template <int QMAX>
+sfbhip_status launch_mixed(int nitems, const DirectItem *items, hipStream_t st, const PointsSoA &pts,
+                           const double *kln, int nmax, const LegendreTables &lt, const HermiteTables &ht,
+                           int lds_nodes, double *partial) {
+  const size_t shm = sizeof(float2) * (size_t)lds_nodes;
+  HIPCHK(hipFuncSetAttribute(reinterpret_cast<const void *>(&sfb_direct_mixed_kernel<QMAX>),
+                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
+  sfb_direct_mixed_kernel<QMAX><<<nitems, 256, shm, st>>>(pts, items, kln, nmax, lt, ht, partial);
+  HIPCHK(hipGetLastError());
+  return SFBHIP_OK;
+}
+
+// Launch every class of a plan; items[] on the device holds the classes back to back from `*off`.
+sfbhip_status launch_plan_f64(const DirectPlan &pl, DirectItem *ditems, size_t *off, hipStream_t st,
+                              const PointsSoA &pts, const double *dk, int nmax, const LegendreTables &lt,
+                              int use_table, const BesselTableRef &tab, double *partial, int *derr) {
+  for (int ci = 0; ci < 5; ++ci) {
+    const int n = (int)pl.items[ci].size();
+    if (!n) continue;
+    const DirectItem *di = ditems + *off;
+    switch (ci) {
+      case 0: launch_f64<1>(n, di, st, pts, dk, nmax, lt, use_table, tab, partial, derr); break;
+      case 1: launch_f64<2>(n, di, st, pts, dk, nmax, lt, use_table, tab, partial, derr); break;
+      case 2: launch_f64<4>(n, di, st, pts, dk, nmax, lt, use_table, tab, partial, derr); break;
+      case 3: launch_f64<8>(n, di, st, pts, dk, nmax, lt, use_table, tab, partial, derr); break;
+      default: launch_f64<16>(n, di, st, pts, dk, nmax, lt, use_table, tab, partial, derr); break;
+    }
+    HIPCHK(hipGetLastError());
+    *off += n;
+  }
+  return SFBHIP_OK;
+}
+
+sfbhip_status launch_plan_mixed(sfbhip_handle *h, const DirectPlan &pl, DirectItem *ditems, size_t *off,
+                                hipStream_t st, const PointsSoA &pts, const double *dk, int nmax,
+                                const LegendreTables &lt, const HermiteTables &ht, double *partial) {
+  for (int ci = 0; ci < 4; ++ci) {
+    const int n = (int)pl.items[ci].size();
+    if (!n) continue;
+    int nodes = 2;
+    for (const DirectItem &it : pl.items[ci]) nodes = std::max(nodes, h->kc.herm_nn[it.l]);
+    const DirectItem *di = ditems + *off;
+    switch (ci) {
+      case 0: CHK(launch_mixed<1>(n, di, st, pts, dk, nmax, lt, ht, nodes, partial)); break;
+      case 1: CHK(launch_mixed<2>(n, di, st, pts, dk, nmax, lt, ht, nodes, partial)); break;
+      case 2: CHK(launch_mixed<4>(n, di, st, pts, dk, nmax, lt, ht, nodes, partial)); break;
+      default: CHK(launch_mixed<8>(n, di, st, pts, dk, nmax, lt, ht, nodes, partial)); break;
+    }
+    *off += n;
+  }
+  return SFBHIP_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -396,6 +503,7 @@ sfbhip_status sfbhip_default_params(sfbhip_params *p) {
   p->zeros = SFBHIP_ZEROS_REFERENCE;
   p->alias_compat = 1;
   p->verbose = 0;
+  p->mixed_fp64_below = 8;
   return SFBHIP_OK;
 }
 
@@ -480,48 +588,41 @@ sfbhip_status sfbhip_direct(sfbhip_handle *h, const double *d_points, int64_t n_
   double *dk;
   CHK(get_buf(h, "kln", (size_t)std::max(p->lmax, 1) * p->nmax, &dk));
 
-  DirectPlan pl;
+  // Multipoles l < lsplit run in the FP64 kernel; the rest in the mixed kernel
+  // when precision is MIXED.
+  const bool mixed = p->precision == SFBHIP_PREC_MIXED;
+  const int lsplit = mixed ? std::min(p->lmax, std::max(0, p->mixed_fp64_below)) : p->lmax;
+  const bool use_table =
+      !mixed && (p->bessel == SFBHIP_BESSEL_REFERENCE || (p->bessel == SFBHIP_BESSEL_AUTO && n_points > 1000));
+  BesselTableRef tab{nullptr, nullptr, nullptr, 0};
+  if (use_table) CHK(ensure_table(h, p, st, &tab));
+  HermiteTables ht{nullptr, nullptr, nullptr};
+  if (mixed && lsplit < p->lmax) CHK(ensure_hermite(h, p, st, &ht));
+  DirectPlan pl64, plmx;
+  plan_direct(h->kc.nl, 0, lsplit, n_points, F64_NT, F64_TP, F64_MC, 16, 4096, &pl64);
+  plmx.partial_size = pl64.partial_size;
+  if (mixed) plan_direct(h->kc.nl, lsplit, p->lmax, n_points, MX_NT, MX_TP, MX_MC, 8, 4096, &plmx);
+  std::vector<ReduceGroup> groups = pl64.groups;
+  groups.insert(groups.end(), plmx.groups.begin(), plmx.groups.end());
   double *partial = nullptr;
+  CHK(get_buf(h, "partial", (size_t)std::max<int64_t>(plmx.partial_size, 1), &partial));
+  std::vector<DirectItem> all;
+  for (auto &v : pl64.items) all.insert(all.end(), v.begin(), v.end());
+  for (auto &v : plmx.items) all.insert(all.end(), v.begin(), v.end());
+  DirectItem *ditems;
+  CHK(get_buf(h, "items", std::max<size_t>(all.size(), 1), &ditems));
+  if (!all.empty())
+    HIPCHK(hipMemcpyAsync(ditems, all.data(), sizeof(DirectItem) * all.size(), hipMemcpyHostToDevice, st));
   h->last_ms = 0.f;
-  if (p->precision == SFBHIP_PREC_FP64) {
-    const bool use_table = p->bessel == SFBHIP_BESSEL_REFERENCE || (p->bessel == SFBHIP_BESSEL_AUTO && n_points > 1000);
-    BesselTableRef tab{nullptr, nullptr, nullptr, 0};
-    if (use_table) CHK(ensure_table(h, p, st, &tab));
-    plan_direct(h->kc.nl, p->lmax, n_points, F64_NT, F64_TP, F64_MC, 16, 4096, &pl);
-    CHK(get_buf(h, "partial", (size_t)std::max<int64_t>(pl.partial_size, 1), &partial));
-    size_t nitems = 0;
-    for (auto &v : pl.items) nitems += v.size();
-    DirectItem *ditems;
-    CHK(get_buf(h, "items", std::max<size_t>(nitems, 1), &ditems));
-    std::vector<DirectItem> all;
-    all.reserve(nitems);
-    for (auto &v : pl.items) all.insert(all.end(), v.begin(), v.end());
-    if (nitems)
-      HIPCHK(hipMemcpyAsync(ditems, all.data(), sizeof(DirectItem) * nitems, hipMemcpyHostToDevice, st));
-    HIPCHK(hipEventRecord(h->ev0, st));
-    size_t off = 0;
-    for (int ci = 0; ci < 5; ++ci) {
-      const int n = (int)pl.items[ci].size();
-      if (!n) continue;
-      const DirectItem *di = ditems + off;
-      switch (ci) {
-        case 0: launch_f64<1>(n, di, st, pts, dk, p->nmax, lt, use_table, tab, partial, derr); break;
-        case 1: launch_f64<2>(n, di, st, pts, dk, p->nmax, lt, use_table, tab, partial, derr); break;
-        case 2: launch_f64<4>(n, di, st, pts, dk, p->nmax, lt, use_table, tab, partial, derr); break;
-        case 3: launch_f64<8>(n, di, st, pts, dk, p->nmax, lt, use_table, tab, partial, derr); break;
-        default: launch_f64<16>(n, di, st, pts, dk, p->nmax, lt, use_table, tab, partial, derr); break;
-      }
-      HIPCHK(hipGetLastError());
-      off += n;
-    }
-    HIPCHK(hipEventRecord(h->ev1, st));
-  } else {
-    return fail(SFBHIP_NOT_SUPPORTED, "mixed precision path not built yet");
-  }
-  CHK(finalize(h, p, pl.groups, partial, norm, d_k_ln, d_c_ln, d_f_lmn, st));
+  HIPCHK(hipEventRecord(h->ev0, st));
+  size_t off = 0;
+  CHK(launch_plan_f64(pl64, ditems, &off, st, pts, dk, p->nmax, lt, use_table, tab, partial, derr));
+  if (mixed) CHK(launch_plan_mixed(h, plmx, ditems, &off, st, pts, dk, p->nmax, lt, ht, partial));
+  HIPCHK(hipEventRecord(h->ev1, st));
+  CHK(finalize(h, p, groups, partial, norm, d_k_ln, d_c_ln, d_f_lmn, st));
   int e = 0;
   CHK(read_err(h, st, derr, &e));
-  if (!pl.groups.empty()) HIPCHK(hipEventElapsedTime(&h->last_ms, h->ev0, h->ev1));
+  if (!groups.empty()) HIPCHK(hipEventElapsedTime(&h->last_ms, h->ev0, h->ev1));
   if (e & ERR_BADPOINT) return fail(SFBHIP_DOMAIN_ERROR, "point with r < 0 or a non-finite coordinate");
   if (e & ERR_DOMAIN) return fail(SFBHIP_DOMAIN_ERROR, "argument (z) outside of defined range.");
   return SFBHIP_OK;

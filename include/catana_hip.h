@@ -39,7 +39,8 @@ typedef enum sfbhip_status {
 /* arithmetic of the point contraction */
 enum {
   SFBHIP_PREC_FP64 = 0,   /* FP64 everywhere: the drop-in and the GPU oracle */
-  SFBHIP_PREC_MIXED = 1   /* FP32 Bessel table + FP64 Legendre rows, split-FP16 MFMA, FP64 tile sums */
+  SFBHIP_PREC_MIXED = 1   /* FP32 MFMA on FP32 operands (Hermite Bessel table with FP64 argument
+                             reduction, FP64 Legendre rows), FP64 tile sums; low l in FP64 */
 };
 /* how j_l(k r) is evaluated in the FP64 path */
 enum {
@@ -64,7 +65,9 @@ typedef struct sfbhip_params {
   int32_t zeros;         /* SFBHIP_ZEROS_* */
   int32_t alias_compat;  /* HEALPix path: 1 keeps the reference's unconjugated fold (sfb_reverse_fft.cpp:202-206) */
   int32_t verbose;       /* print a one-line summary per call to stdout */
-  int32_t reserved[3];
+  int32_t mixed_fp64_below; /* mixed path: multipoles l < this run in FP64 (C_l of low l can be
+                               dominated by cancellation); default 8 */
+  int32_t reserved[2];
 } sfbhip_params;
 
 typedef struct sfbhip_handle sfbhip_handle;

@@ -242,8 +242,11 @@ int orc_interp_table(int l, int nmax, int mode, double *y, int *npts, double *zm
   if (npts) *npts = P;
   if (zmax) *zmax = zm;
   if (dx_inv) *dx_inv = 1.0 / dx;
-  if (y)
+  if (y) {
+    /* FunctionInterpolator.cpp:15 fills the table with an OpenMP loop */
+#pragma omp parallel for schedule(static)
     for (int i = 0; i < P; ++i) y[i] = orc_sbessel_jl(l, 0.0 + i * dx);
+  }
   return ORC_OK;
 }
 

@@ -66,3 +66,56 @@ def test_kmax_cut(handle):
     for l in range(16):
         assert np.all(c[l, nl[l]:] == 0)
     assert nl[0] == 15 and nl[15] == 9
+
+
+def _rel(a, b):
+    m = b != 0
+    return np.abs(a[m] / b[m] - 1)
+
+
+def test_mixed_vs_fp64_uniform(handle):
+    pts = workloads.uniform_ball(20000, 500.0, seed=21)
+    print("case: mixed vs fp64, uniform N=20000 lmax=24 k_max=0.15 rmax=500", flush=True)
+    kw = dict(k_max=0.15, bessel=capi.BESSEL_EXACT)
+    k64, c64, f64 = _run(handle, pts, 24, 30, 500.0, precision=capi.PREC_FP64, **kw)
+    km, cm, fm = _run(handle, pts, 24, 30, 500.0, precision=capi.PREC_MIXED, **kw)
+    np.testing.assert_array_equal(km, k64)
+    rel = _rel(cm, c64)
+    print("max rel c_ln", rel.max(), "median", np.median(rel))
+    assert rel.max() < 1e-6
+    assert _flmn_err(fm, f64) < 1e-6
+    assert np.all((cm == 0) == (c64 == 0))
+
+
+def test_mixed_all_l_in_mixed_kernel(handle):
+    # mixed_fp64_below = 0: every l through the FP32-operand kernel; f_lmn stays
+    # within 1e-6 of FP64 relative to each l block's largest coefficient
+    pts = workloads.uniform_ball(20000, 500.0, seed=24)
+    print("case: mixed kernel for all l, uniform N=20000 lmax=24", flush=True)
+    kw = dict(k_max=0.15, bessel=capi.BESSEL_EXACT)
+    _, c64, f64 = _run(handle, pts, 24, 30, 500.0, precision=capi.PREC_FP64, **kw)
+    _, cm, fm = _run(handle, pts, 24, 30, 500.0, precision=capi.PREC_MIXED, mixed_fp64_below=0, **kw)
+    assert _flmn_err(fm, f64) < 1e-6
+    assert np.median(_rel(cm, c64)) < 2e-7
+
+
+def test_mixed_vs_oracle_points_beyond_rmax(handle):
+    # r up to 1.3 rmax: Bessel arguments past the table fall back to FP64 j_l
+    pts = workloads.uniform_ball(4000, 650.0, seed=22)
+    print("case: mixed, points beyond rmax, N=4000 lmax=10 nmax=8 rmax=500", flush=True)
+    k, c, f = _run(handle, pts, 10, 8, 500.0, precision=capi.PREC_MIXED, bessel=capi.BESSEL_EXACT)
+    ko, co, fo = O.sfb_raw(pts, 10, 8, 500.0, store_flmn=True)
+    assert _rel(c, co).max() < 2e-6
+    assert _flmn_err(f, fo) < 2e-6
+
+
+def test_mixed_vs_fp64_clustered_high_l(handle):
+    pts = workloads.clustered_ball(1500, 1000.0, mean_children=40, sigma=5.0, seed=23)
+    print(f"case: mixed vs fp64, clustered N={pts.shape[0]} lmax=140 k_max=0.2 rmax=1000", flush=True)
+    kw = dict(k_max=0.2, bessel=capi.BESSEL_EXACT)
+    _, c64, f64 = _run(handle, pts, 140, 64, 1000.0, precision=capi.PREC_FP64, **kw)
+    _, cm, fm = _run(handle, pts, 140, 64, 1000.0, precision=capi.PREC_MIXED, **kw)
+    rel = _rel(cm, c64)
+    print("max rel c_ln", rel.max(), "median", np.median(rel))
+    assert rel.max() < 1e-6
+    assert _flmn_err(fm, f64) < 1e-6
