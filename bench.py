@@ -70,6 +70,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="clustered1m", choices=sorted(WORKLOADS))
     ap.add_argument("--precision", default="mixed", choices=["mixed", "fp64"])
+    ap.add_argument("--operands", default="split16", choices=["fp32", "split16"],
+                    help="matrix-core operand format of the mixed path")
+    ap.add_argument("--fp64-below", type=int, default=8, help="mixed path: l below this run in FP64")
     ap.add_argument("--cpu-sample", type=int, default=65536)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true")
@@ -91,7 +94,8 @@ def main():
     d_pts = torch.from_numpy(np.ascontiguousarray(pts)).to(dev)
     h = capi.Handle(0)
     prec = capi.PREC_MIXED if args.precision == "mixed" else capi.PREC_FP64
-    kw = dict(k_max=cfg["k_max"], precision=prec, bessel=capi.BESSEL_EXACT)
+    kw = dict(k_max=cfg["k_max"], precision=prec, bessel=capi.BESSEL_EXACT, mixed_fp64_below=args.fp64_below,
+              mixed_operands=capi.OPERANDS_SPLIT_FP16 if args.operands == "split16" else capi.OPERANDS_FP32)
     p = capi.make_params(cfg["lmax"], cfg["nmax"], cfg["rmax"], **kw)
     _, nl = h.kln(p)
     l = np.arange(cfg["lmax"])
@@ -115,8 +119,12 @@ def main():
     ms = e0.elapsed_time(e1) / args.steps
     kms = float(np.mean(kern_ms))
 
-    dtype = "f64" if prec == capi.PREC_FP64 else "f32"
-    peak = PEAK_TFLOPS["fp64"] if prec == capi.PREC_FP64 else PEAK_TFLOPS["fp32"]
+    if prec == capi.PREC_FP64:
+        dtype, peak = "f64", PEAK_TFLOPS["fp64"]
+    elif args.operands == "split16":
+        dtype, peak = "f16x3 (split FP16 operands, f32 accumulate, f64 tile sums)", PEAK_TFLOPS["fp16"] / 3.0
+    else:
+        dtype, peak = "f32 (f32 operands and accumulate, f64 tile sums)", PEAK_TFLOPS["fp32"]
     achieved = flops / (kms * 1e-3) / 1e12
     out = {
         "metric": METRIC, "value": pairs / (ms * 1e-3), "unit": "pairs/s", "n_gpus": 1,
@@ -124,7 +132,8 @@ def main():
         "scaling": "weak", "vs_baseline": None, "dtype": dtype,
         "data": "synthetic, seeded (catana_amd/workloads.py), inputs resident in HBM",
         "config": {"workload": args.workload, "N": int(N), "lmax": cfg["lmax"], "k_max": cfg["k_max"],
-                   "rmax": cfg["rmax"], "precision": args.precision, "pairs_per_step": pairs,
+                   "rmax": cfg["rmax"], "precision": args.precision, "operands": args.operands,
+                   "fp64_below": args.fp64_below, "pairs_per_step": pairs,
                    "wall_s_timed_region": wall},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
                      "frac": achieved / peak, "traffic": None,

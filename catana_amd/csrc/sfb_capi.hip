@@ -62,12 +62,13 @@ struct KlnCache {
   bool herm_valid = false;    // Hermite tables of the mixed path on the device
   std::vector<int32_t> herm_off, herm_nn;
   int32_t herm_max = 0;
+  double herm_R = 0.0;        // radius the tables cover: k_{l,n_l-1} * herm_R is the last argument
 };
 
 struct sfbhip_handle {
   int device = 0;
   std::map<std::string, DevBuf> bufs;
-  int *h_err = nullptr;  // pinned
+  int *h_err = nullptr;  // pinned: [0] error bits, [2..3] largest radius of the last call (double)
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   float last_ms = 0.f;
   KlnCache kc;
@@ -112,6 +113,8 @@ sfbhip_status validate(const sfbhip_params *p) {
   if (p->bessel < 0 || p->bessel > 2) return fail(SFBHIP_INVALID_ARGUMENT, "unknown bessel mode");
   if (p->zeros != SFBHIP_ZEROS_REFERENCE && p->zeros != SFBHIP_ZEROS_CONVERGED)
     return fail(SFBHIP_INVALID_ARGUMENT, "unknown zeros mode");
+  if (p->mixed_operands != SFBHIP_OPERANDS_FP32 && p->mixed_operands != SFBHIP_OPERANDS_SPLIT_FP16)
+    return fail(SFBHIP_INVALID_ARGUMENT, "unknown mixed operand format");
   return SFBHIP_OK;
 }
 
@@ -330,12 +333,18 @@ void launch_f64(int nitems, const DirectItem *items, hipStream_t st, const Point
   sfb_direct_f64_kernel<QMAX><<<nitems, 256, 0, st>>>(pts, items, kln, nmax, lt, use_table, tab, partial, err);
 }
 
+// Also reads back the catalogue's largest radius into *rdata (one stream synchronisation).
 sfbhip_status prep_points(sfbhip_handle *h, const double *d_points, int64_t n, hipStream_t st, int *err,
-                          PointsSoA *pts) {
+                          PointsSoA *pts, double *rdata) {
   double *buf;
   CHK(get_buf(h, "soa", (size_t)6 * n, &buf));
   double *r = buf, *x = buf + n, *s = buf + 2 * n, *ph = buf + 3 * n, *cp = buf + 4 * n, *sp = buf + 5 * n;
-  prep_points_kernel<<<cdiv(n, 256), 256, 0, st>>>(d_points, n, r, x, s, ph, cp, sp, err);
+  unsigned long long *rbits = reinterpret_cast<unsigned long long *>(err + 2);
+  prep_points_kernel<<<cdiv(n, 256), 256, 0, st>>>(d_points, n, r, x, s, ph, cp, sp, err, rbits);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(h->h_err + 2, rbits, sizeof(double), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  std::memcpy(rdata, h->h_err + 2, sizeof(double));
   HIPCHK(hipGetLastError());
   pts->r = r;
   pts->x = x;
@@ -384,22 +393,25 @@ sfbhip_status finalize(sfbhip_handle *h, const sfbhip_params *p, const std::vect
   return SFBHIP_OK;
 }
 
-// Hermite tables of the mixed path: nodes i*h, i < nn_l = floor(k_{l,n_l-1} rmax / h) + 2.
-sfbhip_status ensure_hermite(sfbhip_handle *h, const sfbhip_params *p, hipStream_t st, HermiteTables *ht) {
+// Hermite tables of the mixed path: nodes i*h, i < nn_l = floor(k_{l,n_l-1} R / h) + 3,
+// R = max(rmax, largest radius of the catalogue), so that no argument leaves the table.
+sfbhip_status ensure_hermite(sfbhip_handle *h, const sfbhip_params *p, double rdata, hipStream_t st,
+                             HermiteTables *ht) {
   KlnCache &c = h->kc;
   const int L = p->lmax;
-  if (!c.herm_valid) {
+  const double R = std::max(p->rmax, rdata);
+  if (!c.herm_valid || c.herm_R < R) {
     c.herm_off.assign(std::max(L, 1), 0);
     c.herm_nn.assign(std::max(L, 1), 2);
     int32_t o = 0, mx = 2;
     for (int l = 0; l < L; ++l) {
       int32_t nn = 2;
       if (c.nl[l] > 0) {
-        const double X = c.kln[(size_t)l * p->nmax + c.nl[l] - 1] * p->rmax;
-        nn = (int32_t)std::floor(X / MX_H) + 2;
+        const double X = c.kln[(size_t)l * p->nmax + c.nl[l] - 1] * R;
+        nn = (int32_t)std::floor(X / MX_H) + 3;
       }
       if (nn > MX_TAB_MAX)
-        return fail(SFBHIP_NOT_SUPPORTED, "mixed path: k_max * rmax too large for the LDS Bessel table (max " +
+        return fail(SFBHIP_NOT_SUPPORTED, "mixed path: largest Bessel argument k r exceeds the LDS table (" +
                                               std::to_string(MX_TAB_MAX * MX_H) + "); use precision FP64");
       c.herm_off[l] = o;
       c.herm_nn[l] = nn;
@@ -420,6 +432,7 @@ sfbhip_status ensure_hermite(sfbhip_handle *h, const sfbhip_params *p, hipStream
     }
     HIPCHK(hipStreamSynchronize(st));
     c.herm_max = mx;
+    c.herm_R = R;
     c.herm_valid = true;
   }
   float2 *tab;
@@ -433,16 +446,24 @@ sfbhip_status ensure_hermite(sfbhip_handle *h, const sfbhip_params *p, hipStream
   return SFBHIP_OK;
 }
 
-template <int QMAX>
+template <int QMAX, bool F16X3>
 sfbhip_status launch_mixed(int nitems, const DirectItem *items, hipStream_t st, const PointsSoA &pts,
                            const double *kln, int nmax, const LegendreTables &lt, const HermiteTables &ht,
                            int lds_nodes, double *partial) {
   const size_t shm = sizeof(float2) * (size_t)lds_nodes;
-  HIPCHK(hipFuncSetAttribute(reinterpret_cast<const void *>(&sfb_direct_mixed_kernel<QMAX>),
+  HIPCHK(hipFuncSetAttribute(reinterpret_cast<const void *>(&sfb_direct_mixed_kernel<QMAX, F16X3>),
                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
-  sfb_direct_mixed_kernel<QMAX><<<nitems, 256, shm, st>>>(pts, items, kln, nmax, lt, ht, partial);
+  sfb_direct_mixed_kernel<QMAX, F16X3><<<nitems, 256, shm, st>>>(pts, items, kln, nmax, lt, ht, partial);
   HIPCHK(hipGetLastError());
   return SFBHIP_OK;
+}
+
+template <int QMAX>
+sfbhip_status launch_mixed_ops(bool f16, int nitems, const DirectItem *items, hipStream_t st, const PointsSoA &pts,
+                               const double *kln, int nmax, const LegendreTables &lt, const HermiteTables &ht,
+                               int lds_nodes, double *partial) {
+  if (f16) return launch_mixed<QMAX, true>(nitems, items, st, pts, kln, nmax, lt, ht, lds_nodes, partial);
+  return launch_mixed<QMAX, false>(nitems, items, st, pts, kln, nmax, lt, ht, lds_nodes, partial);
 }
 
 // Launch every class of a plan; items[] on the device holds the classes back to back from `*off`.
@@ -468,7 +489,7 @@ sfbhip_status launch_plan_f64(const DirectPlan &pl, DirectItem *ditems, size_t *
 
 sfbhip_status launch_plan_mixed(sfbhip_handle *h, const DirectPlan &pl, DirectItem *ditems, size_t *off,
                                 hipStream_t st, const PointsSoA &pts, const double *dk, int nmax,
-                                const LegendreTables &lt, const HermiteTables &ht, double *partial) {
+                                const LegendreTables &lt, const HermiteTables &ht, bool f16, double *partial) {
   for (int ci = 0; ci < 4; ++ci) {
     const int n = (int)pl.items[ci].size();
     if (!n) continue;
@@ -476,10 +497,10 @@ sfbhip_status launch_plan_mixed(sfbhip_handle *h, const DirectPlan &pl, DirectIt
     for (const DirectItem &it : pl.items[ci]) nodes = std::max(nodes, h->kc.herm_nn[it.l]);
     const DirectItem *di = ditems + *off;
     switch (ci) {
-      case 0: CHK(launch_mixed<1>(n, di, st, pts, dk, nmax, lt, ht, nodes, partial)); break;
-      case 1: CHK(launch_mixed<2>(n, di, st, pts, dk, nmax, lt, ht, nodes, partial)); break;
-      case 2: CHK(launch_mixed<4>(n, di, st, pts, dk, nmax, lt, ht, nodes, partial)); break;
-      default: CHK(launch_mixed<8>(n, di, st, pts, dk, nmax, lt, ht, nodes, partial)); break;
+      case 0: CHK(launch_mixed_ops<1>(f16, n, di, st, pts, dk, nmax, lt, ht, nodes, partial)); break;
+      case 1: CHK(launch_mixed_ops<2>(f16, n, di, st, pts, dk, nmax, lt, ht, nodes, partial)); break;
+      case 2: CHK(launch_mixed_ops<4>(f16, n, di, st, pts, dk, nmax, lt, ht, nodes, partial)); break;
+      default: CHK(launch_mixed_ops<8>(f16, n, di, st, pts, dk, nmax, lt, ht, nodes, partial)); break;
     }
     *off += n;
   }
@@ -504,6 +525,7 @@ sfbhip_status sfbhip_default_params(sfbhip_params *p) {
   p->alias_compat = 1;
   p->verbose = 0;
   p->mixed_fp64_below = 8;
+  p->mixed_operands = SFBHIP_OPERANDS_SPLIT_FP16;
   return SFBHIP_OK;
 }
 
@@ -583,7 +605,8 @@ sfbhip_status sfbhip_direct(sfbhip_handle *h, const double *d_points, int64_t n_
   LegendreTables lt;
   CHK(ensure_legendre(h, std::max(p->lmax, 1), st, &lt));
   PointsSoA pts;
-  CHK(prep_points(h, d_points, n_points, st, derr, &pts));
+  double rdata = 0.0;
+  CHK(prep_points(h, d_points, n_points, st, derr, &pts, &rdata));
   const double norm = norm_factor(p, (double)n_points);
   double *dk;
   CHK(get_buf(h, "kln", (size_t)std::max(p->lmax, 1) * p->nmax, &dk));
@@ -591,13 +614,15 @@ sfbhip_status sfbhip_direct(sfbhip_handle *h, const double *d_points, int64_t n_
   // Multipoles l < lsplit run in the FP64 kernel; the rest in the mixed kernel
   // when precision is MIXED.
   const bool mixed = p->precision == SFBHIP_PREC_MIXED;
+  if (mixed && p->lmax > MX_LMAX)
+    return fail(SFBHIP_NOT_SUPPORTED, "mixed path supports lmax <= " + std::to_string(MX_LMAX) + "; use precision FP64");
   const int lsplit = mixed ? std::min(p->lmax, std::max(0, p->mixed_fp64_below)) : p->lmax;
   const bool use_table =
       !mixed && (p->bessel == SFBHIP_BESSEL_REFERENCE || (p->bessel == SFBHIP_BESSEL_AUTO && n_points > 1000));
   BesselTableRef tab{nullptr, nullptr, nullptr, 0};
   if (use_table) CHK(ensure_table(h, p, st, &tab));
   HermiteTables ht{nullptr, nullptr, nullptr};
-  if (mixed && lsplit < p->lmax) CHK(ensure_hermite(h, p, st, &ht));
+  if (mixed && lsplit < p->lmax) CHK(ensure_hermite(h, p, rdata, st, &ht));
   DirectPlan pl64, plmx;
   plan_direct(h->kc.nl, 0, lsplit, n_points, F64_NT, F64_TP, F64_MC, 16, 4096, &pl64);
   plmx.partial_size = pl64.partial_size;
@@ -617,7 +642,9 @@ sfbhip_status sfbhip_direct(sfbhip_handle *h, const double *d_points, int64_t n_
   HIPCHK(hipEventRecord(h->ev0, st));
   size_t off = 0;
   CHK(launch_plan_f64(pl64, ditems, &off, st, pts, dk, p->nmax, lt, use_table, tab, partial, derr));
-  if (mixed) CHK(launch_plan_mixed(h, plmx, ditems, &off, st, pts, dk, p->nmax, lt, ht, partial));
+  if (mixed)
+    CHK(launch_plan_mixed(h, plmx, ditems, &off, st, pts, dk, p->nmax, lt, ht,
+                          p->mixed_operands == SFBHIP_OPERANDS_SPLIT_FP16, partial));
   HIPCHK(hipEventRecord(h->ev1, st));
   CHK(finalize(h, p, groups, partial, norm, d_k_ln, d_c_ln, d_f_lmn, st));
   int e = 0;

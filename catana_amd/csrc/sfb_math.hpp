@@ -86,9 +86,14 @@ SFB_HD double sbessel_jl(int l, double x) {
   if (l == 0) return sj0(x);
   if (x * x < 10.0 * (l + 0.5) / 2.718281828459045235) return sjl_series(l, x);
   if (x >= (double)l) {
-    double jm = sj0(x), j = sj1(x);
+    // upward recurrence j_{k+1} = (2k+1)/x j_k - j_{k-1}, with one reciprocal
+    const double inv = 1.0 / x;
+    double S, C;
+    sincos(x, &S, &C);
+    double jm = S * inv;             // j_0
+    double j = (jm - C) * inv;       // j_1 (x >= 1 here, no cancellation issue)
     for (int k = 1; k < l; ++k) {
-      const double jp = (2.0 * k + 1.0) / x * j - jm;
+      const double jp = fma((2.0 * k + 1.0) * inv, j, -jm);
       jm = j;
       j = jp;
     }

@@ -9,13 +9,26 @@
 namespace sfb {
 
 // AoS (r, theta, phi) -> SoA (r, cos t, sin t, phi, cos phi, sin phi); flags
-// r < 0 and non-finite input.
+// r < 0 and non-finite input; *rmax_bits receives the largest r (the bit
+// pattern of a non-negative double orders like the value).
 __global__ void prep_points_kernel(const double *__restrict__ aos, int64_t n, double *__restrict__ r,
                                    double *__restrict__ x, double *__restrict__ s, double *__restrict__ phi,
-                                   double *__restrict__ cphi, double *__restrict__ sphi, int *__restrict__ err) {
+                                   double *__restrict__ cphi, double *__restrict__ sphi, int *__restrict__ err,
+                                   unsigned long long *__restrict__ rmax_bits) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  double rr = 0.0;
+  if (i < n) rr = aos[3 * i];
+  // block maximum, then one atomic per block
+  __shared__ double red[256];
+  red[threadIdx.x] = (rr >= 0.0 && rr <= 1.7e308) ? rr : 0.0;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + w]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) atomicMax(rmax_bits, (unsigned long long)__double_as_longlong(red[0]));
   if (i >= n) return;
-  const double rr = aos[3 * i], th = aos[3 * i + 1], ph = aos[3 * i + 2];
+  const double th = aos[3 * i + 1], ph = aos[3 * i + 2];
   double st, ctt, sp, cp;
   sincos(th, &st, &ctt);
   sincos(ph, &sp, &cp);

@@ -76,6 +76,15 @@ struct YlmDown {
     sm = sn;
   }
 
+  // branch-free renormalisation (same effect as rescale())
+  __device__ __forceinline__ void rescale_nb() {
+    const bool big = fabs(v) > 0x1p+600;
+    const double f = big ? 0x1p-600 : 1.0;
+    v *= f;
+    vp *= f;
+    E += big ? 600 : 0;
+  }
+
   __device__ __forceinline__ void rescale() {
     if (fabs(v) > 0x1p+600) {
       v = ldexp(v, -600);

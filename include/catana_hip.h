@@ -39,8 +39,13 @@ typedef enum sfbhip_status {
 /* arithmetic of the point contraction */
 enum {
   SFBHIP_PREC_FP64 = 0,   /* FP64 everywhere: the drop-in and the GPU oracle */
-  SFBHIP_PREC_MIXED = 1   /* FP32 MFMA on FP32 operands (Hermite Bessel table with FP64 argument
-                             reduction, FP64 Legendre rows), FP64 tile sums; low l in FP64 */
+  SFBHIP_PREC_MIXED = 1   /* FP32 Bessel table (FP64 argument reduction), FP64 Legendre rows,
+                             matrix-core contraction, FP64 tile sums; low l in FP64 */
+};
+/* matrix-core operand format of the mixed path */
+enum {
+  SFBHIP_OPERANDS_FP32 = 0,      /* FP32 operands, v_mfma_f32_16x16x4f32 */
+  SFBHIP_OPERANDS_SPLIT_FP16 = 1 /* operands split into two FP16 halves, three v_mfma_f32_16x16x32_f16 */
 };
 /* how j_l(k r) is evaluated in the FP64 path */
 enum {
@@ -67,12 +72,14 @@ typedef struct sfbhip_params {
   int32_t verbose;       /* print a one-line summary per call to stdout */
   int32_t mixed_fp64_below; /* mixed path: multipoles l < this run in FP64 (C_l of low l can be
                                dominated by cancellation); default 8 */
-  int32_t reserved[2];
+  int32_t mixed_operands;   /* mixed path: SFBHIP_OPERANDS_* */
+  int32_t reserved[1];
 } sfbhip_params;
 
 typedef struct sfbhip_handle sfbhip_handle;
 
-/* Fills defaults: precision FP64, bessel AUTO, zeros REFERENCE, alias_compat 1. */
+/* Fills defaults: precision FP64, bessel AUTO, zeros REFERENCE, alias_compat 1,
+ * mixed_fp64_below 8, mixed_operands SPLIT_FP16. */
 sfbhip_status sfbhip_default_params(sfbhip_params *p);
 
 /* Library state bound to one device: workspace and cached k_ln tables.
