@@ -45,6 +45,10 @@ def make_points(name):
     raise ValueError(name)
 
 
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
 def cpu_baseline(points, cfg, nl, sample, threads):
     """The oracle (C/OpenMP restatement of sfb_raw.cpp:13-108 with the public
     wrapper's interpolated = N > 1000, sfb_decomposition.cpp:17) timed on a
@@ -88,7 +92,9 @@ def main():
     from catana_amd import capi
 
     cfg = WORKLOADS[args.workload]
+    log(f"generating workload {args.workload}")
     pts = make_points(args.workload)
+    log(f"{pts.shape[0]} points")
     N = pts.shape[0]
     dev = torch.device("cuda:0")
     d_pts = torch.from_numpy(np.ascontiguousarray(pts)).to(dev)
@@ -103,6 +109,7 @@ def main():
     flops = 4.0 * N * int(((l + 1) * nl).sum())
 
     stream = torch.cuda.current_stream(dev)
+    log("warmup")
     for _ in range(args.warmup):
         capi.sfb_direct_torch(h, d_pts, cfg["lmax"], cfg["nmax"], cfg["rmax"], **kw)
     torch.cuda.synchronize()
@@ -116,6 +123,7 @@ def main():
     e1.record(stream)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
+    log(f"timed {args.steps} steps: {wall:.3f} s")
     ms = e0.elapsed_time(e1) / args.steps
     kms = float(np.mean(kern_ms))
 
@@ -141,6 +149,7 @@ def main():
                      "kernel_ms": kms, "algorithmic_flops": flops},
     }
     if prec != capi.PREC_FP64 and not args.no_accuracy:
+        log("FP64 reference run for the accuracy figures")
         k64, c64, _ = capi.sfb_direct_torch(h, d_pts, cfg["lmax"], cfg["nmax"], cfg["rmax"],
                                             k_max=cfg["k_max"], precision=capi.PREC_FP64,
                                             bessel=capi.BESSEL_EXACT)
@@ -149,6 +158,7 @@ def main():
         rel = np.abs(a[mask] / b[mask] - 1)
         out["accuracy"] = {"max_rel_cln_vs_fp64": float(rel.max()), "median_rel_cln_vs_fp64": float(np.median(rel))}
     if not args.no_cpu_baseline:
+        log("CPU baseline")
         threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
         out["cpu_baseline"] = cpu_baseline(pts, cfg, nl, args.cpu_sample, threads)
     h.close()

@@ -164,3 +164,39 @@ def sfb_direct_torch(handle, points, lmax, nmax, rmax, store_flmn=False, stream=
                              ctypes.c_void_p(stream) if stream else None)
     check(rc, "sfbhip_direct")
     return k, c, f
+
+
+def pixelize_torch(handle, points, nside, stream=None):
+    """HEALPix RING binning of device points: returns (offsets int64 [12 nside^2 + 1],
+    radii float64 [N]) on the same device (PixelizedPointContainer(nside, pc))."""
+    torch = _torch()
+    pts = points.contiguous()
+    npix = 12 * nside * nside
+    off = torch.empty(npix + 1, dtype=torch.int64, device=pts.device)
+    rad = torch.empty(max(pts.shape[0], 1), dtype=torch.float64, device=pts.device)
+    if stream is None:
+        stream = torch.cuda.current_stream(pts.device).cuda_stream
+    rc = lib().sfbhip_pixelize(handle.ptr, ctypes.c_void_p(pts.data_ptr()), ctypes.c_int64(pts.shape[0]),
+                               int(nside), ctypes.c_void_p(off.data_ptr()), ctypes.c_void_p(rad.data_ptr()),
+                               ctypes.c_void_p(stream) if stream else None)
+    check(rc, "sfbhip_pixelize")
+    return off, rad[:pts.shape[0]]
+
+
+def sfb_healpix_torch(handle, offsets, radii, nside, lmax, nmax, rmax, store_flmn=False, stream=None, **kw):
+    """Pixelised SFB (catana::_sfb_reverse_fft semantics) on a device CSR."""
+    torch = _torch()
+    p = make_params(lmax, nmax, rmax, **kw)
+    dev = offsets.device
+    k = torch.empty((lmax, nmax), dtype=torch.float64, device=dev)
+    c = torch.empty((lmax, nmax), dtype=torch.float64, device=dev)
+    f = torch.empty(2 * flmn_count(lmax, nmax), dtype=torch.float64, device=dev) if store_flmn else None
+    if stream is None:
+        stream = torch.cuda.current_stream(dev).cuda_stream
+    rc = lib().sfbhip_healpix(handle.ptr, ctypes.c_void_p(offsets.data_ptr()), ctypes.c_void_p(radii.data_ptr()),
+                              int(nside), ctypes.byref(p), ctypes.c_void_p(k.data_ptr()),
+                              ctypes.c_void_p(c.data_ptr()),
+                              ctypes.c_void_p(f.data_ptr()) if f is not None else None,
+                              ctypes.c_void_p(stream) if stream else None)
+    check(rc, "sfbhip_healpix")
+    return k, c, f

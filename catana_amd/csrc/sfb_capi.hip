@@ -8,6 +8,7 @@
 #include "catana_hip.h"
 
 #include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
 
 #include <algorithm>
 #include <cmath>
@@ -20,6 +21,7 @@
 #include "sfb_common.hpp"
 #include "sfb_direct_f64.hpp"
 #include "sfb_direct_mixed.hpp"
+#include "sfb_healpix.hpp"
 #include "sfb_math.hpp"
 #include "sfb_support.hpp"
 
@@ -73,6 +75,7 @@ struct sfbhip_handle {
   float last_ms = 0.f;
   KlnCache kc;
   int leg_lmax = -1;
+  int pix_nside = -1;  // pixel tables on the device are for this nside
 };
 
 namespace {
@@ -327,10 +330,15 @@ void plan_direct(const std::vector<int32_t> &nl, int lmin, int lmax, int64_t npt
 }
 
 template <int QMAX>
-void launch_f64(int nitems, const DirectItem *items, hipStream_t st, const PointsSoA &pts, const double *kln,
-                int nmax, const LegendreTables &lt, int use_table, const BesselTableRef &tab, double *partial,
-                int *err) {
-  sfb_direct_f64_kernel<QMAX><<<nitems, 256, 0, st>>>(pts, items, kln, nmax, lt, use_table, tab, partial, err);
+void launch_f64(int nitems, const DirectItem *items, hipStream_t st, const PointsSoA &pts, const PixelsSoA *pix,
+                const double *kln, int nmax, const LegendreTables &lt, int use_table, const BesselTableRef &tab,
+                double *partial, int *err) {
+  if (pix)
+    sfb_direct_f64_kernel<QMAX, true><<<nitems, 256, 0, st>>>(pts, *pix, items, kln, nmax, lt, use_table, tab,
+                                                                partial, err);
+  else
+    sfb_direct_f64_kernel<QMAX, false><<<nitems, 256, 0, st>>>(pts, PixelsSoA{}, items, kln, nmax, lt, use_table,
+                                                                 tab, partial, err);
 }
 
 // Also reads back the catalogue's largest radius into *rdata (one stream synchronisation).
@@ -468,18 +476,19 @@ sfbhip_status launch_mixed_ops(bool f16, int nitems, const DirectItem *items, hi
 
 // Launch every class of a plan; items[] on the device holds the classes back to back from `*off`.
 sfbhip_status launch_plan_f64(const DirectPlan &pl, DirectItem *ditems, size_t *off, hipStream_t st,
-                              const PointsSoA &pts, const double *dk, int nmax, const LegendreTables &lt,
-                              int use_table, const BesselTableRef &tab, double *partial, int *derr) {
+                              const PointsSoA &pts, const PixelsSoA *pix, const double *dk, int nmax,
+                              const LegendreTables &lt, int use_table, const BesselTableRef &tab, double *partial,
+                              int *derr) {
   for (int ci = 0; ci < 5; ++ci) {
     const int n = (int)pl.items[ci].size();
     if (!n) continue;
     const DirectItem *di = ditems + *off;
     switch (ci) {
-      case 0: launch_f64<1>(n, di, st, pts, dk, nmax, lt, use_table, tab, partial, derr); break;
-      case 1: launch_f64<2>(n, di, st, pts, dk, nmax, lt, use_table, tab, partial, derr); break;
-      case 2: launch_f64<4>(n, di, st, pts, dk, nmax, lt, use_table, tab, partial, derr); break;
-      case 3: launch_f64<8>(n, di, st, pts, dk, nmax, lt, use_table, tab, partial, derr); break;
-      default: launch_f64<16>(n, di, st, pts, dk, nmax, lt, use_table, tab, partial, derr); break;
+      case 0: launch_f64<1>(n, di, st, pts, pix, dk, nmax, lt, use_table, tab, partial, derr); break;
+      case 1: launch_f64<2>(n, di, st, pts, pix, dk, nmax, lt, use_table, tab, partial, derr); break;
+      case 2: launch_f64<4>(n, di, st, pts, pix, dk, nmax, lt, use_table, tab, partial, derr); break;
+      case 3: launch_f64<8>(n, di, st, pts, pix, dk, nmax, lt, use_table, tab, partial, derr); break;
+      default: launch_f64<16>(n, di, st, pts, pix, dk, nmax, lt, use_table, tab, partial, derr); break;
     }
     HIPCHK(hipGetLastError());
     *off += n;
@@ -641,7 +650,7 @@ sfbhip_status sfbhip_direct(sfbhip_handle *h, const double *d_points, int64_t n_
   h->last_ms = 0.f;
   HIPCHK(hipEventRecord(h->ev0, st));
   size_t off = 0;
-  CHK(launch_plan_f64(pl64, ditems, &off, st, pts, dk, p->nmax, lt, use_table, tab, partial, derr));
+  CHK(launch_plan_f64(pl64, ditems, &off, st, pts, nullptr, dk, p->nmax, lt, use_table, tab, partial, derr));
   if (mixed)
     CHK(launch_plan_mixed(h, plmx, ditems, &off, st, pts, dk, p->nmax, lt, ht,
                           p->mixed_operands == SFBHIP_OPERANDS_SPLIT_FP16, partial));
@@ -657,13 +666,127 @@ sfbhip_status sfbhip_direct(sfbhip_handle *h, const double *d_points, int64_t n_
 
 sfbhip_status sfbhip_pixelize(sfbhip_handle *h, const double *d_points, int64_t n_points, int32_t nside,
                               int64_t *d_offsets, double *d_radii, void *stream) {
-  return fail(SFBHIP_NOT_SUPPORTED, "HEALPix path not built yet");
+  g_last_error.clear();
+  if (!h) return fail(SFBHIP_INVALID_ARGUMENT, "handle is NULL");
+  if (nside < 1 || (nside & (nside - 1)) || nside > 8192)
+    return fail(SFBHIP_INVALID_ARGUMENT, "argument (nside) must be a power of 2");  // PixelizedPointContainer.cpp:13-15
+  if (n_points < 0 || (n_points > 0 && (!d_points || !d_radii)) || !d_offsets)
+    return fail(SFBHIP_INVALID_ARGUMENT, "NULL buffer");
+  if (n_points > 0x7fffffffLL) return fail(SFBHIP_NOT_SUPPORTED, "more than 2^31 points");
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  HIPCHK(hipSetDevice(h->device));
+  const int64_t npix = 12LL * nside * nside;
+  int *derr;
+  CHK(zero_err(h, st, &derr));
+  const int64_t n = std::max<int64_t>(n_points, 1);
+  uint32_t *k0, *k1;
+  double *v0;
+  CHK(get_buf(h, "pix_k0", (size_t)n, &k0));
+  CHK(get_buf(h, "pix_k1", (size_t)n, &k1));
+  CHK(get_buf(h, "pix_v0", (size_t)n, &v0));
+  if (n_points > 0) {
+    ang2pix_kernel<<<cdiv(n_points, 256), 256, 0, st>>>(d_points, n_points, nside, k0, v0, derr);
+    HIPCHK(hipGetLastError());
+    int end_bit = 1;
+    while ((1LL << end_bit) < npix) ++end_bit;
+    size_t tmp_bytes = 0;
+    HIPCHK(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_bytes, k0, k1, v0, d_radii, (int)n_points, 0, end_bit, st));
+    char *tmp;
+    CHK(get_buf(h, "pix_tmp", tmp_bytes, &tmp));
+    HIPCHK(hipcub::DeviceRadixSort::SortPairs(tmp, tmp_bytes, k0, k1, v0, d_radii, (int)n_points, 0, end_bit, st));
+  }
+  csr_offsets_kernel<<<cdiv(npix + 1, 256), 256, 0, st>>>(k1, n_points, npix, d_offsets);
+  HIPCHK(hipGetLastError());
+  int e = 0;
+  CHK(read_err(h, st, derr, &e));
+  if (e & ERR_BADPOINT) return fail(SFBHIP_DOMAIN_ERROR, "point with r < 0 or a non-finite coordinate");
+  return SFBHIP_OK;
 }
 
 sfbhip_status sfbhip_healpix(sfbhip_handle *h, const int64_t *d_offsets, const double *d_radii, int32_t nside,
                              const sfbhip_params *p, double *d_k_ln, double *d_c_ln, double *d_f_lmn,
                              void *stream) {
-  return fail(SFBHIP_NOT_SUPPORTED, "HEALPix path not built yet");
+  g_last_error.clear();
+  if (!h) return fail(SFBHIP_INVALID_ARGUMENT, "handle is NULL");
+  CHK(validate(p));
+  if (nside < 1 || (nside & (nside - 1)) || nside > 8192)
+    return fail(SFBHIP_INVALID_ARGUMENT, "argument (nside) must be a power of 2");
+  if (!d_offsets || !d_k_ln || !d_c_ln) return fail(SFBHIP_INVALID_ARGUMENT, "NULL buffer");
+  if (p->precision != SFBHIP_PREC_FP64)
+    return fail(SFBHIP_NOT_SUPPORTED, "the pixelised path runs in FP64 (precision must be SFBHIP_PREC_FP64)");
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  HIPCHK(hipSetDevice(h->device));
+  const int64_t npix = 12LL * nside * nside;
+  // total number of points (the normalisation's N, sfb_reverse_fft.cpp:49-50)
+  int64_t npts = 0;
+  HIPCHK(hipMemcpyAsync(h->h_err, d_offsets + npix, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  std::memcpy(&npts, h->h_err, sizeof(int64_t));
+  if (npts <= 0 || !d_radii)
+    return fail(SFBHIP_INVALID_ARGUMENT, "empty catalogue: the normalisation divides by the number of points");
+  if (p->verbose) {
+    std::printf("SFB decomposition (reverse method): lmax=%d nmax=%d NSide=%d\n", p->lmax, p->nmax, nside);
+    std::printf("Catalog consists of %lld points.\n", (long long)npts);
+  }
+  int *derr;
+  CHK(zero_err(h, st, &derr));
+  CHK(ensure_kln(h, p, st));
+  LegendreTables lt;
+  CHK(ensure_legendre(h, std::max(p->lmax, 1), st, &lt));
+  // pixel tables
+  double *pb;
+  int32_t *pnp;
+  CHK(get_buf(h, "pix_tab", (size_t)8 * npix, &pb));
+  CHK(get_buf(h, "pix_np", (size_t)npix, &pnp));
+  if (h->pix_nside != nside) {
+    pixel_setup_kernel<<<cdiv(npix, 256), 256, 0, st>>>(nside, pb, pb + npix, pb + 2 * npix, pb + 3 * npix,
+                                                          pb + 4 * npix, pb + 5 * npix, pb + 6 * npix, pb + 7 * npix,
+                                                          pnp);
+    HIPCHK(hipGetLastError());
+    h->pix_nside = nside;
+  }
+  PixelsSoA pix;
+  pix.x = pb;
+  pix.s = pb + npix;
+  pix.phi = pb + 2 * npix;
+  pix.cphi = pb + 3 * npix;
+  pix.sphi = pb + 4 * npix;
+  pix.phi2 = pb + 5 * npix;
+  pix.cphi2 = pb + 6 * npix;
+  pix.sphi2 = pb + 7 * npix;
+  pix.np = pnp;
+  pix.off = d_offsets;
+  pix.radii = d_radii;
+  pix.n = npix;
+  pix.alias_compat = p->alias_compat;
+  const double norm = norm_factor(p, (double)npts);
+  double *dk;
+  CHK(get_buf(h, "kln", (size_t)std::max(p->lmax, 1) * p->nmax, &dk));
+  const bool use_table = p->bessel == SFBHIP_BESSEL_REFERENCE || (p->bessel == SFBHIP_BESSEL_AUTO && npts > 1000);
+  BesselTableRef tab{nullptr, nullptr, nullptr, 0};
+  if (use_table) CHK(ensure_table(h, p, st, &tab));
+  DirectPlan pl;
+  plan_direct(h->kc.nl, 0, p->lmax, npix, F64_NT, F64_TP, F64_MC, 16, 4096, &pl);
+  double *partial;
+  CHK(get_buf(h, "partial", (size_t)std::max<int64_t>(pl.partial_size, 1), &partial));
+  std::vector<DirectItem> all;
+  for (auto &v : pl.items) all.insert(all.end(), v.begin(), v.end());
+  DirectItem *ditems;
+  CHK(get_buf(h, "items", std::max<size_t>(all.size(), 1), &ditems));
+  if (!all.empty())
+    HIPCHK(hipMemcpyAsync(ditems, all.data(), sizeof(DirectItem) * all.size(), hipMemcpyHostToDevice, st));
+  h->last_ms = 0.f;
+  HIPCHK(hipEventRecord(h->ev0, st));
+  size_t off = 0;
+  PointsSoA none{};
+  CHK(launch_plan_f64(pl, ditems, &off, st, none, &pix, dk, p->nmax, lt, use_table, tab, partial, derr));
+  HIPCHK(hipEventRecord(h->ev1, st));
+  CHK(finalize(h, p, pl.groups, partial, norm, d_k_ln, d_c_ln, d_f_lmn, st));
+  int e = 0;
+  CHK(read_err(h, st, derr, &e));
+  if (!pl.groups.empty()) HIPCHK(hipEventElapsedTime(&h->last_ms, h->ev0, h->ev1));
+  if (e & ERR_DOMAIN) return fail(SFBHIP_DOMAIN_ERROR, "argument (z) outside of defined range.");
+  return SFBHIP_OK;
 }
 
 }  // extern "C"

@@ -22,6 +22,19 @@ struct PointsSoA {
   int64_t n;
 };
 
+// Per-pixel data of a HEALPix RING map (the "points" of the pixelised path):
+// ring cos(theta) and sin(theta), pixel-centre azimuth phi = phi0 + 2 pi j/np
+// and its cos/sin, the mirrored azimuth 2 phi0 - phi used where the reference
+// reads an unconjugated folded FFT bin, the ring length np and the CSR of radii.
+struct PixelsSoA {
+  const double *x, *s, *phi, *cphi, *sphi, *phi2, *cphi2, *sphi2;
+  const int32_t *np;
+  const int64_t *off;
+  const double *radii;
+  int64_t n;  // pixels
+  int32_t alias_compat;
+};
+
 // One work item of the direct kernels: multipole l, rows [n0, n0+nrows) of
 // the mode index n, points [p0, p1), and where its partial sums go.
 struct DirectItem {

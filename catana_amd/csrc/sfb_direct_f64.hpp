@@ -1,21 +1,26 @@
-// sfb_direct_f64.hpp -- FP64 direct point-mode SFB kernel (the drop-in path
-// and the GPU oracle).
+// sfb_direct_f64.hpp -- FP64 SFB kernel: the drop-in path and the GPU oracle,
+// for points (direct sum) and for HEALPix pixels (pixelised path).
 //
 // Per l the reference computes f_lmn = sum_i k_ln j_l(k_ln r_i) Y*_lm(t_i, p_i)
 // (sfb_raw.cpp:61-80), i.e. a product of the (n x points) Bessel matrix with
-// the (points x 2(l+1)) matrix of real/imaginary Y* columns.  A work item is
-// (l, 32 rows of n, a range of points).  Per 128-point tile the workgroup
-//   1. evaluates the 32 x 128 Bessel tile into LDS (all 256 threads),
+// the (points x 2(l+1)) matrix of real/imaginary Y* columns.  The pixelised
+// path (sfb_reverse_fft.cpp:174-209, sfb_reverse.cpp:63-88) is the same
+// product over pixels with F_l(p, n) = sum_{r in p} j_l(k_ln r) in place of the
+// Bessel value and the pixel's Y* (YlmDownPix) in place of the point's.
+//
+// A work item is (l, 32 rows of n, a range of points or pixels).  Per 128-wide
+// tile the workgroup
+//   1. evaluates the 32 x 128 Bessel (or pixel-sum) tile into LDS (all 256 threads),
 //   2. walks m from l down to 0 in chunks of 16 (32 columns): threads 0..127
 //      advance their point's Legendre/azimuth recurrence and write the chunk
 //      into LDS, then the four waves contract it on the FP64 matrix cores
 //      (v_mfma_f64_16x16x4f64), each wave owning one 16 x 16 output tile of
 //      the 32 x 32 chunk block.
-// Accumulators stay in registers for the whole point range (QMAX chunks of
-// double4 per lane), and the item's 32 x 2(l+1) block is written once.  An
-// item covers m chunks [qlo, qhi); for l >= 16*QMAX the columns are split over
-// several items, each of which first advances the recurrence from m = l down
-// to its own top chunk.
+// Accumulators stay in registers for the whole range (QMAX chunks of double4
+// per lane), and the item's 32 x 2(l+1) block is written once.  An item covers
+// m chunks [qlo, qhi) counted from m = 0; for l >= 16*QMAX the columns are
+// split over several items, each of which first advances the recurrence from
+// m = l down to its own top chunk.
 #pragma once
 
 #include "sfb_common.hpp"
@@ -24,9 +29,9 @@
 
 namespace sfb {
 
-constexpr int F64_NT = 32;    // rows (n) per item
-constexpr int F64_TP = 128;   // points per tile
-constexpr int F64_MC = 16;    // m values per chunk (32 real columns)
+constexpr int F64_NT = 32;           // rows (n) per item
+constexpr int F64_TP = 128;          // points per tile
+constexpr int F64_MC = 16;           // m values per chunk (32 real columns)
 constexpr int F64_LDP = F64_TP + 1;  // LDS row pitch (doubles)
 
 struct BesselTableRef {
@@ -36,11 +41,25 @@ struct BesselTableRef {
   int32_t P;             // nodes per l
 };
 
-template <int QMAX>
+// j_l(z): exact (gsl_sf_bessel_jl semantics) or the reference's linear table
+// (SBesselInterpolator.cpp:20-25, domain error past z_max).
+__device__ __forceinline__ double bessel_f64(int l, double z, int use_table, const BesselTableRef &tab, int &err) {
+  if (!use_table) return sbessel_jl(l, z);
+  if (!(z < tab.zmax[l])) {
+    err |= ERR_DOMAIN;
+    return 0.0;
+  }
+  const double *y = tab.y + (size_t)l * tab.P;
+  const double ia = z * tab.dx_inv[l];
+  int i = (int)ia;
+  i = i > tab.P - 2 ? tab.P - 2 : i;
+  return y[i] + (y[i + 1] - y[i]) * (ia - i);
+}
+
+template <int QMAX, bool PIX>
 __global__ __launch_bounds__(256) void sfb_direct_f64_kernel(
-    PointsSoA pts, const DirectItem *__restrict__ items, const double *__restrict__ kln, int nmax,
-    LegendreTables leg, int use_table, BesselTableRef tab, double *__restrict__ partial,
-    int *__restrict__ err) {
+    PointsSoA pts, PixelsSoA pix, const DirectItem *__restrict__ items, const double *__restrict__ kln, int nmax,
+    LegendreTables leg, int use_table, BesselTableRef tab, double *__restrict__ partial, int *__restrict__ err) {
   __shared__ double Js[F64_NT * F64_LDP];
   __shared__ double Ys[2 * F64_MC * F64_LDP];
 
@@ -62,51 +81,61 @@ __global__ __launch_bounds__(256) void sfb_direct_f64_kernel(
 
   int myerr = 0;
   for (int64_t base = it.p0; base < it.p1; base += F64_TP) {
-    // ---- 1. Bessel tile: thread -> point (tid & 127), rows (tid >> 7) + 2j
+    // ---- 1. Bessel (or pixel-sum) tile: thread -> point tid & 127, rows (tid >> 7) + 2j
     {
       const int p = tid & (F64_TP - 1);
       const int64_t gp = base + p;
-      const bool ok = gp < it.p1;
-      const double r = ok ? pts.r[gp] : 0.0;
-      for (int j = 0; j < F64_NT / 2; ++j) {
-        const int row = (tid >> 7) + 2 * j;
-        double val = 0.0;
-        if (ok && row < it.nrows) {
-          const double z = kl[row] * r;
-          if (use_table) {
-            const double *y = tab.y + (size_t)l * tab.P;
-            if (!(z < tab.zmax[l])) {
-              myerr |= ERR_DOMAIN;
-            } else {
-              const double ia = z * tab.dx_inv[l];
-              int i = (int)ia;
-              i = i > tab.P - 2 ? tab.P - 2 : i;
-              val = y[i] + (y[i + 1] - y[i]) * (ia - i);
-            }
-          } else {
-            val = sbessel_jl(l, z);
+      double accj[F64_NT / 2];
+#pragma unroll
+      for (int j = 0; j < F64_NT / 2; ++j) accj[j] = 0.0;
+      if (gp < it.p1) {
+        int64_t q0 = gp, q1 = gp + 1;
+        if constexpr (PIX) {
+          q0 = pix.off[gp];
+          q1 = pix.off[gp + 1];
+        }
+        for (int64_t q = q0; q < q1; ++q) {
+          const double r = PIX ? pix.radii[q] : pts.r[q];
+#pragma unroll
+          for (int j = 0; j < F64_NT / 2; ++j) {
+            const int row = (tid >> 7) + 2 * j;
+            if (row < it.nrows) accj[j] += bessel_f64(l, kl[row] * r, use_table, tab, myerr);
           }
         }
-        Js[row * F64_LDP + p] = val;
       }
+#pragma unroll
+      for (int j = 0; j < F64_NT / 2; ++j) Js[((tid >> 7) + 2 * j) * F64_LDP + p] = accj[j];
     }
     // ---- 2. Legendre/azimuth rows, chunk by chunk from high m to low m
     YlmDown g;
+    YlmDownPix gx;
     const bool ygen = tid < F64_TP;
     const int64_t gp = base + tid;
-    const bool yok = ygen && gp < it.p1;
     if (ygen) {
-      if (yok) {
-        g.init(l, pts.x[gp], pts.s[gp], pts.phi[gp], pts.cphi[gp], pts.sphi[gp], sect);
-        // advance to the top m of the first chunk this item handles
-        for (int m = l; m >= F64_MC * it.qhi; --m) {
-          g.step(coef[m]);
-          g.rescale();
+      if (gp < it.p1) {
+        if constexpr (PIX) {
+          gx.init(l, pix.x[gp], pix.s[gp], pix.phi[gp], pix.cphi[gp], pix.sphi[gp], pix.phi2[gp], pix.cphi2[gp],
+                  pix.sphi2[gp], pix.np[gp], pix.alias_compat != 0, sect);
+          for (int m = l; m >= F64_MC * it.qhi; --m) {
+            gx.step(coef[m]);
+            gx.rescale();
+          }
+        } else {
+          g.init(l, pts.x[gp], pts.s[gp], pts.phi[gp], pts.cphi[gp], pts.sphi[gp], sect);
+          for (int m = l; m >= F64_MC * it.qhi; --m) {
+            g.step(coef[m]);
+            g.rescale();
+          }
         }
       } else {
         g.v = g.vp = 0.0;
         g.E = 0;
         g.cm = g.sm = g.cot = g.cph = g.sph = 0.0;
+        gx.y = g;
+        gx.cm2 = gx.sm2 = gx.cph2 = gx.sph2 = 0.0;
+        gx.np = 1;
+        gx.smod = 0;
+        gx.quirk = false;
       }
     }
 #pragma unroll
@@ -120,10 +149,18 @@ __global__ __launch_bounds__(256) void sfb_direct_f64_kernel(
             const int m = mlo + mm;
             double re = 0.0, im = 0.0;
             if (m <= l) {
-              g.value(re, im);
-              if (m > 0) {
-                g.step(coef[m]);
-                g.rescale();
+              if constexpr (PIX) {
+                gx.value(m, re, im);
+                if (m > 0) {
+                  gx.step(coef[m]);
+                  gx.rescale();
+                }
+              } else {
+                g.value(re, im);
+                if (m > 0) {
+                  g.step(coef[m]);
+                  g.rescale();
+                }
               }
             }
             Ys[(2 * mm) * F64_LDP + tid] = re;

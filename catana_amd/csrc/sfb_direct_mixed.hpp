@@ -98,10 +98,10 @@ __device__ __forceinline__ float jl_hermite(const float2 *T, int nn, double t) {
 template <bool F16X3, int LDP>
 __device__ __forceinline__ void emit_y(const YlmDown &y, float *Yf, _Float16 *Yhi, _Float16 *Ylo, int mm, int tid) {
   if constexpr (F16X3) {
-    // Y* * 2^12 in FP32 (ample for a split with 2^-22 residual), then two FP16 halves
-    const float P = (float)ldexp(y.v, y.E + 12);
+    // Y* * 2^12 formed in FP64 and rounded once to FP32, then split into two FP16 halves
+    const double P = ldexp(y.v, y.E + 12);
     h2 hi, lo;
-    split_pair(P * (float)y.cm, -P * (float)y.sm, hi, lo);
+    split_pair((float)(P * y.cm), (float)(-P * y.sm), hi, lo);
     Yhi[(2 * mm) * LDP + tid] = hi.x;
     Yhi[(2 * mm + 1) * LDP + tid] = hi.y;
     Ylo[(2 * mm) * LDP + tid] = lo.x;

@@ -94,4 +94,49 @@ struct YlmDown {
   }
 };
 
+// Pixel-mode generator: Y* at a HEALPix pixel as the reference's ring FFT path
+// sees it (sfb_reverse_fft.cpp:192-207).  For m with (m mod np) > np/2 the
+// reference reads the folded bin X[np - s] without conjugating it, which equals
+// the pixel sum with azimuth phi' = 2 phi0 - phi instead of phi; with
+// alias_compat that azimuth is used for those m, otherwise the pixel centre.
+struct YlmDownPix {
+  YlmDown y;
+  double cm2, sm2, cph2, sph2;  // cos/sin(m phi') and the step rotation
+  int np, smod;                 // ring length, m mod np
+  bool quirk;
+
+  __device__ __forceinline__ void init(int l, double x, double s, double phi, double cph, double sph,
+                                       double phi2, double cph2_, double sph2_, int np_, bool alias,
+                                       double2 sect) {
+    y.init(l, x, s, phi, cph, sph, sect);
+    const double hi = (double)l * phi2;
+    const double lo = fma((double)l, phi2, -hi);
+    double S, C;
+    sincos(hi, &S, &C);
+    cm2 = C - lo * S;
+    sm2 = S + lo * C;
+    cph2 = cph2_;
+    sph2 = sph2_;
+    np = np_;
+    smod = l % np_;
+    quirk = alias;
+  }
+  __device__ __forceinline__ void value(int m, double &re, double &im) const {
+    const double P = ldexp(y.v, y.E);
+    const bool flip = quirk && m > 0 && 2 * smod > np;
+    const double c = flip ? cm2 : y.cm, s = flip ? sm2 : y.sm;
+    re = P * c;
+    im = -P * s;
+  }
+  __device__ __forceinline__ void step(double2 ab) {
+    y.step(ab);
+    const double cn = fma(cm2, cph2, sm2 * sph2);
+    const double sn = fma(sm2, cph2, -cm2 * sph2);
+    cm2 = cn;
+    sm2 = sn;
+    smod = smod == 0 ? np - 1 : smod - 1;
+  }
+  __device__ __forceinline__ void rescale() { y.rescale(); }
+};
+
 }  // namespace sfb
