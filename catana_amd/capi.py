@@ -19,7 +19,8 @@ OPERANDS_FP32, OPERANDS_SPLIT_FP16 = 0, 1
 
 EXPORTED = [
     "sfbhip_default_params", "sfbhip_create", "sfbhip_destroy", "sfbhip_flmn_count", "sfbhip_kln",
-    "sfbhip_direct", "sfbhip_pixelize", "sfbhip_healpix", "sfbhip_last_kernel_ms", "sfbhip_last_error",
+    "sfbhip_direct", "sfbhip_pixelize", "sfbhip_ang2pix_ring", "sfbhip_healpix", "sfbhip_last_kernel_ms",
+    "sfbhip_last_error",
 ]
 
 
@@ -57,6 +58,7 @@ def lib():
         L.sfbhip_kln.argtypes = [P, ctypes.POINTER(Params), P, P]
         L.sfbhip_direct.argtypes = [P, P, i64, ctypes.POINTER(Params), P, P, P, P]
         L.sfbhip_pixelize.argtypes = [P, P, i64, i32, P, P, P]
+        L.sfbhip_ang2pix_ring.argtypes = [i32, ctypes.c_double, ctypes.c_double, ctypes.POINTER(i64)]
         L.sfbhip_healpix.argtypes = [P, P, P, i32, ctypes.POINTER(Params), P, P, P, P]
         L.sfbhip_last_kernel_ms.argtypes = [P, ctypes.POINTER(ctypes.c_float)]
         L.sfbhip_last_error.restype = ctypes.c_char_p

@@ -1,0 +1,331 @@
+// catana_api.cpp -- host C++ API of catana_api.hpp over the C-ABI.
+#include "catana_api.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <memory>
+#include <mutex>
+#include <stdexcept>
+
+#include "catana_hip.h"
+
+namespace catana {
+
+namespace {
+
+constexpr double kTwoPi = 6.283185307179586476925;
+
+void throw_status(sfbhip_status st, const char *what) {
+  if (st == SFBHIP_OK) return;
+  const std::string msg = std::string(what) + ": " + sfbhip_last_error();
+  if (st == SFBHIP_INVALID_ARGUMENT) throw std::invalid_argument(msg);
+  if (st == SFBHIP_DOMAIN_ERROR) throw std::domain_error(msg);
+  throw std::runtime_error(msg);
+}
+
+void hip_check(hipError_t e, const char *what) {
+  if (e != hipSuccess) throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(e));
+}
+
+// Device buffer owned by the host API (the C-ABI itself allocates only its workspace).
+struct DevMem {
+  void *p = nullptr;
+  size_t bytes = 0;
+  void reserve(size_t n) {
+    if (n <= bytes) return;
+    if (p) hip_check(hipFree(p), "hipFree");
+    p = nullptr;
+    bytes = 0;
+    hip_check(hipMalloc(&p, n), "hipMalloc");
+    bytes = n;
+  }
+  ~DevMem() {
+    if (p) (void)hipFree(p);
+  }
+};
+
+// Process-wide context on device 0: one handle and reusable staging buffers.
+struct Context {
+  sfbhip_handle *h = nullptr;
+  DevMem pts, off, rad, kln, cln, flmn;
+  std::mutex mu;
+  Context() { throw_status(sfbhip_create(&h, 0), "sfbhip_create"); }
+  ~Context() { sfbhip_destroy(h); }
+};
+
+// Never destroyed: at process exit the HIP runtime may already be gone, and
+// the driver releases the device memory anyway.
+Context &ctx() {
+  static Context *c = new Context;
+  return *c;
+}
+
+sfbhip_params make_params(unsigned short lmax, unsigned short nmax, double rmax, bool verbose, const Options &o) {
+  sfbhip_params p;
+  sfbhip_default_params(&p);
+  p.lmax = lmax;
+  p.nmax = nmax;
+  p.rmax = rmax;
+  p.k_max = o.k_max;
+  p.window_volume = o.window_volume;
+  p.precision = o.precision ? SFBHIP_PREC_MIXED : SFBHIP_PREC_FP64;
+  p.bessel = o.interpolated < 0 ? SFBHIP_BESSEL_AUTO : (o.interpolated ? SFBHIP_BESSEL_REFERENCE : SFBHIP_BESSEL_EXACT);
+  p.zeros = o.zeros ? SFBHIP_ZEROS_CONVERGED : SFBHIP_ZEROS_REFERENCE;
+  p.alias_compat = o.alias_compat ? 1 : 0;
+  p.verbose = verbose ? 1 : 0;
+  return p;
+}
+
+KClkk download(Context &c, unsigned short lmax, unsigned short nmax, bool store_flmn) {
+  KClkk out;
+  out.k_ln = Array2D(lmax, nmax);
+  out.c_ln = Array2D(lmax, nmax);
+  const size_t n2 = (size_t)lmax * nmax;
+  if (n2) {
+    hip_check(hipMemcpy(out.k_ln.data.data(), c.kln.p, n2 * sizeof(double), hipMemcpyDeviceToHost), "copy k_ln");
+    hip_check(hipMemcpy(out.c_ln.data.data(), c.cln.p, n2 * sizeof(double), hipMemcpyDeviceToHost), "copy c_ln");
+  }
+  if (store_flmn) {
+    const size_t nf = sfbhip_flmn_count(lmax, nmax);
+    std::vector<std::complex<double>> flat(nf);
+    if (nf)
+      hip_check(hipMemcpy(flat.data(), c.flmn.p, nf * sizeof(std::complex<double>), hipMemcpyDeviceToHost),
+                "copy f_lmn");
+    size_t off = 0;
+    for (unsigned l = 0; l < lmax; ++l) {
+      ComplexBlock b;
+      b.rows = l + 1;
+      b.cols = nmax;
+      b.data.assign(flat.begin() + off, flat.begin() + off + (size_t)(l + 1) * nmax);
+      off += (size_t)(l + 1) * nmax;
+      out.f_lmn.push_back(std::move(b));
+    }
+  }
+  return out;
+}
+
+void reserve_outputs(Context &c, unsigned short lmax, unsigned short nmax, bool store_flmn) {
+  c.kln.reserve(std::max<size_t>((size_t)lmax * nmax * sizeof(double), 8));
+  c.cln.reserve(std::max<size_t>((size_t)lmax * nmax * sizeof(double), 8));
+  if (store_flmn) c.flmn.reserve(std::max<size_t>(sfbhip_flmn_count(lmax, nmax) * 16, 16));
+}
+
+KClkk run_direct(const PointContainer &points, unsigned short lmax, unsigned short nmax, double rmax, bool store,
+                 bool verbose, const Options &o) {
+  Context &c = ctx();
+  std::lock_guard<std::mutex> lk(c.mu);
+  const size_t n = points.size();
+  c.pts.reserve(std::max<size_t>(n * sizeof(Point), 24));
+  if (n) hip_check(hipMemcpy(c.pts.p, points.data(), n * sizeof(Point), hipMemcpyHostToDevice), "copy points");
+  reserve_outputs(c, lmax, nmax, store);
+  sfbhip_params p = make_params(lmax, nmax, rmax, verbose, o);
+  throw_status(sfbhip_direct(c.h, static_cast<const double *>(c.pts.p), (int64_t)n, &p,
+                             static_cast<double *>(c.kln.p), static_cast<double *>(c.cln.p),
+                             store ? static_cast<double *>(c.flmn.p) : nullptr, nullptr),
+               "sfb_decomposition");
+  return download(c, lmax, nmax, store);
+}
+
+KClkk run_healpix(const PixelizedPointContainer &pix, unsigned short lmax, unsigned short nmax, double rmax,
+                  bool store, bool verbose, const Options &o) {
+  Context &c = ctx();
+  std::lock_guard<std::mutex> lk(c.mu);
+  std::vector<int64_t> off;
+  std::vector<double> rad;
+  pix.to_csr(off, rad);
+  c.off.reserve(off.size() * sizeof(int64_t));
+  c.rad.reserve(std::max<size_t>(rad.size() * sizeof(double), 8));
+  hip_check(hipMemcpy(c.off.p, off.data(), off.size() * sizeof(int64_t), hipMemcpyHostToDevice), "copy offsets");
+  if (!rad.empty())
+    hip_check(hipMemcpy(c.rad.p, rad.data(), rad.size() * sizeof(double), hipMemcpyHostToDevice), "copy radii");
+  reserve_outputs(c, lmax, nmax, store);
+  sfbhip_params p = make_params(lmax, nmax, rmax, verbose, o);
+  p.precision = SFBHIP_PREC_FP64;
+  throw_status(sfbhip_healpix(c.h, static_cast<const int64_t *>(c.off.p), static_cast<const double *>(c.rad.p),
+                              (int32_t)pix.get_nside(), &p, static_cast<double *>(c.kln.p),
+                              static_cast<double *>(c.cln.p), store ? static_cast<double *>(c.flmn.p) : nullptr,
+                              nullptr),
+               "sfb_decomposition");
+  return download(c, lmax, nmax, store);
+}
+
+bool is_power_of_2(unsigned int n) { return n && ((n & (n - 1)) == 0); }
+
+}  // namespace
+
+// ---------------------------------------------------------------- points
+Point::Point(double x, double y, double z) {
+  r = std::sqrt(x * x + y * y + z * z);
+  p.theta = std::atan2(std::sqrt(x * x + y * y), z);
+  double ph = (x == 0.0 && y == 0.0) ? 0.0 : std::atan2(y, x);
+  if (ph < 0.0) ph += kTwoPi;
+  if (ph >= kTwoPi) ph -= kTwoPi;
+  p.phi = ph;
+}
+
+Point point_from_spherical_position(double r, double theta, double phi) {
+  Point o;
+  o.r = r;
+  o.p.theta = theta;
+  o.p.phi = phi;
+  return o;
+}
+
+Point point_from_box_position(double x, double y, double z, double shift) {
+  return Point(x + shift, y + shift, z + shift);
+}
+
+// ---------------------------------------------------------------- KClkk
+Array2D compute_kln(unsigned short lmax, unsigned short nmax, double rmax, int zeros) {
+  Context &c = ctx();
+  std::lock_guard<std::mutex> lk(c.mu);
+  Options o;
+  o.zeros = zeros;
+  sfbhip_params p = make_params(lmax, nmax, rmax, false, o);
+  Array2D k(lmax, nmax);
+  std::vector<int32_t> nl(std::max<int>(lmax, 1));
+  throw_status(sfbhip_kln(c.h, &p, k.data.data(), nl.data()), "compute_kln");
+  return k;
+}
+
+KClkk::KClkk(unsigned short lmax, unsigned short nmax, double rmax)
+    : k_ln(compute_kln(lmax, nmax, rmax)), c_ln(lmax, nmax) {}
+
+void KClkk::savetxt(const std::string &base) const {
+  auto dump = [](const std::string &path, const Array2D &a) {
+    FILE *f = std::fopen(path.c_str(), "w");
+    if (!f) {
+      std::fprintf(stderr, "Error creating output file: %s\n", path.c_str());
+      return;
+    }
+    for (size_t i = 0; i < a.rows; ++i) {
+      for (size_t j = 0; j < a.cols; ++j) {
+        std::fprintf(f, "%.18e", a(i, j));
+        if (j != a.cols - 1) std::fputc(' ', f);
+      }
+      std::fputc('\n', f);
+    }
+    std::fclose(f);
+  };
+  dump(base + ".k_ln", k_ln);
+  dump(base + ".c_ln", c_ln);
+}
+
+// ---------------------------------------------------------------- pixelised container
+PixelizedPointContainer::PixelizedPointContainer(unsigned int nside) : nside_(nside) {
+  if (!is_power_of_2(nside)) throw std::invalid_argument("argument (nside) must be a power of 2");
+  pixels_.resize((size_t)12 * nside * nside);
+}
+
+PixelizedPointContainer::PixelizedPointContainer(unsigned int nside, const PointContainer &pc)
+    : PixelizedPointContainer(nside) {
+  if (pc.empty()) return;
+  Context &c = ctx();
+  std::lock_guard<std::mutex> lk(c.mu);
+  const size_t n = pc.size(), npix = pixels_.size();
+  c.pts.reserve(n * sizeof(Point));
+  c.off.reserve((npix + 1) * sizeof(int64_t));
+  c.rad.reserve(n * sizeof(double));
+  hip_check(hipMemcpy(c.pts.p, pc.data(), n * sizeof(Point), hipMemcpyHostToDevice), "copy points");
+  throw_status(sfbhip_pixelize(c.h, static_cast<const double *>(c.pts.p), (int64_t)n, (int32_t)nside,
+                               static_cast<int64_t *>(c.off.p), static_cast<double *>(c.rad.p), nullptr),
+               "PixelizedPointContainer");
+  std::vector<int64_t> off(npix + 1);
+  std::vector<double> rad(n);
+  hip_check(hipMemcpy(off.data(), c.off.p, off.size() * sizeof(int64_t), hipMemcpyDeviceToHost), "copy offsets");
+  hip_check(hipMemcpy(rad.data(), c.rad.p, n * sizeof(double), hipMemcpyDeviceToHost), "copy radii");
+  for (size_t q = 0; q < npix; ++q) pixels_[q].assign(rad.begin() + off[q], rad.begin() + off[q + 1]);
+}
+
+void PixelizedPointContainer::add_point(const Point &pt) {
+  int64_t q = 0;
+  throw_status(sfbhip_ang2pix_ring((int32_t)nside_, pt.p.theta, pt.p.phi, &q), "add_point");
+  pixels_[(size_t)q].push_back(pt.r);
+}
+
+size_t PixelizedPointContainer::get_npoints() const {
+  size_t s = 0;
+  for (const auto &v : pixels_) s += v.size();
+  return s;
+}
+
+std::vector<int> PixelizedPointContainer::get_countmap() const {
+  std::vector<int> m(pixels_.size());
+  for (size_t i = 0; i < pixels_.size(); ++i) m[i] = (int)pixels_[i].size();
+  return m;
+}
+
+void PixelizedPointContainer::to_csr(std::vector<int64_t> &offsets, std::vector<double> &radii) const {
+  offsets.assign(pixels_.size() + 1, 0);
+  radii.clear();
+  radii.reserve(get_npoints());
+  for (size_t i = 0; i < pixels_.size(); ++i) {
+    radii.insert(radii.end(), pixels_[i].begin(), pixels_[i].end());
+    offsets[i + 1] = (int64_t)radii.size();
+  }
+}
+
+// ---------------------------------------------------------------- decompositions
+KClkk sfb_decomposition(const PointContainer &points, unsigned short lmax, unsigned short nmax, double rmax,
+                        bool store_flmn, bool verbose, const Options &opt) {
+  return run_direct(points, lmax, nmax, rmax, store_flmn, verbose, opt);
+}
+
+KClkk sfb_decomposition(const PixelizedPointContainer &pix, unsigned short lmax, unsigned short nmax, double rmax,
+                        bool store_flmn, bool verbose, const Options &opt) {
+  return run_healpix(pix, lmax, nmax, rmax, store_flmn, verbose, opt);
+}
+
+KClkk _sfb_raw(const PointContainer &points, unsigned short lmax, unsigned short nmax, double rmax, bool store_flmn,
+               bool verbose, bool /*parallel*/, bool interpolated) {
+  Options o;
+  o.interpolated = interpolated ? 1 : 0;
+  return run_direct(points, lmax, nmax, rmax, store_flmn, verbose, o);
+}
+
+KClkk _sfb_reverse_fft(const PixelizedPointContainer &pix, unsigned short lmax, unsigned short nmax, double rmax,
+                       bool store_flmn, bool verbose, bool /*parallel*/, bool interpolated) {
+  Options o;
+  o.interpolated = interpolated ? 1 : 0;
+  o.alias_compat = true;
+  return run_healpix(pix, lmax, nmax, rmax, store_flmn, verbose, o);
+}
+
+KClkk _sfb_reverse(const PixelizedPointContainer &pix, unsigned short lmax, unsigned short nmax, double rmax,
+                   bool store_flmn, bool verbose, bool /*parallel*/, bool interpolated) {
+  Options o;
+  o.interpolated = interpolated ? 1 : 0;
+  o.alias_compat = false;
+  return run_healpix(pix, lmax, nmax, rmax, store_flmn, verbose, o);
+}
+
+KClkk sfb_decomposition_reverse(const PointContainer &points, unsigned short lmax, double k_max, double rmax,
+                                unsigned short nmax, double window_volume, bool store_flmn, bool verbose,
+                                int precision) {
+  Options o;
+  o.precision = precision;
+  o.interpolated = 0;
+  o.k_max = k_max;
+  o.window_volume = window_volume;
+  return run_direct(points, lmax, nmax, rmax, store_flmn, verbose, o);
+}
+
+KClkk sfb_decomposition_healpix(const PixelizedPointContainer &pix, unsigned short lmax, double k_max, double rmax,
+                                unsigned short nmax, double window_volume, bool store_flmn, bool verbose) {
+  Options o;
+  o.interpolated = 0;
+  o.k_max = k_max;
+  o.window_volume = window_volume;
+  return run_healpix(pix, lmax, nmax, rmax, store_flmn, verbose, o);
+}
+
+KClkk sfb_decomposition_healpix(const PointContainer &points, unsigned int nside, unsigned short lmax, double k_max,
+                                double rmax, unsigned short nmax, double window_volume, bool store_flmn,
+                                bool verbose) {
+  return sfb_decomposition_healpix(PixelizedPointContainer(nside, points), lmax, k_max, rmax, nmax, window_volume,
+                                   store_flmn, verbose);
+}
+
+}  // namespace catana

@@ -1,0 +1,125 @@
+// catana_api.hpp -- host C++ API with the names and argument meanings of
+// CatAna's decomposition interface (catana/include/catana/decomposition/,
+// catana/include/catana/points/), implemented over the C-ABI of
+// libcatana_hip.so.  Errors are the reference's exception types:
+// std::invalid_argument (nside, sizes), std::domain_error (Bessel argument out
+// of the interpolation range), std::runtime_error (device failures).
+#pragma once
+
+#include <complex>
+#include <cstddef>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace catana {
+
+// Angular position (theta from the z axis, phi in [0, 2 pi)); the layout of HEALPix's pointing.
+struct pointing {
+  double theta = 0.0, phi = 0.0;
+};
+
+// Point.hpp:10-19: {r, pointing} -- 24 bytes, the AoS the kernels read.
+struct Point {
+  Point() = default;
+  Point(double x, double y, double z);  // from cartesian coordinates (Point.cpp:5-10)
+  double r = 0.0;
+  pointing p;
+};
+static_assert(sizeof(Point) == 3 * sizeof(double), "Point must be the 24-byte (r, theta, phi) record");
+
+Point point_from_spherical_position(double r, double theta, double phi);
+Point point_from_box_position(double x, double y, double z, double shift = 0.0);
+
+// PointContainer.hpp:11-19
+class PointContainer : public std::vector<Point> {
+ public:
+  using std::vector<Point>::vector;
+  void add_point(const Point &p) { push_back(p); }
+};
+
+// Row-major 2-D array of doubles (the reference uses Eigen arrays).
+struct Array2D {
+  Array2D() = default;
+  Array2D(std::size_t r, std::size_t c) : rows(r), cols(c), data(r * c, 0.0) {}
+  double &operator()(std::size_t i, std::size_t j) { return data[i * cols + j]; }
+  double operator()(std::size_t i, std::size_t j) const { return data[i * cols + j]; }
+  std::size_t rows = 0, cols = 0;
+  std::vector<double> data;
+};
+
+// Complex (l+1) x nmax block, row-major (m, n).
+struct ComplexBlock {
+  std::size_t rows = 0, cols = 0;
+  std::vector<std::complex<double>> data;
+};
+
+// return_types.hpp:19-38
+struct KClkk {
+  KClkk(unsigned short lmax, unsigned short nmax, double rmax);  // k_ln from the zeros of j_l, c_ln = 0
+  KClkk() = default;
+  Array2D k_ln;                     // k at which C_l(k, k) is evaluated, (lmax, nmax)
+  Array2D c_ln;                     // C_l(k_ln, k_ln), (lmax, nmax)
+  std::vector<ComplexBlock> f_lmn;  // per l: (l+1) x nmax, m >= 0 (when store_flmn)
+  // "<base>.k_ln" and "<base>.c_ln": rows = l, precision 18, scientific (return_types.cpp:25-64)
+  void savetxt(const std::string &filename_base) const;
+};
+
+// PixelizedPointContainer.hpp: HEALPix RING map of radii per pixel.
+class PixelizedPointContainer {
+ public:
+  explicit PixelizedPointContainer(unsigned int nside);  // throws std::invalid_argument if not a power of 2
+  PixelizedPointContainer(unsigned int nside, const PointContainer &pc);  // binned on the GPU
+  void add_point(const Point &p);
+  unsigned int get_nside() const { return nside_; }
+  std::size_t get_npoints() const;
+  std::size_t size() const { return pixels_.size(); }
+  std::vector<int> get_countmap() const;
+  const std::vector<double> &pixel(std::size_t i) const { return pixels_[i]; }
+  // CSR view: offsets (npix + 1) and radii in pixel order
+  void to_csr(std::vector<int64_t> &offsets, std::vector<double> &radii) const;
+
+ private:
+  unsigned int nside_;
+  std::vector<std::vector<double>> pixels_;
+};
+
+// How the call is evaluated on the GPU (extensions; defaults reproduce the reference).
+struct Options {
+  int precision = 0;         // 0 FP64 (drop-in), 1 mixed
+  int interpolated = -1;     // -1: reference rule N > 1000; 0 exact j_l; 1 reference linear table
+  int zeros = 0;             // 0 reference Newton loop, 1 converged
+  double k_max = 0.0;        // > 0: per-l truncation n_l = #{z_ln < k_max rmax}
+  double window_volume = 0;  // > 0 replaces (4/3) pi rmax^3
+  bool alias_compat = true;  // pixelised path: keep the reference's unconjugated fold
+};
+
+// sfb_decomposition.hpp:29-47 (public wrappers: interpolated = N > 1000)
+KClkk sfb_decomposition(const PointContainer &points, unsigned short lmax, unsigned short nmax, double rmax,
+                        bool store_flmn, bool verbose, const Options &opt = Options());
+KClkk sfb_decomposition(const PixelizedPointContainer &pix, unsigned short lmax, unsigned short nmax, double rmax,
+                        bool store_flmn, bool verbose, const Options &opt = Options());
+
+// sfb_raw.hpp / sfb_reverse.hpp / sfb_reverse_fft.hpp (parallel has no effect: the GPU is always parallel)
+KClkk _sfb_raw(const PointContainer &points, unsigned short lmax, unsigned short nmax, double rmax,
+               bool store_flmn, bool verbose, bool parallel, bool interpolated);
+KClkk _sfb_reverse_fft(const PixelizedPointContainer &pix, unsigned short lmax, unsigned short nmax, double rmax,
+                       bool store_flmn, bool verbose, bool parallel, bool interpolated);
+KClkk _sfb_reverse(const PixelizedPointContainer &pix, unsigned short lmax, unsigned short nmax, double rmax,
+                   bool store_flmn, bool verbose, bool parallel, bool interpolated);
+
+// Variants with a k_max cut and a window volume (n_l per l; nmax bounds the columns);
+// j_l is evaluated exactly (no interpolation table) whatever N.
+KClkk sfb_decomposition_reverse(const PointContainer &points, unsigned short lmax, double k_max, double rmax,
+                                unsigned short nmax, double window_volume, bool store_flmn, bool verbose,
+                                int precision = 1);
+KClkk sfb_decomposition_healpix(const PixelizedPointContainer &pix, unsigned short lmax, double k_max, double rmax,
+                                unsigned short nmax, double window_volume, bool store_flmn, bool verbose);
+KClkk sfb_decomposition_healpix(const PointContainer &points, unsigned int nside, unsigned short lmax,
+                                double k_max, double rmax, unsigned short nmax, double window_volume,
+                                bool store_flmn, bool verbose);
+
+// Zeros-based k_ln table (return_types.cpp:10-19).
+Array2D compute_kln(unsigned short lmax, unsigned short nmax, double rmax, int zeros = 0);
+
+}  // namespace catana

@@ -703,6 +703,17 @@ sfbhip_status sfbhip_pixelize(sfbhip_handle *h, const double *d_points, int64_t 
   return SFBHIP_OK;
 }
 
+sfbhip_status sfbhip_ang2pix_ring(int32_t nside, double theta, double phi, int64_t *pix) {
+  g_last_error.clear();
+  if (!pix) return fail(SFBHIP_INVALID_ARGUMENT, "sfbhip_ang2pix_ring: pix is NULL");
+  if (nside <= 0 || nside > 8192 || (nside & (nside - 1)))
+    return fail(SFBHIP_INVALID_ARGUMENT, "argument (nside) must be a power of 2");
+  if (!std::isfinite(theta) || !std::isfinite(phi))
+    return fail(SFBHIP_INVALID_ARGUMENT, "sfbhip_ang2pix_ring: non-finite angle");
+  *pix = hp_ang2pix(nside, theta, phi);
+  return SFBHIP_OK;
+}
+
 sfbhip_status sfbhip_healpix(sfbhip_handle *h, const int64_t *d_offsets, const double *d_radii, int32_t nside,
                              const sfbhip_params *p, double *d_k_ln, double *d_c_ln, double *d_f_lmn,
                              void *stream) {

@@ -16,7 +16,7 @@ namespace sfb {
 
 constexpr double HP_PI = 3.14159265358979323846;
 
-__device__ __forceinline__ int64_t hp_isqrt(int64_t v) {
+__host__ __device__ inline int64_t hp_isqrt(int64_t v) {
   int64_t r = (int64_t)sqrt((double)v + 0.5);
   while (r * r > v) --r;
   while ((r + 1) * (r + 1) <= v) ++r;
@@ -25,7 +25,7 @@ __device__ __forceinline__ int64_t hp_isqrt(int64_t v) {
 
 // ring index (1-based), first pixel, ring length, shifted flag and 1 - z (exact
 // in the caps) of pixel p.
-__device__ __forceinline__ void hp_ring_of(int64_t ns, int64_t p, int64_t &ring, int64_t &start, int64_t &np,
+__host__ __device__ inline void hp_ring_of(int64_t ns, int64_t p, int64_t &ring, int64_t &start, int64_t &np,
                                            bool &shifted) {
   const int64_t npix = 12 * ns * ns, ncap = 2 * ns * (ns - 1);
   if (p < ncap) {
@@ -91,7 +91,7 @@ __global__ void pixel_setup_kernel(int nside, double *__restrict__ x, double *__
   npr[p] = (int32_t)np;
 }
 
-__device__ __forceinline__ int64_t hp_ang2pix(int64_t ns, double theta, double phi) {
+__host__ __device__ inline int64_t hp_ang2pix(int64_t ns, double theta, double phi) {
   const int64_t npix = 12 * ns * ns, ncap = 2 * ns * (ns - 1);
   const double z = cos(theta), za = fabs(z);
   double tt = fmod(phi * (2.0 / HP_PI), 4.0);

@@ -109,6 +109,11 @@ sfbhip_status sfbhip_direct(sfbhip_handle *h, const double *d_points, int64_t n_
 sfbhip_status sfbhip_pixelize(sfbhip_handle *h, const double *d_points, int64_t n_points, int32_t nside,
                               int64_t *d_offsets, double *d_radii, void *stream);
 
+/* HEALPix RING pixel of one direction, on the host (the same function the
+ * device binning uses).  Replaces Healpix_Base::ang2pix as called by
+ * PixelizedPointContainer::add_point (PixelizedPointContainer.cpp:32-35). */
+sfbhip_status sfbhip_ang2pix_ring(int32_t nside, double theta, double phi, int64_t *pix);
+
 /* HEALPix SFB on a pixelised catalogue (CSR as produced by sfbhip_pixelize).
  * Replaces catana::_sfb_reverse_fft (sfb_reverse_fft.cpp:43-209) and
  * catana::sfb_decomposition(const PixelizedPointContainer&, ...) (sfb_decomposition.cpp:22-26);
