@@ -20,7 +20,7 @@ OPERANDS_FP32, OPERANDS_SPLIT_FP16 = 0, 1
 EXPORTED = [
     "sfbhip_default_params", "sfbhip_create", "sfbhip_destroy", "sfbhip_flmn_count", "sfbhip_kln",
     "sfbhip_direct", "sfbhip_pixelize", "sfbhip_ang2pix_ring", "sfbhip_healpix", "sfbhip_last_kernel_ms",
-    "sfbhip_last_error",
+    "sfbhip_last_phase_ms", "sfbhip_last_error",
 ]
 
 
@@ -61,6 +61,7 @@ def lib():
         L.sfbhip_ang2pix_ring.argtypes = [i32, ctypes.c_double, ctypes.c_double, ctypes.POINTER(i64)]
         L.sfbhip_healpix.argtypes = [P, P, P, i32, ctypes.POINTER(Params), P, P, P, P]
         L.sfbhip_last_kernel_ms.argtypes = [P, ctypes.POINTER(ctypes.c_float)]
+        L.sfbhip_last_phase_ms.argtypes = [P, ctypes.POINTER(ctypes.c_float)]
         L.sfbhip_last_error.restype = ctypes.c_char_p
         L.sfbhip_last_error.argtypes = []
         _lib = L
@@ -140,6 +141,12 @@ class Handle:
         ms = ctypes.c_float()
         check(lib().sfbhip_last_kernel_ms(self._h, ctypes.byref(ms)), "sfbhip_last_kernel_ms")
         return ms.value
+
+    def last_phase_ms(self):
+        """(FP64 contraction, mixed contraction, reduction + C_l) of the last call, ms."""
+        ms = (ctypes.c_float * 3)()
+        check(lib().sfbhip_last_phase_ms(self._h, ms), "sfbhip_last_phase_ms")
+        return tuple(ms)
 
 
 def _torch():

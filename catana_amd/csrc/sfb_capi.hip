@@ -21,6 +21,7 @@
 #include "sfb_common.hpp"
 #include "sfb_direct_f64.hpp"
 #include "sfb_direct_mixed.hpp"
+#include "sfb_lowl.hpp"
 #include "sfb_healpix.hpp"
 #include "sfb_math.hpp"
 #include "sfb_support.hpp"
@@ -63,6 +64,9 @@ struct KlnCache {
   bool table_valid = false;   // reference interpolation table on the device
   bool herm_valid = false;    // Hermite tables of the mixed path on the device
   std::vector<int32_t> herm_off, herm_nn;
+  bool q5_valid = false;      // quintic FP64 tables of the low multipoles on the device
+  int q5_lmax = 0;
+  double q5_R = 0.0;
   int32_t herm_max = 0;
   double herm_R = 0.0;        // radius the tables cover: k_{l,n_l-1} * herm_R is the last argument
 };
@@ -71,7 +75,8 @@ struct sfbhip_handle {
   int device = 0;
   std::map<std::string, DevBuf> bufs;
   int *h_err = nullptr;  // pinned: [0] error bits, [2..3] largest radius of the last call (double)
-  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};  // phase boundaries of the last call
+  float phase_ms[3] = {0.f, 0.f, 0.f};                      // FP64 contraction, mixed contraction, finalize
   float last_ms = 0.f;
   KlnCache kc;
   int leg_lmax = -1;
@@ -131,6 +136,7 @@ sfbhip_status ensure_kln(sfbhip_handle *h, const sfbhip_params *p, hipStream_t s
   c.valid = false;
   c.table_valid = false;
   c.herm_valid = false;
+  c.q5_valid = false;
   const int L = p->lmax, nz = p->nmax + 1;
   c.zeros.assign((size_t)std::max(L, 0) * nz, 0.0);
   c.kln.assign((size_t)std::max(L, 0) * p->nmax, 0.0);
@@ -386,7 +392,10 @@ sfbhip_status finalize(sfbhip_handle *h, const sfbhip_params *p, const std::vect
     HIPCHK(hipMemcpyAsync(dg, groups.data(), sizeof(ReduceGroup) * groups.size(), hipMemcpyHostToDevice, st));
     double *dk;
     CHK(get_buf(h, "kln", (size_t)L * nmax, &dk));
-    reduce_partials_kernel<<<(unsigned)groups.size(), 256, 0, st>>>(dg, partial, 0, dk, nmax, doff, norm, f);
+    int chunks = 1;
+    for (const ReduceGroup &g : groups) chunks = std::max<int>(chunks, (int)cdiv((int64_t)g.nrows * 2 * (g.l + 1), 256));
+    dim3 grid((unsigned)groups.size(), (unsigned)std::min(chunks, 64));
+    reduce_partials_kernel<<<grid, 256, 0, st>>>(dg, partial, dk, nmax, doff, norm, f);
     HIPCHK(hipGetLastError());
   }
   if (L > 0) {
@@ -398,6 +407,49 @@ sfbhip_status finalize(sfbhip_handle *h, const sfbhip_params *p, const std::vect
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(d_k_ln, dk, sizeof(double) * L * nmax, hipMemcpyDeviceToDevice, st));
   }
+  return SFBHIP_OK;
+}
+
+// Quintic FP64 j_l tables for l < L (the mixed path's FP64 multipoles):
+// intervals of Q5_H covering [0, k_{l,n_l-1} R], R = max(rmax, largest radius).
+sfbhip_status ensure_quintic(sfbhip_handle *h, const sfbhip_params *p, int L, double rdata, hipStream_t st,
+                             BesselTableRef *t) {
+  KlnCache &c = h->kc;
+  const double R = std::max(p->rmax, rdata);
+  int32_t *doff, *dni;
+  double *tab;
+  if (!c.q5_valid || c.q5_R < R || c.q5_lmax < L) {
+    std::vector<int32_t> off(std::max(L, 1), 0), ni(std::max(L, 1), 1);
+    int32_t o = 0, mx = 1;
+    for (int l = 0; l < L; ++l) {
+      int32_t n = 1;
+      if (c.nl[l] > 0) n = (int32_t)std::floor(c.kln[(size_t)l * p->nmax + c.nl[l] - 1] * R * Q5_HINV) + 2;
+      off[l] = o;
+      ni[l] = n;
+      o += n;
+      mx = std::max(mx, n);
+    }
+    CHK(get_buf(h, "q5_tab", (size_t)6 * std::max(o, 1), &tab));
+    CHK(get_buf(h, "q5_off", off.size(), &doff));
+    CHK(get_buf(h, "q5_nint", ni.size(), &dni));
+    HIPCHK(hipMemcpyAsync(doff, off.data(), sizeof(int32_t) * off.size(), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(dni, ni.data(), sizeof(int32_t) * ni.size(), hipMemcpyHostToDevice, st));
+    if (L > 0) {
+      dim3 grid((unsigned)cdiv(mx, 256), (unsigned)L);
+      quintic_table_kernel<<<grid, 256, 0, st>>>(L, doff, dni, tab);
+      HIPCHK(hipGetLastError());
+    }
+    HIPCHK(hipStreamSynchronize(st));
+    c.q5_valid = true;
+    c.q5_R = R;
+    c.q5_lmax = L;
+  }
+  CHK(get_buf(h, "q5_tab", 1, &tab));
+  CHK(get_buf(h, "q5_off", 1, &doff));
+  CHK(get_buf(h, "q5_nint", 1, &dni));
+  t->q5 = tab;
+  t->q5_off = doff;
+  t->q5_nint = dni;
   return SFBHIP_OK;
 }
 
@@ -516,6 +568,19 @@ sfbhip_status launch_plan_mixed(sfbhip_handle *h, const DirectPlan &pl, DirectIt
   return SFBHIP_OK;
 }
 
+void clear_timings(sfbhip_handle *h) {
+  h->last_ms = 0.f;
+  for (float &t : h->phase_ms) t = 0.f;
+}
+
+// After the stream has been synchronised: phase durations from the four
+// events; the dominant kernel is the contraction of phase `dominant`.
+sfbhip_status read_timings(sfbhip_handle *h, int dominant) {
+  for (int i = 0; i < 3; ++i) HIPCHK(hipEventElapsedTime(&h->phase_ms[i], h->ev[i], h->ev[i + 1]));
+  h->last_ms = h->phase_ms[dominant];
+  return SFBHIP_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -551,8 +616,7 @@ sfbhip_status sfbhip_create(sfbhip_handle **out, int device) {
     delete h;
     return fail(SFBHIP_HIP_ERROR, std::string("hipHostMalloc: ") + hipGetErrorString(e));
   }
-  HIPCHK(hipEventCreate(&h->ev0));
-  HIPCHK(hipEventCreate(&h->ev1));
+  for (auto &e : h->ev) HIPCHK(hipEventCreate(&e));
   *out = h;
   return SFBHIP_OK;
 }
@@ -564,8 +628,8 @@ sfbhip_status sfbhip_destroy(sfbhip_handle *h) {
   for (auto &kv : h->bufs)
     if (kv.second.p) (void)hipFree(kv.second.p);
   if (h->h_err) (void)hipHostFree(h->h_err);
-  if (h->ev0) (void)hipEventDestroy(h->ev0);
-  if (h->ev1) (void)hipEventDestroy(h->ev1);
+  for (auto &e : h->ev)
+    if (e) (void)hipEventDestroy(e);
   delete h;
   return SFBHIP_OK;
 }
@@ -589,6 +653,12 @@ sfbhip_status sfbhip_kln(sfbhip_handle *h, const sfbhip_params *p, double *k_ln_
 sfbhip_status sfbhip_last_kernel_ms(sfbhip_handle *h, float *ms) {
   if (!h || !ms) return fail(SFBHIP_INVALID_ARGUMENT, "NULL argument");
   *ms = h->last_ms;
+  return SFBHIP_OK;
+}
+
+sfbhip_status sfbhip_last_phase_ms(sfbhip_handle *h, float *ms3) {
+  if (!h || !ms3) return fail(SFBHIP_INVALID_ARGUMENT, "NULL argument");
+  for (int i = 0; i < 3; ++i) ms3[i] = h->phase_ms[i];
   return SFBHIP_OK;
 }
 
@@ -626,39 +696,61 @@ sfbhip_status sfbhip_direct(sfbhip_handle *h, const double *d_points, int64_t n_
   if (mixed && p->lmax > MX_LMAX)
     return fail(SFBHIP_NOT_SUPPORTED, "mixed path supports lmax <= " + std::to_string(MX_LMAX) + "; use precision FP64");
   const int lsplit = mixed ? std::min(p->lmax, std::max(0, p->mixed_fp64_below)) : p->lmax;
-  const bool use_table =
-      !mixed && (p->bessel == SFBHIP_BESSEL_REFERENCE || (p->bessel == SFBHIP_BESSEL_AUTO && n_points > 1000));
-  BesselTableRef tab{nullptr, nullptr, nullptr, 0};
-  if (use_table) CHK(ensure_table(h, p, st, &tab));
+  // FP64 kernel's j_l: in FP64 precision the reference's choice (exact or its
+  // linear table); in the mixed path the quintic table (error ~1e-10 of the
+  // amplitude, far below the mixed kernels' own).
+  int use_table = JL_EXACT;
+  if (mixed)
+    use_table = JL_QUINTIC;
+  else if (p->bessel == SFBHIP_BESSEL_REFERENCE || (p->bessel == SFBHIP_BESSEL_AUTO && n_points > 1000))
+    use_table = JL_REFERENCE_TABLE;
+  BesselTableRef tab{};
+  if (use_table == JL_REFERENCE_TABLE) CHK(ensure_table(h, p, st, &tab));
+  if (use_table == JL_QUINTIC && lsplit > 0) CHK(ensure_quintic(h, p, lsplit, rdata, st, &tab));
   HermiteTables ht{nullptr, nullptr, nullptr};
   if (mixed && lsplit < p->lmax) CHK(ensure_hermite(h, p, rdata, st, &ht));
-  DirectPlan pl64, plmx;
-  plan_direct(h->kc.nl, 0, lsplit, n_points, F64_NT, F64_TP, F64_MC, 16, 4096, &pl64);
+  // mixed path: l < min(lsplit, LW_LMAX) in the low-l FP64 kernel, the rest of
+  // l < lsplit in the general FP64 kernel (both with the quintic table)
+  const int llow = mixed ? std::min(lsplit, LW_LMAX) : 0;
+  DirectPlan pllw, pl64, plmx;
+  if (llow > 0) plan_direct(h->kc.nl, 0, llow, n_points, LW_ROWS, LW_STEP, F64_MC, 1, 4096, &pllw);
+  pl64.partial_size = pllw.partial_size;
+  plan_direct(h->kc.nl, llow, lsplit, n_points, F64_NT, F64_TP, F64_MC, 16, 4096, &pl64);
   plmx.partial_size = pl64.partial_size;
   if (mixed) plan_direct(h->kc.nl, lsplit, p->lmax, n_points, MX_NT, MX_TP, MX_MC, 8, 4096, &plmx);
-  std::vector<ReduceGroup> groups = pl64.groups;
+  std::vector<ReduceGroup> groups = pllw.groups;
+  groups.insert(groups.end(), pl64.groups.begin(), pl64.groups.end());
   groups.insert(groups.end(), plmx.groups.begin(), plmx.groups.end());
   double *partial = nullptr;
   CHK(get_buf(h, "partial", (size_t)std::max<int64_t>(plmx.partial_size, 1), &partial));
   std::vector<DirectItem> all;
+  for (auto &v : pllw.items) all.insert(all.end(), v.begin(), v.end());
   for (auto &v : pl64.items) all.insert(all.end(), v.begin(), v.end());
   for (auto &v : plmx.items) all.insert(all.end(), v.begin(), v.end());
   DirectItem *ditems;
   CHK(get_buf(h, "items", std::max<size_t>(all.size(), 1), &ditems));
   if (!all.empty())
     HIPCHK(hipMemcpyAsync(ditems, all.data(), sizeof(DirectItem) * all.size(), hipMemcpyHostToDevice, st));
-  h->last_ms = 0.f;
-  HIPCHK(hipEventRecord(h->ev0, st));
+  clear_timings(h);
+  HIPCHK(hipEventRecord(h->ev[0], st));
   size_t off = 0;
+  if (!pllw.items[0].empty()) {
+    const int n = (int)pllw.items[0].size();
+    sfb_lowl_f64_kernel<<<n, 256, 0, st>>>(pts, ditems + off, dk, p->nmax, tab, partial);
+    HIPCHK(hipGetLastError());
+    off += n;
+  }
   CHK(launch_plan_f64(pl64, ditems, &off, st, pts, nullptr, dk, p->nmax, lt, use_table, tab, partial, derr));
+  HIPCHK(hipEventRecord(h->ev[1], st));
   if (mixed)
     CHK(launch_plan_mixed(h, plmx, ditems, &off, st, pts, dk, p->nmax, lt, ht,
                           p->mixed_operands == SFBHIP_OPERANDS_SPLIT_FP16, partial));
-  HIPCHK(hipEventRecord(h->ev1, st));
+  HIPCHK(hipEventRecord(h->ev[2], st));
   CHK(finalize(h, p, groups, partial, norm, d_k_ln, d_c_ln, d_f_lmn, st));
+  HIPCHK(hipEventRecord(h->ev[3], st));
   int e = 0;
   CHK(read_err(h, st, derr, &e));
-  if (!groups.empty()) HIPCHK(hipEventElapsedTime(&h->last_ms, h->ev0, h->ev1));
+  CHK(read_timings(h, mixed && lsplit < p->lmax ? 1 : 0));
   if (e & ERR_BADPOINT) return fail(SFBHIP_DOMAIN_ERROR, "point with r < 0 or a non-finite coordinate");
   if (e & ERR_DOMAIN) return fail(SFBHIP_DOMAIN_ERROR, "argument (z) outside of defined range.");
   return SFBHIP_OK;
@@ -786,16 +878,18 @@ sfbhip_status sfbhip_healpix(sfbhip_handle *h, const int64_t *d_offsets, const d
   CHK(get_buf(h, "items", std::max<size_t>(all.size(), 1), &ditems));
   if (!all.empty())
     HIPCHK(hipMemcpyAsync(ditems, all.data(), sizeof(DirectItem) * all.size(), hipMemcpyHostToDevice, st));
-  h->last_ms = 0.f;
-  HIPCHK(hipEventRecord(h->ev0, st));
+  clear_timings(h);
+  HIPCHK(hipEventRecord(h->ev[0], st));
   size_t off = 0;
   PointsSoA none{};
   CHK(launch_plan_f64(pl, ditems, &off, st, none, &pix, dk, p->nmax, lt, use_table, tab, partial, derr));
-  HIPCHK(hipEventRecord(h->ev1, st));
+  HIPCHK(hipEventRecord(h->ev[1], st));
+  HIPCHK(hipEventRecord(h->ev[2], st));
   CHK(finalize(h, p, pl.groups, partial, norm, d_k_ln, d_c_ln, d_f_lmn, st));
+  HIPCHK(hipEventRecord(h->ev[3], st));
   int e = 0;
   CHK(read_err(h, st, derr, &e));
-  if (!pl.groups.empty()) HIPCHK(hipEventElapsedTime(&h->last_ms, h->ev0, h->ev1));
+  CHK(read_timings(h, 0));
   if (e & ERR_DOMAIN) return fail(SFBHIP_DOMAIN_ERROR, "argument (z) outside of defined range.");
   return SFBHIP_OK;
 }

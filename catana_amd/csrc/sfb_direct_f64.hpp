@@ -34,17 +34,79 @@ constexpr int F64_TP = 128;          // points per tile
 constexpr int F64_MC = 16;           // m values per chunk (32 real columns)
 constexpr int F64_LDP = F64_TP + 1;  // LDS row pitch (doubles)
 
+enum { JL_EXACT = 0, JL_REFERENCE_TABLE = 1, JL_QUINTIC = 2 };
+
+// Quintic Hermite spacing of the FP64 j_l table of the mixed path's low
+// multipoles: interpolation error <= h^6/46080 max|j_l^(6)| ~ 6e-11 of the
+// amplitude at h = 1/8.
+constexpr double Q5_H = 0.125;
+constexpr double Q5_HINV = 8.0;
+
 struct BesselTableRef {
   const double *y;       // [lmax][P] values of j_l at i*dx
   const double *zmax;    // [lmax]
   const double *dx_inv;  // [lmax]
   int32_t P;             // nodes per l
+  // quintic table: per l, nint[l] intervals [i h, (i+1) h] of 6 coefficients in t = x/h - i, from off[l]
+  const double *q5;
+  const int32_t *q5_off;
+  const int32_t *q5_nint;
 };
 
-// j_l(z): exact (gsl_sf_bessel_jl semantics) or the reference's linear table
-// (SBesselInterpolator.cpp:20-25, domain error past z_max).
+// j_l, j_l' and j_l'' at x (FP64).  j_l' = (l/x) j_l - j_{l+1}; j_l'' from
+// the differential equation x^2 j'' + 2x j' + (x^2 - l(l+1)) j = 0.
+__device__ inline void jl_d012(int l, double x, double &f, double &d, double &s) {
+  if (x == 0.0) {
+    f = l == 0 ? 1.0 : 0.0;
+    d = l == 1 ? 1.0 / 3.0 : 0.0;
+    s = l == 0 ? -1.0 / 3.0 : (l == 2 ? 2.0 / 15.0 : 0.0);
+    return;
+  }
+  f = sbessel_jl(l, x);
+  const double ix = 1.0 / x;
+  d = l * ix * f - sbessel_jl(l + 1, x);
+  s = -2.0 * ix * d - (1.0 - l * (l + 1.0) * ix * ix) * f;
+}
+
+// One thread per interval: the quintic Hermite polynomial matching j, h j',
+// h^2 j'' at both ends, as coefficients of t^0 .. t^5.
+__global__ void quintic_table_kernel(int lmax, const int32_t *__restrict__ off, const int32_t *__restrict__ nint,
+                                     double *__restrict__ c) {
+  const int l = blockIdx.y;
+  if (l >= lmax) return;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nint[l]; i += gridDim.x * blockDim.x) {
+    double f0, d0, s0, f1, d1, s1;
+    jl_d012(l, i * Q5_H, f0, d0, s0);
+    jl_d012(l, (i + 1) * Q5_H, f1, d1, s1);
+    d0 *= Q5_H;
+    d1 *= Q5_H;
+    s0 *= Q5_H * Q5_H;
+    s1 *= Q5_H * Q5_H;
+    const double df = f1 - f0;
+    double *o = c + 6 * ((size_t)off[l] + i);
+    o[0] = f0;
+    o[1] = d0;
+    o[2] = 0.5 * s0;
+    o[3] = 10.0 * df - 6.0 * d0 - 4.0 * d1 - 1.5 * s0 + 0.5 * s1;
+    o[4] = -15.0 * df + 8.0 * d0 + 7.0 * d1 + 1.5 * s0 - s1;
+    o[5] = 6.0 * df - 3.0 * d0 - 3.0 * d1 - 0.5 * s0 + 0.5 * s1;
+  }
+}
+
+// j_l(z): exact (gsl_sf_bessel_jl semantics), the reference's linear table
+// (SBesselInterpolator.cpp:20-25, domain error past z_max), or the quintic table.
 __device__ __forceinline__ double bessel_f64(int l, double z, int use_table, const BesselTableRef &tab, int &err) {
-  if (!use_table) return sbessel_jl(l, z);
+  if (use_table == JL_EXACT) return sbessel_jl(l, z);
+  if (use_table == JL_QUINTIC) {
+    const double u = z * Q5_HINV;
+    int i = (int)u;
+    const int last = tab.q5_nint[l] - 1;
+    i = i < last ? i : last;
+    const double t = u - (double)i;
+    const double2 *cc = reinterpret_cast<const double2 *>(tab.q5 + 6 * ((size_t)tab.q5_off[l] + i));
+    const double2 a = cc[0], b = cc[1], e = cc[2];
+    return fma(t, fma(t, fma(t, fma(t, fma(t, e.y, e.x), b.y), b.x), a.y), a.x);
+  }
   if (!(z < tab.zmax[l])) {
     err |= ERR_DOMAIN;
     return 0.0;

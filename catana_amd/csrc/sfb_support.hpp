@@ -60,15 +60,15 @@ __global__ void bessel_table_kernel(int lmax, int P, const double *__restrict__ 
 }
 
 // Sum the partial blocks of a group in split order and scale by norm * k_ln:
-// f[l][m][n] (complex, m >= 0, n < nmax).  One block per group.
+// f[l][m][n] (complex, m >= 0, n < nmax).  Block (g, c) sums elements c, c + gridDim.y*256, ... of group g.
 __global__ void reduce_partials_kernel(const ReduceGroup *__restrict__ groups, const double *__restrict__ partial,
-                                       int row_pitch_rows, const double *__restrict__ kln, int nmax,
+                                       const double *__restrict__ kln, int nmax,
                                        const int64_t *__restrict__ flmn_off, double norm,
                                        double *__restrict__ flmn) {
   const ReduceGroup g = groups[blockIdx.x];
   const int ncol = 2 * (g.l + 1);
   const int total = g.nrows * ncol;
-  for (int e = threadIdx.x; e < total; e += blockDim.x) {
+  for (int e = blockIdx.y * blockDim.x + threadIdx.x; e < total; e += gridDim.y * blockDim.x) {
     const int row = e / ncol, col = e - row * ncol;
     const double *src = partial + g.base + (size_t)row * ncol + col;
     double acc = 0.0;
@@ -78,7 +78,6 @@ __global__ void reduce_partials_kernel(const ReduceGroup *__restrict__ groups, c
     const double k = kln[(size_t)g.l * nmax + n];
     flmn[(flmn_off[g.l] + (size_t)m * nmax + n) * 2 + (col & 1)] = acc * norm * k;
   }
-  (void)row_pitch_rows;
 }
 
 // c_ln = (|f_l0n|^2 + 2 sum_{m>=1} |f_lmn|^2) / (2l+1)   (sfb_raw.cpp:91-96);

@@ -122,9 +122,15 @@ sfbhip_status sfbhip_healpix(sfbhip_handle *h, const int64_t *d_offsets, const d
                              int32_t nside, const sfbhip_params *p, double *d_k_ln, double *d_c_ln,
                              double *d_f_lmn, void *stream);
 
-/* Duration in ms of the dominant kernel of the last call on this handle,
- * measured with HIP events on the call's stream (0 if none). */
+/* Duration in ms of the dominant kernel of the last call on this handle
+ * (the mixed contraction kernels when the call ran any, else the FP64
+ * contraction kernels), measured with HIP events on the call's stream. */
 sfbhip_status sfbhip_last_kernel_ms(sfbhip_handle *h, float *ms);
+
+/* Durations in ms of the three phases of the last call, from HIP events on
+ * its stream: [0] FP64 contraction kernels, [1] mixed contraction kernels,
+ * [2] reduction of partial blocks and C_l assembly. */
+sfbhip_status sfbhip_last_phase_ms(sfbhip_handle *h, float *ms3);
 
 /* Message of the last error on the calling thread ("" if none). */
 const char *sfbhip_last_error(void);

@@ -119,3 +119,17 @@ def test_mixed_vs_fp64_clustered_high_l(handle):
     print("max rel c_ln", rel.max(), "median", np.median(rel))
     assert rel.max() < 1e-6
     assert _flmn_err(fm, f64) < 1e-6
+
+
+def test_mixed_fp64_multipoles_quintic_table(handle):
+    # mixed_fp64_below = lmax: every l in the FP64 kernel with the quintic j_l
+    # table (interpolation error ~1e-10 of the amplitude); points up to 1.2 rmax
+    pts = workloads.clustered_ball(400, 600.0, mean_children=30, sigma=8.0, seed=25)
+    print(f"case: FP64 kernel with quintic table, N={pts.shape[0]} lmax=12 k_max=0.3 rmax=500", flush=True)
+    kw = dict(k_max=0.3, bessel=capi.BESSEL_EXACT)
+    _, c64, f64 = _run(handle, pts, 12, 60, 500.0, precision=capi.PREC_FP64, **kw)
+    _, cq, fq = _run(handle, pts, 12, 60, 500.0, precision=capi.PREC_MIXED, mixed_fp64_below=12, **kw)
+    rel = _rel(cq, c64)
+    print("max rel c_ln", rel.max(), "f_lmn", _flmn_err(fq, f64))
+    assert rel.max() < 1e-7
+    assert _flmn_err(fq, f64) < 1e-8
