@@ -74,8 +74,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="clustered1m", choices=sorted(WORKLOADS))
     ap.add_argument("--precision", default="mixed", choices=["mixed", "fp64"])
-    ap.add_argument("--operands", default="split16", choices=["fp32", "split16"],
+    ap.add_argument("--operands", default="split16", choices=["fp32", "split16", "split3"],
                     help="matrix-core operand format of the mixed path")
+    ap.add_argument("--chunks", type=int, default=4, choices=[1, 2, 4, 8],
+                    help="mixed path: m chunks accumulated per work item")
     ap.add_argument("--fp64-below", type=int, default=8, help="mixed path: l below this run in FP64")
     ap.add_argument("--cpu-sample", type=int, default=65536)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -100,8 +102,9 @@ def main():
     d_pts = torch.from_numpy(np.ascontiguousarray(pts)).to(dev)
     h = capi.Handle(0)
     prec = capi.PREC_MIXED if args.precision == "mixed" else capi.PREC_FP64
+    ops = {"fp32": capi.OPERANDS_FP32, "split16": capi.OPERANDS_SPLIT_FP16, "split3": capi.OPERANDS_SPLIT3_FP16}
     kw = dict(k_max=cfg["k_max"], precision=prec, bessel=capi.BESSEL_EXACT, mixed_fp64_below=args.fp64_below,
-              mixed_operands=capi.OPERANDS_SPLIT_FP16 if args.operands == "split16" else capi.OPERANDS_FP32)
+              mixed_operands=ops[args.operands], mixed_chunks=args.chunks)
     p = capi.make_params(cfg["lmax"], cfg["nmax"], cfg["rmax"], **kw)
     _, nl = h.kln(p)
     l = np.arange(cfg["lmax"])
@@ -135,7 +138,9 @@ def main():
     if prec == capi.PREC_FP64:
         dtype, peak = "f64", PEAK_TFLOPS["fp64"]
     elif args.operands == "split16":
-        dtype, peak = "f16x3 (split FP16 operands, f32 accumulate, f64 tile sums)", PEAK_TFLOPS["fp16"] / 3.0
+        dtype, peak = "f16 2-piece split, 3 products (f32 accumulate, f64 tile sums)", PEAK_TFLOPS["fp16"] / 3.0
+    elif args.operands == "split3":
+        dtype, peak = "f16 3-piece split, 6 products (f32 accumulate, f64 tile sums)", PEAK_TFLOPS["fp16"] / 6.0
     else:
         dtype, peak = "f32 (f32 operands and accumulate, f64 tile sums)", PEAK_TFLOPS["fp32"]
     achieved = flops_dom / (kms * 1e-3) / 1e12
@@ -146,6 +151,7 @@ def main():
         "data": "synthetic, seeded (catana_amd/workloads.py), inputs resident in HBM",
         "config": {"workload": args.workload, "N": int(N), "lmax": cfg["lmax"], "k_max": cfg["k_max"],
                    "rmax": cfg["rmax"], "precision": args.precision, "operands": args.operands,
+                   "chunks": args.chunks,
                    "fp64_below": args.fp64_below, "pairs_per_step": pairs,
                    "wall_s_timed_region": wall},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",

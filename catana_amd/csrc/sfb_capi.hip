@@ -63,11 +63,9 @@ struct KlnCache {
   std::vector<int32_t> nl;
   bool table_valid = false;   // reference interpolation table on the device
   bool herm_valid = false;    // Hermite tables of the mixed path on the device
-  std::vector<int32_t> herm_off, herm_nn;
   bool q5_valid = false;      // quintic FP64 tables of the low multipoles on the device
   int q5_lmax = 0;
   double q5_R = 0.0;
-  int32_t herm_max = 0;
   double herm_R = 0.0;        // radius the tables cover: k_{l,n_l-1} * herm_R is the last argument
 };
 
@@ -121,7 +119,11 @@ sfbhip_status validate(const sfbhip_params *p) {
   if (p->bessel < 0 || p->bessel > 2) return fail(SFBHIP_INVALID_ARGUMENT, "unknown bessel mode");
   if (p->zeros != SFBHIP_ZEROS_REFERENCE && p->zeros != SFBHIP_ZEROS_CONVERGED)
     return fail(SFBHIP_INVALID_ARGUMENT, "unknown zeros mode");
-  if (p->mixed_operands != SFBHIP_OPERANDS_FP32 && p->mixed_operands != SFBHIP_OPERANDS_SPLIT_FP16)
+  if (p->mixed_chunks != 0 && p->mixed_chunks != 1 && p->mixed_chunks != 2 && p->mixed_chunks != 4 &&
+      p->mixed_chunks != 8)
+    return fail(SFBHIP_INVALID_ARGUMENT, "mixed_chunks must be 0, 1, 2, 4 or 8");
+  if (p->mixed_operands != SFBHIP_OPERANDS_FP32 && p->mixed_operands != SFBHIP_OPERANDS_SPLIT_FP16 &&
+      p->mixed_operands != SFBHIP_OPERANDS_SPLIT3_FP16)
     return fail(SFBHIP_INVALID_ARGUMENT, "unknown mixed operand format");
   return SFBHIP_OK;
 }
@@ -370,6 +372,38 @@ sfbhip_status prep_points(sfbhip_handle *h, const double *d_points, int64_t n, h
   return SFBHIP_OK;
 }
 
+// The point table reordered by radius (stable), so that consecutive points
+// have close Bessel arguments k r: the table loads of a tile then stay within
+// a few cache lines.  The sum over points does not depend on the order beyond
+// rounding.
+sfbhip_status sort_points_by_radius(sfbhip_handle *h, const PointsSoA &in, hipStream_t st, PointsSoA *out) {
+  const int64_t n = in.n;
+  if (n < 2 || n > INT32_MAX) {
+    *out = in;
+    return SFBHIP_OK;
+  }
+  unsigned long long *k0, *k1;
+  int32_t *i0, *i1;
+  double *buf;
+  CHK(get_buf(h, "sort_k0", (size_t)n, &k0));
+  CHK(get_buf(h, "sort_k1", (size_t)n, &k1));
+  CHK(get_buf(h, "sort_i0", (size_t)n, &i0));
+  CHK(get_buf(h, "sort_i1", (size_t)n, &i1));
+  CHK(get_buf(h, "soa_sorted", (size_t)6 * n, &buf));
+  radius_keys_kernel<<<cdiv(n, 256), 256, 0, st>>>(in.r, n, k0, i0);
+  HIPCHK(hipGetLastError());
+  size_t tmp_bytes = 0;
+  HIPCHK(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_bytes, k0, k1, i0, i1, (int)n, 0, 64, st));
+  char *tmp;
+  CHK(get_buf(h, "sort_tmp", tmp_bytes, &tmp));
+  HIPCHK(hipcub::DeviceRadixSort::SortPairs(tmp, tmp_bytes, k0, k1, i0, i1, (int)n, 0, 64, st));
+  double *r = buf, *x = buf + n, *s = buf + 2 * n, *ph = buf + 3 * n, *cp = buf + 4 * n, *sp = buf + 5 * n;
+  gather_points_kernel<<<cdiv(n, 256), 256, 0, st>>>(in, i1, n, r, x, s, ph, cp, sp);
+  HIPCHK(hipGetLastError());
+  *out = PointsSoA{r, x, s, ph, cp, sp, n};
+  return SFBHIP_OK;
+}
+
 // f_lmn buffer (the caller's or the workspace), zeroed; reduce; c_ln; k_ln copy.
 sfbhip_status finalize(sfbhip_handle *h, const sfbhip_params *p, const std::vector<ReduceGroup> &groups,
                        const double *partial, double norm, double *d_k_ln, double *d_c_ln, double *d_f_lmn,
@@ -453,77 +487,74 @@ sfbhip_status ensure_quintic(sfbhip_handle *h, const sfbhip_params *p, int L, do
   return SFBHIP_OK;
 }
 
-// Hermite tables of the mixed path: nodes i*h, i < nn_l = floor(k_{l,n_l-1} R / h) + 3,
-// R = max(rmax, largest radius of the catalogue), so that no argument leaves the table.
+// Hermite tables of the mixed path: intervals [i h, (i+1) h], i < nint_l =
+// floor(k_{l,n_l-1} R / h) + 2, R = max(rmax, largest radius of the catalogue),
+// so that no argument leaves the table.
 sfbhip_status ensure_hermite(sfbhip_handle *h, const sfbhip_params *p, double rdata, hipStream_t st,
                              HermiteTables *ht) {
   KlnCache &c = h->kc;
   const int L = p->lmax;
   const double R = std::max(p->rmax, rdata);
+  float4 *tab;
+  int32_t *doff, *dni;
   if (!c.herm_valid || c.herm_R < R) {
-    c.herm_off.assign(std::max(L, 1), 0);
-    c.herm_nn.assign(std::max(L, 1), 2);
-    int32_t o = 0, mx = 2;
+    std::vector<int32_t> off(std::max(L, 1), 0), ni(std::max(L, 1), 1);
+    int64_t o = 0;
+    int32_t mx = 1;
     for (int l = 0; l < L; ++l) {
-      int32_t nn = 2;
-      if (c.nl[l] > 0) {
-        const double X = c.kln[(size_t)l * p->nmax + c.nl[l] - 1] * R;
-        nn = (int32_t)std::floor(X / MX_H) + 3;
-      }
-      if (nn > MX_TAB_MAX)
-        return fail(SFBHIP_NOT_SUPPORTED, "mixed path: largest Bessel argument k r exceeds the LDS table (" +
-                                              std::to_string(MX_TAB_MAX * MX_H) + "); use precision FP64");
-      c.herm_off[l] = o;
-      c.herm_nn[l] = nn;
-      o += nn;
-      mx = std::max(mx, nn);
+      int64_t n = 1;
+      if (c.nl[l] > 0) n = (int64_t)std::floor(c.kln[(size_t)l * p->nmax + c.nl[l] - 1] * R / MX_H) + 2;
+      if (n > MX_TAB_MAX)
+        return fail(SFBHIP_NOT_SUPPORTED, "mixed path: largest Bessel argument k r exceeds " +
+                                              std::to_string(MX_TAB_MAX * MX_H) + "; use precision FP64");
+      off[l] = (int32_t)o;
+      ni[l] = (int32_t)n;
+      o += n;
+      mx = std::max<int32_t>(mx, (int32_t)n);
+      if (o > INT32_MAX) return fail(SFBHIP_NOT_SUPPORTED, "mixed path: Bessel tables too large");
     }
-    float2 *tab;
-    int32_t *doff, *dnn;
-    CHK(get_buf(h, "herm_tab", (size_t)std::max(o, 1), &tab));
-    CHK(get_buf(h, "herm_off", c.herm_off.size(), &doff));
-    CHK(get_buf(h, "herm_nn", c.herm_nn.size(), &dnn));
-    HIPCHK(hipMemcpyAsync(doff, c.herm_off.data(), sizeof(int32_t) * c.herm_off.size(), hipMemcpyHostToDevice, st));
-    HIPCHK(hipMemcpyAsync(dnn, c.herm_nn.data(), sizeof(int32_t) * c.herm_nn.size(), hipMemcpyHostToDevice, st));
+    CHK(get_buf(h, "herm_tab", (size_t)std::max<int64_t>(o, 1), &tab));
+    CHK(get_buf(h, "herm_off", off.size(), &doff));
+    CHK(get_buf(h, "herm_nint", ni.size(), &dni));
+    HIPCHK(hipMemcpyAsync(doff, off.data(), sizeof(int32_t) * off.size(), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(dni, ni.data(), sizeof(int32_t) * ni.size(), hipMemcpyHostToDevice, st));
     if (L > 0) {
       dim3 grid((unsigned)cdiv(mx, 256), (unsigned)L);
-      hermite_table_kernel<<<grid, 256, 0, st>>>(L, doff, dnn, tab);
+      hermite_table_kernel<<<grid, 256, 0, st>>>(L, doff, dni, tab);
       HIPCHK(hipGetLastError());
     }
     HIPCHK(hipStreamSynchronize(st));
-    c.herm_max = mx;
     c.herm_R = R;
     c.herm_valid = true;
   }
-  float2 *tab;
-  int32_t *doff, *dnn;
   CHK(get_buf(h, "herm_tab", 1, &tab));
   CHK(get_buf(h, "herm_off", 1, &doff));
-  CHK(get_buf(h, "herm_nn", 1, &dnn));
+  CHK(get_buf(h, "herm_nint", 1, &dni));
   ht->tab = tab;
   ht->off = doff;
-  ht->nnodes = dnn;
+  ht->nint = dni;
   return SFBHIP_OK;
 }
 
-template <int QMAX, bool F16X3>
+template <int QMAX, int NP>
 sfbhip_status launch_mixed(int nitems, const DirectItem *items, hipStream_t st, const PointsSoA &pts,
                            const double *kln, int nmax, const LegendreTables &lt, const HermiteTables &ht,
-                           int lds_nodes, double *partial) {
-  const size_t shm = sizeof(float2) * (size_t)lds_nodes;
-  HIPCHK(hipFuncSetAttribute(reinterpret_cast<const void *>(&sfb_direct_mixed_kernel<QMAX, F16X3>),
-                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
-  sfb_direct_mixed_kernel<QMAX, F16X3><<<nitems, 256, shm, st>>>(pts, items, kln, nmax, lt, ht, partial);
+                           double *partial) {
+  sfb_direct_mixed_kernel<QMAX, NP><<<nitems, 256, 0, st>>>(pts, items, kln, nmax, lt, ht, partial);
   HIPCHK(hipGetLastError());
   return SFBHIP_OK;
 }
 
+// operand format SFBHIP_OPERANDS_* -> number of FP16 pieces (0: FP32 operands)
 template <int QMAX>
-sfbhip_status launch_mixed_ops(bool f16, int nitems, const DirectItem *items, hipStream_t st, const PointsSoA &pts,
+sfbhip_status launch_mixed_ops(int ops, int nitems, const DirectItem *items, hipStream_t st, const PointsSoA &pts,
                                const double *kln, int nmax, const LegendreTables &lt, const HermiteTables &ht,
-                               int lds_nodes, double *partial) {
-  if (f16) return launch_mixed<QMAX, true>(nitems, items, st, pts, kln, nmax, lt, ht, lds_nodes, partial);
-  return launch_mixed<QMAX, false>(nitems, items, st, pts, kln, nmax, lt, ht, lds_nodes, partial);
+                               double *partial) {
+  if (ops == SFBHIP_OPERANDS_SPLIT_FP16)
+    return launch_mixed<QMAX, 2>(nitems, items, st, pts, kln, nmax, lt, ht, partial);
+  if (ops == SFBHIP_OPERANDS_SPLIT3_FP16)
+    return launch_mixed<QMAX, 3>(nitems, items, st, pts, kln, nmax, lt, ht, partial);
+  return launch_mixed<QMAX, 0>(nitems, items, st, pts, kln, nmax, lt, ht, partial);
 }
 
 // Launch every class of a plan; items[] on the device holds the classes back to back from `*off`.
@@ -548,20 +579,18 @@ sfbhip_status launch_plan_f64(const DirectPlan &pl, DirectItem *ditems, size_t *
   return SFBHIP_OK;
 }
 
-sfbhip_status launch_plan_mixed(sfbhip_handle *h, const DirectPlan &pl, DirectItem *ditems, size_t *off,
+sfbhip_status launch_plan_mixed(const DirectPlan &pl, DirectItem *ditems, size_t *off,
                                 hipStream_t st, const PointsSoA &pts, const double *dk, int nmax,
-                                const LegendreTables &lt, const HermiteTables &ht, bool f16, double *partial) {
+                                const LegendreTables &lt, const HermiteTables &ht, int ops, double *partial) {
   for (int ci = 0; ci < 4; ++ci) {
     const int n = (int)pl.items[ci].size();
     if (!n) continue;
-    int nodes = 2;
-    for (const DirectItem &it : pl.items[ci]) nodes = std::max(nodes, h->kc.herm_nn[it.l]);
     const DirectItem *di = ditems + *off;
     switch (ci) {
-      case 0: CHK(launch_mixed_ops<1>(f16, n, di, st, pts, dk, nmax, lt, ht, nodes, partial)); break;
-      case 1: CHK(launch_mixed_ops<2>(f16, n, di, st, pts, dk, nmax, lt, ht, nodes, partial)); break;
-      case 2: CHK(launch_mixed_ops<4>(f16, n, di, st, pts, dk, nmax, lt, ht, nodes, partial)); break;
-      default: CHK(launch_mixed_ops<8>(f16, n, di, st, pts, dk, nmax, lt, ht, nodes, partial)); break;
+      case 0: CHK(launch_mixed_ops<1>(ops, n, di, st, pts, dk, nmax, lt, ht, partial)); break;
+      case 1: CHK(launch_mixed_ops<2>(ops, n, di, st, pts, dk, nmax, lt, ht, partial)); break;
+      case 2: CHK(launch_mixed_ops<4>(ops, n, di, st, pts, dk, nmax, lt, ht, partial)); break;
+      default: CHK(launch_mixed_ops<8>(ops, n, di, st, pts, dk, nmax, lt, ht, partial)); break;
     }
     *off += n;
   }
@@ -600,6 +629,7 @@ sfbhip_status sfbhip_default_params(sfbhip_params *p) {
   p->verbose = 0;
   p->mixed_fp64_below = 8;
   p->mixed_operands = SFBHIP_OPERANDS_SPLIT_FP16;
+  p->mixed_chunks = 4;
   return SFBHIP_OK;
 }
 
@@ -691,10 +721,11 @@ sfbhip_status sfbhip_direct(sfbhip_handle *h, const double *d_points, int64_t n_
   CHK(get_buf(h, "kln", (size_t)std::max(p->lmax, 1) * p->nmax, &dk));
 
   // Multipoles l < lsplit run in the FP64 kernel; the rest in the mixed kernel
-  // when precision is MIXED.
+  // when precision is MIXED.  The mixed path reads the catalogue sorted by radius.
   const bool mixed = p->precision == SFBHIP_PREC_MIXED;
   if (mixed && p->lmax > MX_LMAX)
     return fail(SFBHIP_NOT_SUPPORTED, "mixed path supports lmax <= " + std::to_string(MX_LMAX) + "; use precision FP64");
+  if (mixed) CHK(sort_points_by_radius(h, pts, st, &pts));
   const int lsplit = mixed ? std::min(p->lmax, std::max(0, p->mixed_fp64_below)) : p->lmax;
   // FP64 kernel's j_l: in FP64 precision the reference's choice (exact or its
   // linear table); in the mixed path the quintic table (error ~1e-10 of the
@@ -717,7 +748,9 @@ sfbhip_status sfbhip_direct(sfbhip_handle *h, const double *d_points, int64_t n_
   pl64.partial_size = pllw.partial_size;
   plan_direct(h->kc.nl, llow, lsplit, n_points, F64_NT, F64_TP, F64_MC, 16, 4096, &pl64);
   plmx.partial_size = pl64.partial_size;
-  if (mixed) plan_direct(h->kc.nl, lsplit, p->lmax, n_points, MX_NT, MX_TP, MX_MC, 8, 4096, &plmx);
+  if (mixed)
+    plan_direct(h->kc.nl, lsplit, p->lmax, n_points, MX_NT, MX_TP, MX_MC, p->mixed_chunks ? p->mixed_chunks : 4,
+                4096, &plmx);
   std::vector<ReduceGroup> groups = pllw.groups;
   groups.insert(groups.end(), pl64.groups.begin(), pl64.groups.end());
   groups.insert(groups.end(), plmx.groups.begin(), plmx.groups.end());
@@ -743,8 +776,8 @@ sfbhip_status sfbhip_direct(sfbhip_handle *h, const double *d_points, int64_t n_
   CHK(launch_plan_f64(pl64, ditems, &off, st, pts, nullptr, dk, p->nmax, lt, use_table, tab, partial, derr));
   HIPCHK(hipEventRecord(h->ev[1], st));
   if (mixed)
-    CHK(launch_plan_mixed(h, plmx, ditems, &off, st, pts, dk, p->nmax, lt, ht,
-                          p->mixed_operands == SFBHIP_OPERANDS_SPLIT_FP16, partial));
+    CHK(launch_plan_mixed(plmx, ditems, &off, st, pts, dk, p->nmax, lt, ht,
+                          p->mixed_operands, partial));
   HIPCHK(hipEventRecord(h->ev[2], st));
   CHK(finalize(h, p, groups, partial, norm, d_k_ln, d_c_ln, d_f_lmn, st));
   HIPCHK(hipEventRecord(h->ev[3], st));

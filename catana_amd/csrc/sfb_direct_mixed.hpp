@@ -2,30 +2,40 @@
 //
 // Same contraction as the FP64 kernel (per l: Bessel rows x Y* columns summed
 // over points, sfb_raw.cpp:61-80), with the operands generated cheaply:
-//   * j_l(k_ln r) from a per-l table of (j_l(x_i), h j_l'(x_i)) on the grid
-//     x_i = i h (h = 1/16) held in LDS as FP32 and evaluated by cubic Hermite
-//     interpolation (error <= h^4/384 max|j_l''''| ~ 4e-8 of the local
-//     amplitude).  The argument t = r k_ln / h and its split into node index
+//   * j_l(k_ln r) by cubic Hermite interpolation of (j_l(x_i), h j_l'(x_i)) on
+//     the grid x_i = i h (h = 1/16), FP32 (error <= h^4/384 max|j_l''''| ~ 4e-8
+//     of the local amplitude).  The table lives in global memory, one float4
+//     (j_i, h j_i', j_{i+1}, h j_{i+1}') per interval, so one 16-byte load per
+//     value; the points are sorted by radius beforehand, so that the lanes of
+//     a row touch a few neighbouring intervals and the loads hit the L1/L2
+//     caches.  The argument t = r k_ln / h and its split into interval index
 //     and fraction are formed in FP64, so the phase error does not grow with
 //     k r.  The table of l reaches k_{l,n_l-1} max(rmax, max_i r_i), so every
 //     argument of the catalogue lies inside it.
-//   * Y*_lm from the same FP64 downward-m recurrence as the FP64 kernel.
+//   * Y*_lm from the same FP64 downward-m recurrence as the FP64 kernel; the
+//     recurrence coefficients are read with wave-uniform (scalar) loads.
 //   * each wave evaluates its Bessel values straight into MFMA A-operand
 //     registers (it owns 16 rows of n, so every value is computed once and
 //     never staged); the Y* chunk is staged in LDS as the B operand.
-//   * contraction on the matrix cores, in one of two operand formats:
-//       F16X3 = false: FP32 operands, v_mfma_f32_16x16x4f32;
-//       F16X3 = true : each operand split as v*2^12 = hi + lo (two FP16
-//                      values, relative residual 2^-22) and
-//                      hi*hi + hi*lo + lo*hi on v_mfma_f32_16x16x32_f16.
-//     Each 256-point tile is summed in FP32 and then added into FP64
-//     accumulators, so FP32 rounding never sees more than 256 terms.
+//   * contraction on the matrix cores, in one of three operand formats
+//     (NP = number of FP16 pieces per operand):
+//       NP = 0: FP32 operands, v_mfma_f32_16x16x4f32;
+//       NP = 2: each operand split as v*2^12 = p0 + p1 (two FP16 values,
+//               22 bits) and p0*p0 + p0*p1 + p1*p0 on v_mfma_f32_16x16x32_f16
+//               (relative error per product ~2^-22);
+//       NP = 3: three FP16 pieces (33 bits, exact for the FP32 values) and the
+//               six products of order <= 2^-22 -- p0p0, p0p1, p1p0, p0p2, p1p1,
+//               p2p0 -- so that a product is as accurate as with FP32 operands.
+//     FP32 partial sums are added into FP64 accumulators every 64 points
+//     (FP32 rounding in the matrix-core accumulation is the largest error
+//     term otherwise); with FP16 pieces the leading products and the
+//     correction products accumulate separately, so only the leading chain
+//     needs the frequent fold.
 // Work item: (l, 64 rows of n, m chunks [qlo, qhi), a point range).  Chunks
 // are counted from the top: chunk c holds m = l - 16c - mm, mm = 0..15, so
 // only the lowest chunk is partial.  All 256 threads generate the Legendre rows
 // (one point each); wave w owns rows 16w..16w+15 and both 16-column halves of
-// every 32-column chunk.  Waves and column halves that only cover padding skip
-// their MFMAs.
+// every 32-column chunk.  Waves whose rows are all padding skip their MFMAs.
 #pragma once
 
 #include "sfb_common.hpp"
@@ -39,9 +49,8 @@ constexpr int MX_TP = 256;           // points per tile
 constexpr int MX_MC = 16;            // m values per chunk (32 real columns)
 constexpr double MX_H = 1.0 / 16.0;  // Bessel table spacing
 constexpr float MX_SCALE = 4096.f;   // FP16 operands carry v * 2^12
-constexpr int MX_LMAX = 512;         // recurrence coefficients of l < MX_LMAX fit in LDS
-// LDS: Y* chunk (<= 2 x 32 x 288 halves = 36 KB) + radii (2 KB) + coefficients (8 KB) + table
-constexpr int MX_TAB_MAX = 14336;    // table nodes (x up to 896)
+constexpr int MX_TAB_MAX = 1 << 24;  // table intervals per l (x up to 2^20)
+constexpr int MX_LMAX = 512;         // recurrence coefficients of l < MX_LMAX fit in LDS (8 KB)
 
 typedef _Float16 h8 __attribute__((ext_vector_type(8)));
 typedef _Float16 h2 __attribute__((ext_vector_type(2)));
@@ -55,91 +64,119 @@ __device__ __forceinline__ void split_pair(float a, float b, h2 &hi, h2 &lo) {
   lo = __builtin_convertvector((f2){a - back.x, b - back.y}, h2);
 }
 
-// Per-l Hermite tables: node i of l at tab[off[l] + i] = (j_l(i h), h j_l'(i h)).
+// (a, b) -> three FP16 pairs with a = p0.x + p1.x + p2.x (and b likewise) to
+// 33 bits, i.e. exactly for FP32 inputs away from the FP16 range limits; the
+// remainders after each rounding are exact in FP32.
+__device__ __forceinline__ void split3_pair(float a, float b, h2 &p0, h2 &p1, h2 &p2) {
+  p0 = __builtin_convertvector((f2){a, b}, h2);
+  f2 back = __builtin_convertvector(p0, f2);
+  const f2 r1 = (f2){a - back.x, b - back.y};
+  p1 = __builtin_convertvector(r1, h2);
+  back = __builtin_convertvector(p1, f2);
+  p2 = __builtin_convertvector((f2){r1.x - back.x, r1.y - back.y}, h2);
+}
+
+// Per-l Hermite tables: interval i of l at tab[off[l] + i] =
+// (j_l(i h), h j_l'(i h), j_l((i+1) h), h j_l'((i+1) h)), i < nint[l].
 struct HermiteTables {
-  const float2 *tab;
+  const float4 *tab;
   const int32_t *off;
-  const int32_t *nnodes;
+  const int32_t *nint;
 };
 
-// j_l and h*j_l' at x = i*h, FP64, rounded to FP32.
-__global__ void hermite_table_kernel(int lmax, const int32_t *__restrict__ off, const int32_t *__restrict__ nn,
-                                     float2 *__restrict__ tab) {
+__device__ inline float2 hermite_node(int l, int i) {
+  const double x = i * MX_H;
+  double j, dj;
+  if (x == 0.0) {
+    j = (l == 0) ? 1.0 : 0.0;
+    dj = (l == 1) ? 1.0 / 3.0 : 0.0;
+  } else {
+    j = sbessel_jl(l, x);
+    dj = (double)l / x * j - sbessel_jl(l + 1, x);
+  }
+  return make_float2((float)j, (float)(MX_H * dj));
+}
+
+// j_l and h*j_l' at both ends of each interval, FP64 rounded to FP32.
+__global__ void hermite_table_kernel(int lmax, const int32_t *__restrict__ off, const int32_t *__restrict__ nint,
+                                     float4 *__restrict__ tab) {
   const int l = blockIdx.y;
   if (l >= lmax) return;
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nn[l]; i += gridDim.x * blockDim.x) {
-    const double x = i * MX_H;
-    double j, dj;
-    if (x == 0.0) {
-      j = (l == 0) ? 1.0 : 0.0;
-      dj = (l == 1) ? 1.0 / 3.0 : 0.0;
-    } else {
-      j = sbessel_jl(l, x);
-      dj = (double)l / x * j - sbessel_jl(l + 1, x);
-    }
-    tab[off[l] + i] = make_float2((float)j, (float)(MX_H * dj));
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nint[l]; i += gridDim.x * blockDim.x) {
+    const float2 a = hermite_node(l, i), b = hermite_node(l, i + 1);
+    tab[off[l] + i] = make_float4(a.x, a.y, b.x, b.y);
   }
 }
 
-// j_l(t h) from the LDS table (0 <= t < nn - 1 in units of h; the index is
+// j_l(t h) from the table of l (0 <= t <= nint in units of h; the index is
 // clamped so that a rounding at the last node stays inside the table).
-__device__ __forceinline__ float jl_hermite(const float2 *T, int nn, double t) {
+__device__ __forceinline__ float jl_hermite(const float4 *__restrict__ T, int nint, double t) {
   int i = (int)t;
-  i = i < nn - 2 ? i : nn - 2;
+  i = i < nint - 1 ? i : nint - 1;
+  i = i > 0 ? i : 0;
   const float f = (float)(t - (double)i);
-  const float2 a = T[i], b = T[i + 1];
-  const float d = b.x - a.x;
-  const float c2 = 3.f * d - 2.f * a.y - b.y;
-  const float c3 = -2.f * d + a.y + b.y;
-  return fmaf(f, fmaf(f, fmaf(f, c3, c2), a.y), a.x);
+  const float4 q = T[i];
+  const float d = q.z - q.x;
+  const float c2 = 3.f * d - 2.f * q.y - q.w;
+  const float c3 = -2.f * d + q.y + q.w;
+  return fmaf(f, fmaf(f, fmaf(f, c3, c2), q.y), q.x);
 }
 
-// Write Y*_lm of the generator's current m into local columns (2mm, 2mm+1).
-template <bool F16X3, int LDP>
-__device__ __forceinline__ void emit_y(const YlmDown &y, float *Yf, _Float16 *Yhi, _Float16 *Ylo, int mm, int tid) {
-  if constexpr (F16X3) {
-    // Y* * 2^12 formed in FP64 and rounded once to FP32, then split into two FP16 halves
-    const double P = ldexp(y.v, y.E + 12);
-    h2 hi, lo;
-    split_pair((float)(P * y.cm), (float)(-P * y.sm), hi, lo);
-    Yhi[(2 * mm) * LDP + tid] = hi.x;
-    Yhi[(2 * mm + 1) * LDP + tid] = hi.y;
-    Ylo[(2 * mm) * LDP + tid] = lo.x;
-    Ylo[(2 * mm + 1) * LDP + tid] = lo.y;
-  } else {
+// Write Y*_lm of the generator's current m into local columns (2mm, 2mm+1)
+// of each operand piece (Yf: FP32; Yh: NP FP16 planes of 2*MX_MC*LDP).
+template <int NP, int LDP>
+__device__ __forceinline__ void emit_y(const YlmDown &y, float *Yf, _Float16 *Yh, int mm, int tid) {
+  if constexpr (NP == 0) {
     double re, im;
     y.value(re, im);
     Yf[(2 * mm) * LDP + tid] = (float)re;
     Yf[(2 * mm + 1) * LDP + tid] = (float)im;
+  } else {
+    // Y* * 2^12 formed in FP64 and rounded once to FP32, then split into FP16 pieces
+    const double P = ldexp(y.v, y.E + 12);
+    const float re = (float)(P * y.cm), im = (float)(-P * y.sm);
+    h2 pc[3];
+    if constexpr (NP == 2)
+      split_pair(re, im, pc[0], pc[1]);
+    else
+      split3_pair(re, im, pc[0], pc[1], pc[2]);
+#pragma unroll
+    for (int k = 0; k < NP; ++k) {
+      Yh[k * 2 * MX_MC * LDP + (2 * mm) * LDP + tid] = pc[k].x;
+      Yh[k * 2 * MX_MC * LDP + (2 * mm + 1) * LDP + tid] = pc[k].y;
+    }
   }
 }
 
-template <bool F16X3, int LDP>
-__device__ __forceinline__ void emit_zero(float *Yf, _Float16 *Yhi, _Float16 *Ylo, int mm, int tid) {
-  if constexpr (F16X3) {
-    Yhi[(2 * mm) * LDP + tid] = (_Float16)0.f;
-    Yhi[(2 * mm + 1) * LDP + tid] = (_Float16)0.f;
-    Ylo[(2 * mm) * LDP + tid] = (_Float16)0.f;
-    Ylo[(2 * mm + 1) * LDP + tid] = (_Float16)0.f;
-  } else {
+template <int NP, int LDP>
+__device__ __forceinline__ void emit_zero(float *Yf, _Float16 *Yh, int mm, int tid) {
+  if constexpr (NP == 0) {
     Yf[(2 * mm) * LDP + tid] = 0.f;
     Yf[(2 * mm + 1) * LDP + tid] = 0.f;
+  } else {
+#pragma unroll
+    for (int k = 0; k < NP; ++k) {
+      Yh[k * 2 * MX_MC * LDP + (2 * mm) * LDP + tid] = (_Float16)0.f;
+      Yh[k * 2 * MX_MC * LDP + (2 * mm + 1) * LDP + tid] = (_Float16)0.f;
+    }
   }
 }
 
-template <int QMAX, bool F16X3>
+template <int QMAX, int NP>
 __global__ __launch_bounds__(256) void sfb_direct_mixed_kernel(
     PointsSoA pts, const DirectItem *__restrict__ items, const double *__restrict__ kln, int nmax,
     LegendreTables leg, HermiteTables ht, double *__restrict__ partial) {
-  constexpr int LDP = F16X3 ? MX_TP + 32 : MX_TP + 4;  // row pitch of the Y* chunk (elements)
-  constexpr int YBYTES = F16X3 ? 2 * 2 * MX_MC * LDP * 2 : 2 * MX_MC * LDP * 4;
-  extern __shared__ float2 T[];  // Bessel table of this item's l (dynamic: largest of the launch)
+  static_assert(NP == 0 || NP == 2 || NP == 3, "operand pieces");
+  constexpr int LDP = NP ? MX_TP + 32 : MX_TP + 4;  // row pitch of the Y* chunk (elements)
+  constexpr int PLANE = 2 * MX_MC * LDP;            // elements of one piece of the chunk
+  constexpr int YBYTES = NP ? NP * PLANE * 2 : PLANE * 4;
+  constexpr int FOLD = 64;                          // points per FP32 partial sum
+  constexpr int NA = NP ? NP : 1;
   __shared__ __attribute__((aligned(16))) char ybuf[YBYTES];
   __shared__ double rs[MX_TP];
   __shared__ double2 cf[MX_LMAX];  // recurrence coefficients (a_m, b_m) of this l
   float *Yf = reinterpret_cast<float *>(ybuf);
-  _Float16 *Yhi = reinterpret_cast<_Float16 *>(ybuf);
-  _Float16 *Ylo = Yhi + 2 * MX_MC * LDP;
+  _Float16 *Yh = reinterpret_cast<_Float16 *>(ybuf);
 
   const DirectItem it = items[blockIdx.x];
   const int l = it.l;
@@ -152,9 +189,8 @@ __global__ __launch_bounds__(256) void sfb_direct_mixed_kernel(
   const bool wave_rows = 16 * wave < it.nrows;  // wave-uniform
   const bool rowok = row < it.nrows;
 
-  const int nn = ht.nnodes[l];
-  const float2 *gt = ht.tab + ht.off[l];
-  for (int i = tid; i < nn; i += 256) T[i] = gt[i];
+  const int nint = ht.nint[l];
+  const float4 *__restrict__ T = ht.tab + ht.off[l];
   const double khr = rowok ? kln[(size_t)l * nmax + it.n0 + row] * (1.0 / MX_H) : 0.0;
   const double2 sect = leg.sect[l];
   {
@@ -162,49 +198,47 @@ __global__ __launch_bounds__(256) void sfb_direct_mixed_kernel(
     for (int m = tid; m <= l; m += 256) cf[m] = gc[m];
   }
 
-  v4f acc[QMAX][2];
   v4d sum[QMAX][2];
 #pragma unroll
   for (int q = 0; q < QMAX; ++q)
 #pragma unroll
-    for (int c = 0; c < 2; ++c) {
-      acc[q][c] = v4f{0.f, 0.f, 0.f, 0.f};
-      sum[q][c] = v4d{0.0, 0.0, 0.0, 0.0};
-    }
+    for (int c = 0; c < 2; ++c) sum[q][c] = v4d{0.0, 0.0, 0.0, 0.0};
 
   for (int64_t base = it.p0; base < it.p1; base += MX_TP) {
     const int64_t gp = base + tid;
     const bool pok = gp < it.p1;
     rs[tid] = pok ? pts.r[gp] : -1.0;
-    __syncthreads();  // rs and (first tile) T ready; previous tile's readers done
+    __syncthreads();  // rs ready; previous tile's readers done
 
     // ---- 1. Bessel values into A-operand registers
-    float af[F16X3 ? 1 : MX_TP / 4];
-    h8 ahi[F16X3 ? MX_TP / 32 : 1], alo[F16X3 ? MX_TP / 32 : 1];
+    float af[NP ? 1 : MX_TP / 4];
+    h8 ah[NA][NP ? MX_TP / 32 : 1];
     if (wave_rows) {
-      if constexpr (F16X3) {
+      if constexpr (NP) {
 #pragma unroll
         for (int s = 0; s < MX_TP / 32; ++s) {
-          h2 hp[4], lp[4];
+          h2 pc[3][4];
 #pragma unroll
           for (int e = 0; e < 8; e += 2) {
             const double r0 = rs[32 * s + 8 * g + e], r1 = rs[32 * s + 8 * g + e + 1];
-            const float v0 = (rowok && r0 >= 0.0) ? jl_hermite(T, nn, r0 * khr) * MX_SCALE : 0.f;
-            const float v1 = (rowok && r1 >= 0.0) ? jl_hermite(T, nn, r1 * khr) * MX_SCALE : 0.f;
-            split_pair(v0, v1, hp[e / 2], lp[e / 2]);
+            const float v0 = (rowok && r0 >= 0.0) ? jl_hermite(T, nint, r0 * khr) * MX_SCALE : 0.f;
+            const float v1 = (rowok && r1 >= 0.0) ? jl_hermite(T, nint, r1 * khr) * MX_SCALE : 0.f;
+            if constexpr (NP == 2)
+              split_pair(v0, v1, pc[0][e / 2], pc[1][e / 2]);
+            else
+              split3_pair(v0, v1, pc[0][e / 2], pc[1][e / 2], pc[2][e / 2]);
           }
-          ahi[s] = __builtin_shufflevector(__builtin_shufflevector(hp[0], hp[1], 0, 1, 2, 3),
-                                           __builtin_shufflevector(hp[2], hp[3], 0, 1, 2, 3), 0, 1, 2, 3, 4, 5,
-                                           6, 7);
-          alo[s] = __builtin_shufflevector(__builtin_shufflevector(lp[0], lp[1], 0, 1, 2, 3),
-                                           __builtin_shufflevector(lp[2], lp[3], 0, 1, 2, 3), 0, 1, 2, 3, 4, 5,
-                                           6, 7);
+#pragma unroll
+          for (int k = 0; k < NA; ++k)
+            ah[k][s] = __builtin_shufflevector(__builtin_shufflevector(pc[k][0], pc[k][1], 0, 1, 2, 3),
+                                               __builtin_shufflevector(pc[k][2], pc[k][3], 0, 1, 2, 3), 0, 1, 2,
+                                               3, 4, 5, 6, 7);
         }
       } else {
 #pragma unroll
         for (int s = 0; s < MX_TP / 4; ++s) {
           const double r = rs[4 * s + g];
-          af[s] = (rowok && r >= 0.0) ? jl_hermite(T, nn, r * khr) : 0.f;
+          af[s] = (rowok && r >= 0.0) ? jl_hermite(T, nint, r * khr) : 0.f;
         }
       }
     }
@@ -232,7 +266,7 @@ __global__ __launch_bounds__(256) void sfb_direct_mixed_kernel(
           // full chunk: every m of it is >= 1, no per-step guards
 #pragma unroll
           for (int mm = 0; mm < MX_MC; ++mm) {
-            emit_y<F16X3, LDP>(y, Yf, Yhi, Ylo, mm, tid);
+            emit_y<NP, LDP>(y, Yf, Yh, mm, tid);
             y.step(cf[mtop - mm]);
             if ((mm & 3) == 3) y.rescale_nb();
           }
@@ -241,66 +275,99 @@ __global__ __launch_bounds__(256) void sfb_direct_mixed_kernel(
           for (int mm = 0; mm < MX_MC; ++mm) {
             const int m = mtop - mm;
             if (m >= 0) {
-              emit_y<F16X3, LDP>(y, Yf, Yhi, Ylo, mm, tid);
+              emit_y<NP, LDP>(y, Yf, Yh, mm, tid);
               if (m > 0) {
                 y.step(cf[m]);
                 y.rescale_nb();
               }
             } else {
-              emit_zero<F16X3, LDP>(Yf, Yhi, Ylo, mm, tid);
+              emit_zero<NP, LDP>(Yf, Yh, mm, tid);
             }
           }
         }
         __syncthreads();
         if (wave_rows) {
-          // two independent accumulator chains (column halves); the second half
-          // is skipped when it only covers padding (wave-uniform)
-          const bool c1 = mtop >= 8;  // local columns 16..31 hold m = mtop-8 .. mtop-15
+          // two independent accumulator chains (column halves).  Both always
+          // run (padding columns hold zeros), so that the MFMA chain and the
+          // VALU reads of its results below form one basic block, where the
+          // compiler accounts for the MFMA result latency.
           const int col = lane & 15;
-          v4f a0 = acc[qq][0], a1 = acc[qq][1];
-          if constexpr (F16X3) {
+          v4f a0 = v4f{0.f, 0.f, 0.f, 0.f}, a1 = v4f{0.f, 0.f, 0.f, 0.f};
+          if constexpr (NP) {
+            // the leading products p0*q0 go to (a0, a1), folded every FOLD
+            // points; the correction products, 2^-11 smaller, to (c0, c1),
+            // whose FP32 rounding is correspondingly smaller
+            v4f c0 = v4f{0.f, 0.f, 0.f, 0.f}, c1 = v4f{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
             for (int s = 0; s < MX_TP / 32; ++s) {
               const int o = col * LDP + 32 * s + 8 * g;
-              const h8 bh0 = *reinterpret_cast<const h8 *>(Yhi + o);
-              const h8 bl0 = *reinterpret_cast<const h8 *>(Ylo + o);
-              a0 = __builtin_amdgcn_mfma_f32_16x16x32_f16(ahi[s], bh0, a0, 0, 0, 0);
-              a0 = __builtin_amdgcn_mfma_f32_16x16x32_f16(ahi[s], bl0, a0, 0, 0, 0);
-              a0 = __builtin_amdgcn_mfma_f32_16x16x32_f16(alo[s], bh0, a0, 0, 0, 0);
-              if (c1) {
-                const h8 bh1 = *reinterpret_cast<const h8 *>(Yhi + o + 16 * LDP);
-                const h8 bl1 = *reinterpret_cast<const h8 *>(Ylo + o + 16 * LDP);
-                a1 = __builtin_amdgcn_mfma_f32_16x16x32_f16(ahi[s], bh1, a1, 0, 0, 0);
-                a1 = __builtin_amdgcn_mfma_f32_16x16x32_f16(ahi[s], bl1, a1, 0, 0, 0);
-                a1 = __builtin_amdgcn_mfma_f32_16x16x32_f16(alo[s], bh1, a1, 0, 0, 0);
+              h8 b0[NA], b1[NA];
+#pragma unroll
+              for (int k = 0; k < NP; ++k) {
+                b0[k] = *reinterpret_cast<const h8 *>(Yh + k * PLANE + o);
+                b1[k] = *reinterpret_cast<const h8 *>(Yh + k * PLANE + o + 16 * LDP);
               }
+              // correction products p_i q_j, 1 <= i + j <= NP - 1, smallest first
+              if constexpr (NP == 3) {
+                c0 = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[0][s], b0[2], c0, 0, 0, 0);
+                c0 = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[1][s], b0[1], c0, 0, 0, 0);
+                c0 = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[2][s], b0[0], c0, 0, 0, 0);
+                c1 = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[0][s], b1[2], c1, 0, 0, 0);
+                c1 = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[1][s], b1[1], c1, 0, 0, 0);
+                c1 = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[2][s], b1[0], c1, 0, 0, 0);
+              }
+              c0 = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[0][s], b0[1], c0, 0, 0, 0);
+              c0 = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[1][s], b0[0], c0, 0, 0, 0);
+              c1 = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[0][s], b1[1], c1, 0, 0, 0);
+              c1 = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[1][s], b1[0], c1, 0, 0, 0);
+              a0 = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[0][s], b0[0], a0, 0, 0, 0);
+              a1 = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[0][s], b1[0], a1, 0, 0, 0);
+              if ((32 * (s + 1)) % FOLD == 0 && 32 * (s + 1) < MX_TP) {
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                  sum[qq][0][j] += (double)a0[j];
+                  sum[qq][1][j] += (double)a1[j];
+                }
+                a0 = v4f{0.f, 0.f, 0.f, 0.f};
+                a1 = v4f{0.f, 0.f, 0.f, 0.f};
+              }
+            }
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+              sum[qq][0][j] += (double)c0[j];
+              sum[qq][1][j] += (double)c1[j];
             }
           } else {
             const float *bp = Yf + col * LDP + g;
 #pragma unroll
             for (int s = 0; s < MX_TP / 4; ++s) {
               a0 = __builtin_amdgcn_mfma_f32_16x16x4f32(af[s], bp[4 * s], a0, 0, 0, 0);
-              if (c1) a1 = __builtin_amdgcn_mfma_f32_16x16x4f32(af[s], bp[4 * s + 16 * LDP], a1, 0, 0, 0);
+              a1 = __builtin_amdgcn_mfma_f32_16x16x4f32(af[s], bp[4 * s + 16 * LDP], a1, 0, 0, 0);
+              if ((4 * (s + 1)) % FOLD == 0 && 4 * (s + 1) < MX_TP) {
+                // fold this FOLD-point partial sum into the FP64 accumulators
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                  sum[qq][0][j] += (double)a0[j];
+                  sum[qq][1][j] += (double)a1[j];
+                }
+                a0 = v4f{0.f, 0.f, 0.f, 0.f};
+                a1 = v4f{0.f, 0.f, 0.f, 0.f};
+              }
             }
           }
-          acc[qq][0] = a0;
-          acc[qq][1] = a1;
+          // fold the chunk's FP32 partial sums into the FP64 accumulators
+#pragma unroll
+          for (int j = 0; j < 4; ++j) {
+            sum[qq][0][j] += (double)a0[j];
+            sum[qq][1][j] += (double)a1[j];
+          }
         }
       }
     }
-    // ---- 3. fold the tile's FP32 sums into the FP64 accumulators
-#pragma unroll
-    for (int qq = 0; qq < QMAX; ++qq)
-#pragma unroll
-      for (int c = 0; c < 2; ++c) {
-#pragma unroll
-        for (int j = 0; j < 4; ++j) sum[qq][c][j] += (double)acc[qq][c][j];
-        acc[qq][c] = v4f{0.f, 0.f, 0.f, 0.f};
-      }
   }
 
   // ---- write the item's partial block (64 rows x its columns), FP64
-  const double scale = F16X3 ? 1.0 / ((double)MX_SCALE * (double)MX_SCALE) : 1.0;
+  const double scale = NP ? 1.0 / ((double)MX_SCALE * (double)MX_SCALE) : 1.0;
   double *out = partial + it.out;
 #pragma unroll
   for (int qq = 0; qq < QMAX; ++qq) {

@@ -41,6 +41,32 @@ __global__ void prep_points_kernel(const double *__restrict__ aos, int64_t n, do
   if (!(rr >= 0.0) || !isfinite(rr) || !isfinite(th) || !isfinite(ph)) atomicOr(err, ERR_BADPOINT);
 }
 
+// Sort keys for ordering the catalogue by radius: the bit pattern of r >= 0
+// orders like r.  idx = 0 .. n-1.
+__global__ void radius_keys_kernel(const double *__restrict__ r, int64_t n, unsigned long long *__restrict__ keys,
+                                   int32_t *__restrict__ idx) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  keys[i] = (unsigned long long)__double_as_longlong(fmax(r[i], 0.0));
+  idx[i] = (int32_t)i;
+}
+
+// out[i] = in[idx[i]] for the six arrays of the point table.
+__global__ void gather_points_kernel(PointsSoA in, const int32_t *__restrict__ idx, int64_t n,
+                                     double *__restrict__ r, double *__restrict__ x, double *__restrict__ s,
+                                     double *__restrict__ phi, double *__restrict__ cphi,
+                                     double *__restrict__ sphi) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int64_t j = idx[i];
+  r[i] = in.r[j];
+  x[i] = in.x[j];
+  s[i] = in.s[j];
+  phi[i] = in.phi[j];
+  cphi[i] = in.cphi[j];
+  sphi[i] = in.sphi[j];
+}
+
 // One thread per l: zeros z_{l,0..nz-1} of j_l, row-major [l][nz].
 __global__ void bessel_zeros_kernel(int lmax, int nz, int mode, double *__restrict__ z) {
   const int l = blockIdx.x * blockDim.x + threadIdx.x;

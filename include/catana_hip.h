@@ -44,8 +44,11 @@ enum {
 };
 /* matrix-core operand format of the mixed path */
 enum {
-  SFBHIP_OPERANDS_FP32 = 0,      /* FP32 operands, v_mfma_f32_16x16x4f32 */
-  SFBHIP_OPERANDS_SPLIT_FP16 = 1 /* operands split into two FP16 halves, three v_mfma_f32_16x16x32_f16 */
+  SFBHIP_OPERANDS_FP32 = 0,        /* FP32 operands, v_mfma_f32_16x16x4f32 */
+  SFBHIP_OPERANDS_SPLIT_FP16 = 1,  /* operands split into two FP16 pieces (22 bits), three
+                                      v_mfma_f32_16x16x32_f16 per product block */
+  SFBHIP_OPERANDS_SPLIT3_FP16 = 2  /* three FP16 pieces (33 bits), six v_mfma_f32_16x16x32_f16:
+                                      products as accurate as with FP32 operands */
 };
 /* how j_l(k r) is evaluated in the FP64 path */
 enum {
@@ -73,13 +76,15 @@ typedef struct sfbhip_params {
   int32_t mixed_fp64_below; /* mixed path: multipoles l < this run in FP64 (C_l of low l can be
                                dominated by cancellation); default 8 */
   int32_t mixed_operands;   /* mixed path: SFBHIP_OPERANDS_* */
-  int32_t reserved[1];
+  int32_t mixed_chunks;     /* mixed path: largest number of 16-wide m chunks one work item
+                               accumulates (1, 2, 4 or 8; 0 = 4) -- more chunks reuse each Bessel
+                               value more often but hold more accumulators in registers */
 } sfbhip_params;
 
 typedef struct sfbhip_handle sfbhip_handle;
 
 /* Fills defaults: precision FP64, bessel AUTO, zeros REFERENCE, alias_compat 1,
- * mixed_fp64_below 8, mixed_operands SPLIT_FP16. */
+ * mixed_fp64_below 8, mixed_operands SPLIT_FP16, mixed_chunks 4. */
 sfbhip_status sfbhip_default_params(sfbhip_params *p);
 
 /* Library state bound to one device: workspace and cached k_ln tables.

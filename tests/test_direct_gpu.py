@@ -133,3 +133,20 @@ def test_mixed_fp64_multipoles_quintic_table(handle):
     print("max rel c_ln", rel.max(), "f_lmn", _flmn_err(fq, f64))
     assert rel.max() < 1e-7
     assert _flmn_err(fq, f64) < 1e-8
+
+
+@pytest.mark.parametrize("ops,chunks", [(capi.OPERANDS_FP32, 4), (capi.OPERANDS_SPLIT_FP16, 1),
+                                        (capi.OPERANDS_SPLIT_FP16, 8), (capi.OPERANDS_SPLIT3_FP16, 2)])
+def test_mixed_operand_formats_and_chunks(handle, ops, chunks):
+    # every operand format and work-item chunk count against FP64; l up to 47
+    # spans 3 chunks, so chunks=1 and 2 split the m range over several passes
+    pts = workloads.clustered_ball(600, 800.0, mean_children=40, sigma=6.0, seed=26)
+    print(f"case: operands={ops} chunks={chunks} N={pts.shape[0]} lmax=48 k_max=0.2 rmax=800", flush=True)
+    kw = dict(k_max=0.2, bessel=capi.BESSEL_EXACT)
+    _, c64, f64 = _run(handle, pts, 48, 60, 800.0, precision=capi.PREC_FP64, **kw)
+    _, cm, fm = _run(handle, pts, 48, 60, 800.0, precision=capi.PREC_MIXED, mixed_operands=ops,
+                     mixed_chunks=chunks, **kw)
+    rel = _rel(cm, c64)
+    print("max rel c_ln", rel.max(), "median", np.median(rel))
+    assert rel.max() < 1e-6
+    assert _flmn_err(fm, f64) < 1e-6
