@@ -34,6 +34,13 @@ WORKLOADS = {
 }
 
 
+# HBM bytes per call of the mixed contraction kernels (all launches of one call), from
+# rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE on this build, scaled by what the counters read
+# for a 512 MiB device-to-device copy (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0).  Counters cannot
+# be read from inside the timed run, so the figure is carried here per workload.
+PROFILED_TRAFFIC_BYTES = {("clustered1m", "mixed", "split16"): 2.507e9}
+
+
 def make_points(name):
     from catana_amd import workloads as W
     if name == "clustered1m":
@@ -82,6 +89,9 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=65536)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="run one device-to-device copy of MIB MiB before the workload (a kernel of known "
+                         "traffic for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -92,6 +102,15 @@ def main():
 
     import torch
     from catana_amd import capi
+
+    if args.calibrate_copy > 0:
+        n = args.calibrate_copy * (1 << 20) // 8
+        src = torch.ones(n, dtype=torch.float64, device="cuda:0")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"calibration copy: {args.calibrate_copy} MiB read + {args.calibrate_copy} MiB written")
+        del src, dst
 
     cfg = WORKLOADS[args.workload]
     log(f"generating workload {args.workload}")
@@ -155,7 +174,8 @@ def main():
                    "fp64_below": args.fp64_below, "pairs_per_step": pairs,
                    "wall_s_timed_region": wall},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
-                     "frac": achieved / peak, "traffic": None,
+                     "frac": achieved / peak,
+                     "traffic": PROFILED_TRAFFIC_BYTES.get((args.workload, args.precision, args.operands)),
                      "kernel": "sfb_direct_%s_kernel" % ("f64" if prec == capi.PREC_FP64 else "mixed"),
                      "kernel_ms": kms, "algorithmic_flops_per_launch": flops_dom,
                      "l_range": [lo, cfg["lmax"]],
@@ -165,9 +185,17 @@ def main():
     if prec != capi.PREC_FP64 and not args.no_accuracy:
         log("FP64 reference run for the accuracy figures")
         _, cm, fm = capi.sfb_direct_torch(h, d_pts, cfg["lmax"], cfg["nmax"], cfg["rmax"], store_flmn=True, **kw)
+        torch.cuda.synchronize()
+        t64 = time.perf_counter()
         k64, c64, f64 = capi.sfb_direct_torch(h, d_pts, cfg["lmax"], cfg["nmax"], cfg["rmax"],
                                               k_max=cfg["k_max"], precision=capi.PREC_FP64,
                                               bessel=capi.BESSEL_EXACT, store_flmn=True)
+        torch.cuda.synchronize()
+        t64 = time.perf_counter() - t64
+        # the plain FP64 GPU path on the same points, one call (it includes the
+        # f_lmn store, which the timed steps skip; its cost is negligible)
+        out["fp64_gpu"] = {"ms_per_step": t64 * 1e3, "value": pairs / t64, "unit": "pairs/s",
+                           "speedup_of_value": out["value"] / (pairs / t64)}
         a, b = cm.cpu().numpy(), c64.cpu().numpy()
         assert np.array_equal(a, c.cpu().numpy()), "store_flmn changed c_ln"
         mask = b != 0

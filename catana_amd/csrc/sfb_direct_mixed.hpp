@@ -167,7 +167,10 @@ __global__ __launch_bounds__(256) void sfb_direct_mixed_kernel(
     PointsSoA pts, const DirectItem *__restrict__ items, const double *__restrict__ kln, int nmax,
     LegendreTables leg, HermiteTables ht, double *__restrict__ partial) {
   static_assert(NP == 0 || NP == 2 || NP == 3, "operand pieces");
-  constexpr int LDP = NP ? MX_TP + 32 : MX_TP + 4;  // row pitch of the Y* chunk (elements)
+  // row pitch of the Y* chunk (elements): a column starts 4 banks after the
+  // previous one, so the 16-byte B-fragment reads of 8 consecutive columns
+  // (FP16) and the 4-byte reads of 16 columns x 4 points (FP32) hit distinct banks
+  constexpr int LDP = NP ? MX_TP + 8 : MX_TP + 4;
   constexpr int PLANE = 2 * MX_MC * LDP;            // elements of one piece of the chunk
   constexpr int YBYTES = NP ? NP * PLANE * 2 : PLANE * 4;
   constexpr int FOLD = 64;                          // points per FP32 partial sum
