@@ -148,20 +148,6 @@ __device__ __forceinline__ void emit_y(const YlmDown &y, float *Yf, _Float16 *Yh
   }
 }
 
-template <int NP, int LDP>
-__device__ __forceinline__ void emit_zero(float *Yf, _Float16 *Yh, int mm, int tid) {
-  if constexpr (NP == 0) {
-    Yf[(2 * mm) * LDP + tid] = 0.f;
-    Yf[(2 * mm + 1) * LDP + tid] = 0.f;
-  } else {
-#pragma unroll
-    for (int k = 0; k < NP; ++k) {
-      Yh[k * 2 * MX_MC * LDP + (2 * mm) * LDP + tid] = (_Float16)0.f;
-      Yh[k * 2 * MX_MC * LDP + (2 * mm + 1) * LDP + tid] = (_Float16)0.f;
-    }
-  }
-}
-
 template <int QMAX, int NP>
 __global__ __launch_bounds__(256) void sfb_direct_mixed_kernel(
     PointsSoA pts, const DirectItem *__restrict__ items, const double *__restrict__ kln, int nmax,
@@ -177,7 +163,10 @@ __global__ __launch_bounds__(256) void sfb_direct_mixed_kernel(
   constexpr int NA = NP ? NP : 1;
   __shared__ __attribute__((aligned(16))) char ybuf[YBYTES];
   __shared__ double rs[MX_TP];
-  __shared__ double2 cf[MX_LMAX];  // recurrence coefficients (a_m, b_m) of this l
+  // recurrence coefficients (a_m, b_m) of this l at cf[MX_MC + m]; zero for m <= 0, so that
+  // stepping past m = 0 leaves P = 0 and the lowest chunk needs no per-step guards
+  __shared__ double2 cfs[MX_MC + MX_LMAX];
+  double2 *cf = cfs + MX_MC;
   float *Yf = reinterpret_cast<float *>(ybuf);
   _Float16 *Yh = reinterpret_cast<_Float16 *>(ybuf);
 
@@ -198,7 +187,7 @@ __global__ __launch_bounds__(256) void sfb_direct_mixed_kernel(
   const double2 sect = leg.sect[l];
   {
     const double2 *gc = leg.coef + leg.coef_off[l];
-    for (int m = tid; m <= l; m += 256) cf[m] = gc[m];
+    for (int m = tid - MX_MC; m <= l; m += 256) cf[m] = m > 0 ? gc[m] : make_double2(0.0, 0.0);
   }
 
   v4d sum[QMAX][2];
@@ -265,28 +254,13 @@ __global__ __launch_bounds__(256) void sfb_direct_mixed_kernel(
       if (q < it.qhi) {
         __syncthreads();  // previous chunk's readers are done
         const int mtop = l - MX_MC * q;
-        if (mtop >= MX_MC) {
-          // full chunk: every m of it is >= 1, no per-step guards
+        // m = mtop - mm; in the lowest chunk the steps at m <= 0 use zero
+        // coefficients, so the columns of m < 0 come out as zeros
 #pragma unroll
-          for (int mm = 0; mm < MX_MC; ++mm) {
-            emit_y<NP, LDP>(y, Yf, Yh, mm, tid);
-            y.step(cf[mtop - mm]);
-            if ((mm & 3) == 3) y.rescale_nb();
-          }
-        } else {
-          // lowest chunk: m = mtop .. 0 are generated, the rest are zero columns
-          for (int mm = 0; mm < MX_MC; ++mm) {
-            const int m = mtop - mm;
-            if (m >= 0) {
-              emit_y<NP, LDP>(y, Yf, Yh, mm, tid);
-              if (m > 0) {
-                y.step(cf[m]);
-                y.rescale_nb();
-              }
-            } else {
-              emit_zero<NP, LDP>(Yf, Yh, mm, tid);
-            }
-          }
+        for (int mm = 0; mm < MX_MC; ++mm) {
+          emit_y<NP, LDP>(y, Yf, Yh, mm, tid);
+          y.step(cf[mtop - mm]);
+          if ((mm & 3) == 3) y.rescale_nb();
         }
         __syncthreads();
         if (wave_rows) {
