@@ -63,6 +63,7 @@ struct KlnCache {
   std::vector<int32_t> nl;
   bool table_valid = false;   // reference interpolation table on the device
   bool herm_valid = false;    // Hermite tables of the mixed path on the device
+  std::vector<int32_t> herm_nint;  // their interval counts per l (host copy)
   bool q5_valid = false;      // quintic FP64 tables of the low multipoles on the device
   int q5_lmax = 0;
   double q5_R = 0.0;
@@ -525,6 +526,7 @@ sfbhip_status ensure_hermite(sfbhip_handle *h, const sfbhip_params *p, double rd
     }
     HIPCHK(hipStreamSynchronize(st));
     c.herm_R = R;
+    c.herm_nint = ni;
     c.herm_valid = true;
   }
   CHK(get_buf(h, "herm_tab", 1, &tab));
@@ -536,11 +538,22 @@ sfbhip_status ensure_hermite(sfbhip_handle *h, const sfbhip_params *p, double rd
   return SFBHIP_OK;
 }
 
+// Bessel tables of up to MX_LDS_NODES nodes (36 KB) are staged in LDS: the
+// block then still fits twice per CU.
+constexpr int MX_LDS_NODES = 4608;
+
 template <int QMAX, int NP>
 sfbhip_status launch_mixed(int nitems, const DirectItem *items, hipStream_t st, const PointsSoA &pts,
                            const double *kln, int nmax, const LegendreTables &lt, const HermiteTables &ht,
-                           double *partial) {
-  sfb_direct_mixed_kernel<QMAX, NP><<<nitems, 256, 0, st>>>(pts, items, kln, nmax, lt, ht, partial);
+                           int lds_nodes, double *partial) {
+  if (lds_nodes > 0) {
+    const size_t shm = sizeof(float2) * (size_t)lds_nodes;
+    HIPCHK(hipFuncSetAttribute(reinterpret_cast<const void *>(&sfb_direct_mixed_kernel<QMAX, NP, true>),
+                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
+    sfb_direct_mixed_kernel<QMAX, NP, true><<<nitems, 256, shm, st>>>(pts, items, kln, nmax, lt, ht, partial);
+  } else {
+    sfb_direct_mixed_kernel<QMAX, NP, false><<<nitems, 256, 0, st>>>(pts, items, kln, nmax, lt, ht, partial);
+  }
   HIPCHK(hipGetLastError());
   return SFBHIP_OK;
 }
@@ -549,12 +562,12 @@ sfbhip_status launch_mixed(int nitems, const DirectItem *items, hipStream_t st, 
 template <int QMAX>
 sfbhip_status launch_mixed_ops(int ops, int nitems, const DirectItem *items, hipStream_t st, const PointsSoA &pts,
                                const double *kln, int nmax, const LegendreTables &lt, const HermiteTables &ht,
-                               double *partial) {
+                               int lds_nodes, double *partial) {
   if (ops == SFBHIP_OPERANDS_SPLIT_FP16)
-    return launch_mixed<QMAX, 2>(nitems, items, st, pts, kln, nmax, lt, ht, partial);
+    return launch_mixed<QMAX, 2>(nitems, items, st, pts, kln, nmax, lt, ht, lds_nodes, partial);
   if (ops == SFBHIP_OPERANDS_SPLIT3_FP16)
-    return launch_mixed<QMAX, 3>(nitems, items, st, pts, kln, nmax, lt, ht, partial);
-  return launch_mixed<QMAX, 0>(nitems, items, st, pts, kln, nmax, lt, ht, partial);
+    return launch_mixed<QMAX, 3>(nitems, items, st, pts, kln, nmax, lt, ht, lds_nodes, partial);
+  return launch_mixed<QMAX, 0>(nitems, items, st, pts, kln, nmax, lt, ht, lds_nodes, partial);
 }
 
 // Launch every class of a plan; items[] on the device holds the classes back to back from `*off`.
@@ -579,20 +592,33 @@ sfbhip_status launch_plan_f64(const DirectPlan &pl, DirectItem *ditems, size_t *
   return SFBHIP_OK;
 }
 
-sfbhip_status launch_plan_mixed(const DirectPlan &pl, DirectItem *ditems, size_t *off,
-                                hipStream_t st, const PointsSoA &pts, const double *dk, int nmax,
+// Within each class, items whose l has a Bessel table of at most MX_LDS_NODES
+// nodes come first (they run the LDS-table kernel), the rest after them.
+void order_mixed_items(DirectPlan *pl, const std::vector<int32_t> &nint) {
+  for (auto &v : pl->items)
+    std::stable_partition(v.begin(), v.end(), [&](const DirectItem &it) { return nint[it.l] + 1 <= MX_LDS_NODES; });
+}
+
+sfbhip_status launch_plan_mixed(const DirectPlan &pl, const std::vector<int32_t> &nint, DirectItem *ditems,
+                                size_t *off, hipStream_t st, const PointsSoA &pts, const double *dk, int nmax,
                                 const LegendreTables &lt, const HermiteTables &ht, int ops, double *partial) {
   for (int ci = 0; ci < 4; ++ci) {
-    const int n = (int)pl.items[ci].size();
-    if (!n) continue;
-    const DirectItem *di = ditems + *off;
-    switch (ci) {
-      case 0: CHK(launch_mixed_ops<1>(ops, n, di, st, pts, dk, nmax, lt, ht, partial)); break;
-      case 1: CHK(launch_mixed_ops<2>(ops, n, di, st, pts, dk, nmax, lt, ht, partial)); break;
-      case 2: CHK(launch_mixed_ops<4>(ops, n, di, st, pts, dk, nmax, lt, ht, partial)); break;
-      default: CHK(launch_mixed_ops<8>(ops, n, di, st, pts, dk, nmax, lt, ht, partial)); break;
+    const auto &v = pl.items[ci];
+    int nlds = 0, nodes = 0;
+    while (nlds < (int)v.size() && nint[v[nlds].l] + 1 <= MX_LDS_NODES) nodes = std::max(nodes, nint[v[nlds++].l] + 1);
+    for (int part = 0; part < 2; ++part) {
+      const int n = part == 0 ? nlds : (int)v.size() - nlds;
+      if (!n) continue;
+      const int lds = part == 0 ? nodes : 0;
+      const DirectItem *di = ditems + *off;
+      switch (ci) {
+        case 0: CHK(launch_mixed_ops<1>(ops, n, di, st, pts, dk, nmax, lt, ht, lds, partial)); break;
+        case 1: CHK(launch_mixed_ops<2>(ops, n, di, st, pts, dk, nmax, lt, ht, lds, partial)); break;
+        case 2: CHK(launch_mixed_ops<4>(ops, n, di, st, pts, dk, nmax, lt, ht, lds, partial)); break;
+        default: CHK(launch_mixed_ops<8>(ops, n, di, st, pts, dk, nmax, lt, ht, lds, partial)); break;
+      }
+      *off += n;
     }
-    *off += n;
   }
   return SFBHIP_OK;
 }
@@ -748,9 +774,11 @@ sfbhip_status sfbhip_direct(sfbhip_handle *h, const double *d_points, int64_t n_
   pl64.partial_size = pllw.partial_size;
   plan_direct(h->kc.nl, llow, lsplit, n_points, F64_NT, F64_TP, F64_MC, 16, 4096, &pl64);
   plmx.partial_size = pl64.partial_size;
-  if (mixed)
+  if (mixed) {
     plan_direct(h->kc.nl, lsplit, p->lmax, n_points, MX_NT, MX_TP, MX_MC, p->mixed_chunks ? p->mixed_chunks : 4,
                 4096, &plmx);
+    order_mixed_items(&plmx, h->kc.herm_nint);
+  }
   std::vector<ReduceGroup> groups = pllw.groups;
   groups.insert(groups.end(), pl64.groups.begin(), pl64.groups.end());
   groups.insert(groups.end(), plmx.groups.begin(), plmx.groups.end());
@@ -776,7 +804,7 @@ sfbhip_status sfbhip_direct(sfbhip_handle *h, const double *d_points, int64_t n_
   CHK(launch_plan_f64(pl64, ditems, &off, st, pts, nullptr, dk, p->nmax, lt, use_table, tab, partial, derr));
   HIPCHK(hipEventRecord(h->ev[1], st));
   if (mixed)
-    CHK(launch_plan_mixed(plmx, ditems, &off, st, pts, dk, p->nmax, lt, ht,
+    CHK(launch_plan_mixed(plmx, h->kc.herm_nint, ditems, &off, st, pts, dk, p->nmax, lt, ht,
                           p->mixed_operands, partial));
   HIPCHK(hipEventRecord(h->ev[2], st));
   CHK(finalize(h, p, groups, partial, norm, d_k_ln, d_c_ln, d_f_lmn, st));

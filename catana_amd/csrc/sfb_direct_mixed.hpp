@@ -122,6 +122,19 @@ __device__ __forceinline__ float jl_hermite(const float4 *__restrict__ T, int ni
   return fmaf(f, fmaf(f, fmaf(f, c3, c2), q.y), q.x);
 }
 
+// The same from a node copy in LDS: node i = (j_l(i h), h j_l'(i h)), i <= nint.
+__device__ __forceinline__ float jl_hermite_lds(const float2 *T, int nint, double t) {
+  int i = (int)t;
+  i = i < nint - 1 ? i : nint - 1;
+  i = i > 0 ? i : 0;
+  const float f = (float)(t - (double)i);
+  const float2 a = T[i], b = T[i + 1];
+  const float d = b.x - a.x;
+  const float c2 = 3.f * d - 2.f * a.y - b.y;
+  const float c3 = -2.f * d + a.y + b.y;
+  return fmaf(f, fmaf(f, fmaf(f, c3, c2), a.y), a.x);
+}
+
 // Write Y*_lm of the generator's current m into local columns (2mm, 2mm+1)
 // of each operand piece (Yf: FP32; Yh: NP FP16 planes of 2*MX_MC*LDP).
 template <int NP, int LDP>
@@ -148,7 +161,10 @@ __device__ __forceinline__ void emit_y(const YlmDown &y, float *Yf, _Float16 *Yh
   }
 }
 
-template <int QMAX, int NP>
+// TL: the Bessel table of the item's l is copied into LDS (dynamic shared
+// memory, nint + 1 nodes) when it is small enough to keep two blocks per CU;
+// otherwise it is read from global memory (L1/L2).
+template <int QMAX, int NP, bool TL>
 __global__ __launch_bounds__(256) void sfb_direct_mixed_kernel(
     PointsSoA pts, const DirectItem *__restrict__ items, const double *__restrict__ kln, int nmax,
     LegendreTables leg, HermiteTables ht, double *__restrict__ partial) {
@@ -183,6 +199,19 @@ __global__ __launch_bounds__(256) void sfb_direct_mixed_kernel(
 
   const int nint = ht.nint[l];
   const float4 *__restrict__ T = ht.tab + ht.off[l];
+  extern __shared__ float2 TN[];  // TL: node table
+  if constexpr (TL) {
+    for (int i = tid; i <= nint; i += 256) {
+      const float4 q = T[i < nint ? i : nint - 1];
+      TN[i] = i < nint ? make_float2(q.x, q.y) : make_float2(q.z, q.w);
+    }
+  }
+  auto jl = [&](double t) -> float {
+    if constexpr (TL)
+      return jl_hermite_lds(TN, nint, t);
+    else
+      return jl_hermite(T, nint, t);
+  };
   const double khr = rowok ? kln[(size_t)l * nmax + it.n0 + row] * (1.0 / MX_H) : 0.0;
   const double2 sect = leg.sect[l];
   {
@@ -213,8 +242,8 @@ __global__ __launch_bounds__(256) void sfb_direct_mixed_kernel(
 #pragma unroll
           for (int e = 0; e < 8; e += 2) {
             const double r0 = rs[32 * s + 8 * g + e], r1 = rs[32 * s + 8 * g + e + 1];
-            const float v0 = (rowok && r0 >= 0.0) ? jl_hermite(T, nint, r0 * khr) * MX_SCALE : 0.f;
-            const float v1 = (rowok && r1 >= 0.0) ? jl_hermite(T, nint, r1 * khr) * MX_SCALE : 0.f;
+            const float v0 = (rowok && r0 >= 0.0) ? jl(r0 * khr) * MX_SCALE : 0.f;
+            const float v1 = (rowok && r1 >= 0.0) ? jl(r1 * khr) * MX_SCALE : 0.f;
             if constexpr (NP == 2)
               split_pair(v0, v1, pc[0][e / 2], pc[1][e / 2]);
             else
@@ -230,7 +259,7 @@ __global__ __launch_bounds__(256) void sfb_direct_mixed_kernel(
 #pragma unroll
         for (int s = 0; s < MX_TP / 4; ++s) {
           const double r = rs[4 * s + g];
-          af[s] = (rowok && r >= 0.0) ? jl_hermite(T, nint, r * khr) : 0.f;
+          af[s] = (rowok && r >= 0.0) ? jl(r * khr) : 0.f;
         }
       }
     }
