@@ -31,7 +31,24 @@ WORKLOADS = {
     "lognormal2p25": dict(lmax=256, nmax=160, rmax=1000.0, k_max=0.5),
     # 10,000 uniform points, lmax 16, k_max 0.1, rmax 500
     "uniform10k": dict(lmax=16, nmax=16, rmax=500.0, k_max=0.1),
+    # 64 catalogues: N log-uniform in [1e4, 2e6], rmax U[300, 1500], lmax U{32..128}, k_max 0.15,
+    # each uniform in its ball, decomposed in one batched call
+    "batched64": dict(lmax=128, nmax=0, rmax=0.0, k_max=0.15, batched=True),
 }
+
+
+def make_batch(seed=0, ncat=64):
+    """The 64-catalogue batch: per catalogue (points, lmax, nmax, rmax, k_max)."""
+    from catana_amd import workloads as W
+    rng = np.random.default_rng(seed)
+    cats = []
+    for i in range(ncat):
+        n = int(round(10 ** rng.uniform(4.0, np.log10(2e6))))
+        rmax = float(rng.uniform(300.0, 1500.0))
+        lmax = int(rng.integers(32, 129))
+        nmax = int(np.ceil(0.15 * rmax / np.pi)) + 1  # > the number of zeros below k_max rmax for every l
+        cats.append(dict(np=W.uniform_ball(n, rmax, seed=1000 + i), lmax=lmax, nmax=nmax, rmax=rmax, k_max=0.15))
+    return cats
 
 
 # HBM bytes per call of the mixed contraction kernels (all launches of one call), from
@@ -113,6 +130,9 @@ def main():
         del src, dst
 
     cfg = WORKLOADS[args.workload]
+    if cfg.get("batched"):
+        run_batched(args)
+        return
     log(f"generating workload {args.workload}")
     pts = make_points(args.workload)
     log(f"{pts.shape[0]} points")
@@ -211,6 +231,104 @@ def main():
         log("CPU baseline")
         threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
         out["cpu_baseline"] = cpu_baseline(pts, cfg, nl, args.cpu_sample, threads)
+    h.close()
+    print(json.dumps(out))
+
+
+def run_batched(args):
+    """The 64-catalogue workload: one sfbhip_direct_batched call per step."""
+    import torch
+    from catana_amd import capi
+    log("generating 64 catalogues")
+    cats = make_batch()
+    dev = torch.device("cuda:0")
+    for c in cats:
+        c["points"] = torch.from_numpy(np.ascontiguousarray(c["np"])).to(dev)
+    h = capi.Handle(0)
+    prec = capi.PREC_MIXED if args.precision == "mixed" else capi.PREC_FP64
+    ops = {"fp32": capi.OPERANDS_FP32, "split16": capi.OPERANDS_SPLIT_FP16, "split3": capi.OPERANDS_SPLIT3_FP16}
+    kw = dict(precision=prec, mixed_fp64_below=args.fp64_below, mixed_operands=ops[args.operands],
+              mixed_chunks=args.chunks)
+    pairs, flops_dom = 0, 0.0
+    lo = min(args.fp64_below, 10 ** 9) if prec == capi.PREC_MIXED else 0
+    for c in cats:
+        _, nl = h.kln(capi.make_params(c["lmax"], c["nmax"], c["rmax"], k_max=c["k_max"]))
+        l = np.arange(c["lmax"])
+        pairs += c["np"].shape[0] * int(((2 * l + 1) * nl).sum())
+        flops_dom += 4.0 * c["np"].shape[0] * int(((l[lo:] + 1) * nl[lo:]).sum())
+    ntot = sum(c["np"].shape[0] for c in cats)
+    log(f"{ntot} points in {len(cats)} catalogues")
+    for _ in range(args.warmup):
+        capi.sfb_direct_batched_torch(h, cats, **kw)
+    torch.cuda.synchronize()
+    stream = torch.cuda.current_stream(dev)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    kern, phases = [], []
+    e0.record(stream)
+    for _ in range(args.steps):
+        res = capi.sfb_direct_batched_torch(h, cats, **kw)
+        kern.append(h.last_kernel_ms())
+        phases.append(h.last_phase_ms())
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / args.steps
+    kms = float(np.mean(kern))
+    ph = np.mean(np.array(phases), axis=0)
+    # the same catalogues one call each (what a caller looping over sfb_decomposition does)
+    t0 = time.perf_counter()
+    for c in cats:
+        capi.sfb_direct_torch(h, c["points"], c["lmax"], c["nmax"], c["rmax"], k_max=c["k_max"],
+                              bessel=capi.BESSEL_EXACT, **kw)
+    torch.cuda.synchronize()
+    seq_ms = (time.perf_counter() - t0) * 1e3
+    if prec == capi.PREC_FP64:
+        dtype, peak = "f64", PEAK_TFLOPS["fp64"]
+    elif args.operands == "split16":
+        dtype, peak = "f16 2-piece split, 3 products (f32 accumulate, f64 tile sums)", PEAK_TFLOPS["fp16"] / 3.0
+    elif args.operands == "split3":
+        dtype, peak = "f16 3-piece split, 6 products (f32 accumulate, f64 tile sums)", PEAK_TFLOPS["fp16"] / 6.0
+    else:
+        dtype, peak = "f32 (f32 operands and accumulate, f64 tile sums)", PEAK_TFLOPS["fp32"]
+    achieved = flops_dom / (kms * 1e-3) / 1e12
+    out = {
+        "metric": METRIC, "value": pairs / (ms * 1e-3), "unit": "pairs/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": dtype, "data": "synthetic, seeded (bench.make_batch), inputs resident in HBM",
+        "config": {"workload": "batched64", "catalogues": len(cats), "N_total": int(ntot),
+                   "lmax_range": [min(c["lmax"] for c in cats), max(c["lmax"] for c in cats)], "k_max": 0.15,
+                   "precision": args.precision, "operands": args.operands, "chunks": args.chunks,
+                   "pairs_per_step": pairs, "sequential_single_calls_ms": seq_ms},
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s", "frac": achieved / peak,
+                     "traffic": None, "kernel": "sfb_direct_%s_kernel" % ("f64" if prec == capi.PREC_FP64 else "mixed"),
+                     "kernel_ms": kms, "algorithmic_flops_per_launch": flops_dom,
+                     "phases_ms": {"fp64_contraction": float(ph[0]), "mixed_contraction": float(ph[1]),
+                                   "reduce_and_cln": float(ph[2])}},
+    }
+    if prec != capi.PREC_FP64 and not args.no_accuracy:
+        log("FP64 batched run for the accuracy figures")
+        torch.cuda.synchronize()
+        t64 = time.perf_counter()
+        r64 = capi.sfb_direct_batched_torch(h, cats, precision=capi.PREC_FP64)
+        torch.cuda.synchronize()
+        t64 = time.perf_counter() - t64
+        worst, meds = 0.0, []
+        for (_, cm, _), (_, c64, _) in zip(res, r64):
+            a, b = cm.cpu().numpy(), c64.cpu().numpy()
+            m = b != 0
+            rel = np.abs(a[m] / b[m] - 1)
+            worst = max(worst, float(rel.max()))
+            meds.append(float(np.median(rel)))
+        out["fp64_gpu"] = {"ms_per_step": t64 * 1e3, "value": pairs / t64, "unit": "pairs/s",
+                           "speedup_of_value": out["value"] / (pairs / t64)}
+        out["accuracy"] = {"max_rel_cln_vs_fp64": worst, "median_rel_cln_vs_fp64": float(np.median(meds))}
+    if not args.no_cpu_baseline:
+        log("CPU baseline")
+        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        c = max(cats, key=lambda c: c["np"].shape[0])
+        _, nl = h.kln(capi.make_params(c["lmax"], c["nmax"], c["rmax"], k_max=c["k_max"]))
+        out["cpu_baseline"] = cpu_baseline(c["np"], dict(lmax=c["lmax"], nmax=c["nmax"], rmax=c["rmax"]), nl,
+                                           args.cpu_sample, threads)
+        out["cpu_baseline"]["sample"] += " (largest catalogue of the batch)"
     h.close()
     print(json.dumps(out))
 

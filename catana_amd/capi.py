@@ -19,7 +19,7 @@ OPERANDS_FP32, OPERANDS_SPLIT_FP16, OPERANDS_SPLIT3_FP16 = 0, 1, 2
 
 EXPORTED = [
     "sfbhip_default_params", "sfbhip_create", "sfbhip_destroy", "sfbhip_flmn_count", "sfbhip_kln",
-    "sfbhip_direct", "sfbhip_pixelize", "sfbhip_ang2pix_ring", "sfbhip_healpix", "sfbhip_last_kernel_ms",
+    "sfbhip_direct", "sfbhip_direct_batched", "sfbhip_pixelize", "sfbhip_ang2pix_ring", "sfbhip_healpix", "sfbhip_last_kernel_ms",
     "sfbhip_last_phase_ms", "sfbhip_last_error",
 ]
 
@@ -31,6 +31,15 @@ class Params(ctypes.Structure):
         ("bessel", ctypes.c_int32), ("zeros", ctypes.c_int32), ("alias_compat", ctypes.c_int32),
         ("verbose", ctypes.c_int32), ("mixed_fp64_below", ctypes.c_int32), ("mixed_operands", ctypes.c_int32),
         ("mixed_chunks", ctypes.c_int32),
+    ]
+
+
+class Catalog(ctypes.Structure):
+    _fields_ = [
+        ("d_points", ctypes.c_void_p), ("n_points", ctypes.c_int64), ("lmax", ctypes.c_int32),
+        ("nmax", ctypes.c_int32), ("rmax", ctypes.c_double), ("k_max", ctypes.c_double),
+        ("window_volume", ctypes.c_double), ("d_k_ln", ctypes.c_void_p), ("d_c_ln", ctypes.c_void_p),
+        ("d_f_lmn", ctypes.c_void_p),
     ]
 
 
@@ -57,6 +66,7 @@ def lib():
         L.sfbhip_flmn_count.argtypes = [i32, i32]
         L.sfbhip_kln.argtypes = [P, ctypes.POINTER(Params), P, P]
         L.sfbhip_direct.argtypes = [P, P, i64, ctypes.POINTER(Params), P, P, P, P]
+        L.sfbhip_direct_batched.argtypes = [P, ctypes.POINTER(Catalog), i32, ctypes.POINTER(Params), P]
         L.sfbhip_pixelize.argtypes = [P, P, i64, i32, P, P, P]
         L.sfbhip_ang2pix_ring.argtypes = [i32, ctypes.c_double, ctypes.c_double, ctypes.POINTER(i64)]
         L.sfbhip_healpix.argtypes = [P, P, P, i32, ctypes.POINTER(Params), P, P, P, P]
@@ -175,6 +185,35 @@ def sfb_direct_torch(handle, points, lmax, nmax, rmax, store_flmn=False, stream=
                              ctypes.c_void_p(stream) if stream else None)
     check(rc, "sfbhip_direct")
     return k, c, f
+
+
+def sfb_direct_batched_torch(handle, catalogs, store_flmn=False, stream=None, **kw):
+    """catalogs: list of dicts with points (torch float64 (N, 3) on the GPU), lmax,
+    nmax, rmax and optional k_max, window_volume.  Common options (precision,
+    zeros, mixed_*) as for sfb_direct_torch.  Returns a list of (k_ln, c_ln,
+    f_lmn|None) torch tensors, one per catalogue, from one batched call."""
+    torch = _torch()
+    p = make_params(1, 1, 1.0, **kw)
+    arr = (Catalog * len(catalogs))()
+    outs, keep = [], []
+    for i, c in enumerate(catalogs):
+        pts = c["points"].contiguous()
+        assert pts.dtype == torch.float64 and pts.dim() == 2 and pts.shape[1] == 3
+        lmax, nmax = int(c["lmax"]), int(c["nmax"])
+        k = torch.empty((lmax, nmax), dtype=torch.float64, device=pts.device)
+        cl = torch.empty((lmax, nmax), dtype=torch.float64, device=pts.device)
+        f = torch.empty(2 * flmn_count(lmax, nmax), dtype=torch.float64, device=pts.device) if store_flmn else None
+        arr[i] = Catalog(pts.data_ptr(), pts.shape[0], lmax, nmax, float(c["rmax"]), float(c.get("k_max", 0.0)),
+                         float(c.get("window_volume", 0.0)), k.data_ptr(), cl.data_ptr(),
+                         f.data_ptr() if f is not None else None)
+        keep.append(pts)
+        outs.append((k, cl, f))
+    if stream is None:
+        stream = torch.cuda.current_stream(keep[0].device if keep else None).cuda_stream
+    rc = lib().sfbhip_direct_batched(handle.ptr, arr, len(catalogs), ctypes.byref(p),
+                                     ctypes.c_void_p(stream) if stream else None)
+    check(rc, "sfbhip_direct_batched")
+    return outs
 
 
 def pixelize_torch(handle, points, nside, stream=None):

@@ -290,8 +290,20 @@ struct DirectPlan {
   int64_t partial_size = 0;
 };
 
+// Where one catalogue's items and groups point: its k_ln rows (k_{l,n} at
+// kbase + l nmax + n), its f_lmn block (fbase, in complex entries), its first
+// point in the point table (pbase), its normalisation and index.
+struct PlanCtx {
+  int nmax = 1;
+  double norm = 1.0;
+  int64_t kbase = 0, fbase = 0, pbase = 0;
+  int cat = 0;
+  const double *kln = nullptr;  // this catalogue's k_ln rows (host), for the table reach
+  double R = 0.0;               // max(rmax, largest radius)
+};
+
 void plan_direct(const std::vector<int32_t> &nl, int lmin, int lmax, int64_t npts, int nt, int tp, int mc,
-                 int qmax_cap, int64_t target_items, DirectPlan *pl) {
+                 int qmax_cap, int64_t target_items, const PlanCtx &cx, DirectPlan *pl) {
   int64_t ngroups = 0, passes_total = 0;
   for (int l = lmin; l < lmax; ++l) {
     const int64_t g = cdiv(nl[l], nt);
@@ -314,6 +326,11 @@ void plan_direct(const std::vector<int32_t> &nl, int lmin, int lmax, int64_t npt
       g.nsplit = (int32_t)S;
       g.base = pl->partial_size;
       g.stride = (int64_t)nt * ncol;
+      g.fofs = cx.fbase + (int64_t)l * (l + 1) / 2 * cx.nmax;
+      g.kofs = cx.kbase + (int64_t)l * cx.nmax + n0;
+      g.norm = cx.norm;
+      g.nmax = cx.nmax;
+      g.pad = 0;
       pl->partial_size += S * g.stride;
       pl->groups.push_back(g);
       for (int64_t s = 0; s < S; ++s) {
@@ -324,10 +341,15 @@ void plan_direct(const std::vector<int32_t> &nl, int lmin, int lmax, int64_t npt
           it.nrows = g.nrows;
           it.qlo = qlo;
           it.qhi = std::min(Q, qlo + qmax_cap);
+          it.cat = cx.cat;
+          it.tab = cx.kln ? (int32_t)std::min<double>(
+                                std::floor(cx.kln[(size_t)l * cx.nmax + n0 + g.nrows - 1] * cx.R / MX_H) + 2, 1e9)
+                          : 0;
           it.pad = 0;
-          it.p0 = s * chunk;
-          it.p1 = std::min(npts, (s + 1) * chunk);
+          it.p0 = cx.pbase + s * chunk;
+          it.p1 = cx.pbase + std::min(npts, (s + 1) * chunk);
           it.out = g.base + s * g.stride;
+          it.kofs = g.kofs;
           const int qc = qclass(it.qhi - it.qlo);
           int ci = 0;
           while ((1 << ci) < qc) ++ci;
@@ -430,7 +452,7 @@ sfbhip_status finalize(sfbhip_handle *h, const sfbhip_params *p, const std::vect
     int chunks = 1;
     for (const ReduceGroup &g : groups) chunks = std::max<int>(chunks, (int)cdiv((int64_t)g.nrows * 2 * (g.l + 1), 256));
     dim3 grid((unsigned)groups.size(), (unsigned)std::min(chunks, 64));
-    reduce_partials_kernel<<<grid, 256, 0, st>>>(dg, partial, dk, nmax, doff, norm, f);
+    reduce_partials_kernel<<<grid, 256, 0, st>>>(dg, partial, dk, f);
     HIPCHK(hipGetLastError());
   }
   if (L > 0) {
@@ -546,14 +568,11 @@ template <int QMAX, int NP>
 sfbhip_status launch_mixed(int nitems, const DirectItem *items, hipStream_t st, const PointsSoA &pts,
                            const double *kln, int nmax, const LegendreTables &lt, const HermiteTables &ht,
                            int lds_nodes, double *partial) {
-  if (lds_nodes > 0) {
-    const size_t shm = sizeof(float2) * (size_t)lds_nodes;
-    HIPCHK(hipFuncSetAttribute(reinterpret_cast<const void *>(&sfb_direct_mixed_kernel<QMAX, NP, true>),
+  const size_t shm = sizeof(float2) * (size_t)lds_nodes;
+  if (shm > 0)
+    HIPCHK(hipFuncSetAttribute(reinterpret_cast<const void *>(&sfb_direct_mixed_kernel<QMAX, NP>),
                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
-    sfb_direct_mixed_kernel<QMAX, NP, true><<<nitems, 256, shm, st>>>(pts, items, kln, nmax, lt, ht, partial);
-  } else {
-    sfb_direct_mixed_kernel<QMAX, NP, false><<<nitems, 256, 0, st>>>(pts, items, kln, nmax, lt, ht, partial);
-  }
+  sfb_direct_mixed_kernel<QMAX, NP><<<nitems, 256, shm, st>>>(pts, items, kln, nmax, lt, ht, lds_nodes, partial);
   HIPCHK(hipGetLastError());
   return SFBHIP_OK;
 }
@@ -592,33 +611,29 @@ sfbhip_status launch_plan_f64(const DirectPlan &pl, DirectItem *ditems, size_t *
   return SFBHIP_OK;
 }
 
-// Within each class, items whose l has a Bessel table of at most MX_LDS_NODES
-// nodes come first (they run the LDS-table kernel), the rest after them.
-void order_mixed_items(DirectPlan *pl, const std::vector<int32_t> &nint) {
-  for (auto &v : pl->items)
-    std::stable_partition(v.begin(), v.end(), [&](const DirectItem &it) { return nint[it.l] + 1 <= MX_LDS_NODES; });
-}
+// Bessel-table nodes an item needs in LDS
+int lds_reach(const DirectItem &it, const std::vector<int32_t> &nint) { return std::min(it.tab, nint[it.l]); }
 
+// One launch per class; its dynamic LDS holds the largest node table among
+// the class's items that fit (MX_LDS_NODES); the others read global memory.
 sfbhip_status launch_plan_mixed(const DirectPlan &pl, const std::vector<int32_t> &nint, DirectItem *ditems,
                                 size_t *off, hipStream_t st, const PointsSoA &pts, const double *dk, int nmax,
                                 const LegendreTables &lt, const HermiteTables &ht, int ops, double *partial) {
   for (int ci = 0; ci < 4; ++ci) {
     const auto &v = pl.items[ci];
-    int nlds = 0, nodes = 0;
-    while (nlds < (int)v.size() && nint[v[nlds].l] + 1 <= MX_LDS_NODES) nodes = std::max(nodes, nint[v[nlds++].l] + 1);
-    for (int part = 0; part < 2; ++part) {
-      const int n = part == 0 ? nlds : (int)v.size() - nlds;
-      if (!n) continue;
-      const int lds = part == 0 ? nodes : 0;
-      const DirectItem *di = ditems + *off;
-      switch (ci) {
-        case 0: CHK(launch_mixed_ops<1>(ops, n, di, st, pts, dk, nmax, lt, ht, lds, partial)); break;
-        case 1: CHK(launch_mixed_ops<2>(ops, n, di, st, pts, dk, nmax, lt, ht, lds, partial)); break;
-        case 2: CHK(launch_mixed_ops<4>(ops, n, di, st, pts, dk, nmax, lt, ht, lds, partial)); break;
-        default: CHK(launch_mixed_ops<8>(ops, n, di, st, pts, dk, nmax, lt, ht, lds, partial)); break;
-      }
-      *off += n;
+    const int n = (int)v.size();
+    if (!n) continue;
+    int nodes = 0;
+    for (const DirectItem &it : v)
+      if (lds_reach(it, nint) + 1 <= MX_LDS_NODES) nodes = std::max(nodes, lds_reach(it, nint) + 1);
+    const DirectItem *di = ditems + *off;
+    switch (ci) {
+      case 0: CHK(launch_mixed_ops<1>(ops, n, di, st, pts, dk, nmax, lt, ht, nodes, partial)); break;
+      case 1: CHK(launch_mixed_ops<2>(ops, n, di, st, pts, dk, nmax, lt, ht, nodes, partial)); break;
+      case 2: CHK(launch_mixed_ops<4>(ops, n, di, st, pts, dk, nmax, lt, ht, nodes, partial)); break;
+      default: CHK(launch_mixed_ops<8>(ops, n, di, st, pts, dk, nmax, lt, ht, nodes, partial)); break;
     }
+    *off += n;
   }
   return SFBHIP_OK;
 }
@@ -770,14 +785,20 @@ sfbhip_status sfbhip_direct(sfbhip_handle *h, const double *d_points, int64_t n_
   // l < lsplit in the general FP64 kernel (both with the quintic table)
   const int llow = mixed ? std::min(lsplit, LW_LMAX) : 0;
   DirectPlan pllw, pl64, plmx;
-  if (llow > 0) plan_direct(h->kc.nl, 0, llow, n_points, LW_ROWS, LW_STEP, F64_MC, 1, 4096, &pllw);
+  PlanCtx cx;
+  cx.nmax = p->nmax;
+  cx.norm = norm;
+  cx.kln = h->kc.kln.data();
+  cx.R = std::max(p->rmax, rdata);
+  if (llow > 0) plan_direct(h->kc.nl, 0, llow, n_points, LW_ROWS, LW_STEP, F64_MC, 1, 4096, cx, &pllw);
   pl64.partial_size = pllw.partial_size;
-  plan_direct(h->kc.nl, llow, lsplit, n_points, F64_NT, F64_TP, F64_MC, 16, 4096, &pl64);
+  plan_direct(h->kc.nl, llow, lsplit, n_points, F64_NT, F64_TP, F64_MC, 16, 4096, cx, &pl64);
   plmx.partial_size = pl64.partial_size;
   if (mixed) {
+    // large catalogues get more, smaller items: the launch's tail (one item) stays short
+    const int64_t target = std::min<int64_t>(16384, std::max<int64_t>(4096, n_points / 2048));
     plan_direct(h->kc.nl, lsplit, p->lmax, n_points, MX_NT, MX_TP, MX_MC, p->mixed_chunks ? p->mixed_chunks : 4,
-                4096, &plmx);
-    order_mixed_items(&plmx, h->kc.herm_nint);
+                target, cx, &plmx);
   }
   std::vector<ReduceGroup> groups = pllw.groups;
   groups.insert(groups.end(), pl64.groups.begin(), pl64.groups.end());
@@ -812,6 +833,312 @@ sfbhip_status sfbhip_direct(sfbhip_handle *h, const double *d_points, int64_t n_
   int e = 0;
   CHK(read_err(h, st, derr, &e));
   CHK(read_timings(h, mixed && lsplit < p->lmax ? 1 : 0));
+  if (e & ERR_BADPOINT) return fail(SFBHIP_DOMAIN_ERROR, "point with r < 0 or a non-finite coordinate");
+  if (e & ERR_DOMAIN) return fail(SFBHIP_DOMAIN_ERROR, "argument (z) outside of defined range.");
+  return SFBHIP_OK;
+}
+
+sfbhip_status sfbhip_direct_batched(sfbhip_handle *h, const sfbhip_catalog *cats, int32_t ncat,
+                                    const sfbhip_params *p, void *stream) {
+  g_last_error.clear();
+  if (!h) return fail(SFBHIP_INVALID_ARGUMENT, "handle is NULL");
+  if (!cats || ncat < 1 || ncat > 128) return fail(SFBHIP_INVALID_ARGUMENT, "ncat must be in [1, 128]");
+  if (!p) return fail(SFBHIP_INVALID_ARGUMENT, "params is NULL");
+  // per-catalogue parameters, each validated as a single call would be
+  std::vector<sfbhip_params> cp(ncat, *p);
+  int64_t ntot = 0;
+  int L = 0, NZ = 1;
+  for (int c = 0; c < ncat; ++c) {
+    cp[c].lmax = cats[c].lmax;
+    cp[c].nmax = cats[c].nmax;
+    cp[c].rmax = cats[c].rmax;
+    cp[c].k_max = cats[c].k_max;
+    cp[c].window_volume = cats[c].window_volume;
+    CHK(validate(&cp[c]));
+    if (cats[c].n_points <= 0 || !cats[c].d_points)
+      return fail(SFBHIP_INVALID_ARGUMENT, "catalogue " + std::to_string(c) +
+                                               ": empty catalogue: the normalisation divides by the number of points");
+    if (!cats[c].d_k_ln || !cats[c].d_c_ln)
+      return fail(SFBHIP_INVALID_ARGUMENT, "catalogue " + std::to_string(c) + ": k_ln / c_ln output is NULL");
+    ntot += cats[c].n_points;
+    L = std::max(L, cats[c].lmax);
+    NZ = std::max(NZ, cats[c].nmax + 1);
+  }
+  if (ntot > INT32_MAX) return fail(SFBHIP_NOT_SUPPORTED, "more than 2^31 points in one batch");
+  const bool mixed = p->precision == SFBHIP_PREC_MIXED;
+  if (mixed && L > MX_LMAX)
+    return fail(SFBHIP_NOT_SUPPORTED, "mixed path supports lmax <= " + std::to_string(MX_LMAX) + "; use precision FP64");
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  HIPCHK(hipSetDevice(h->device));
+  if (p->verbose) std::printf("SFB decomposition (raw method), batch of %d catalogues, %lld points\n", ncat, (long long)ntot);
+
+  int *derr;
+  CHK(zero_err(h, st, &derr));
+  // zeros of j_l, l < L, shared by all catalogues (the first nmax_c + 1 of them are
+  // what a single call with nmax_c computes)
+  std::vector<double> zeros((size_t)std::max(L, 1) * NZ, 0.0);
+  if (L > 0) {
+    double *dz;
+    CHK(get_buf(h, "b_zeros", (size_t)L * NZ, &dz));
+    bessel_zeros_kernel<<<cdiv(L, 64), 64, 0, st>>>(L, NZ, p->zeros, dz);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(zeros.data(), dz, sizeof(double) * L * NZ, hipMemcpyDeviceToHost, st));
+  }
+  // point tables, concatenated; largest radius per catalogue
+  double *soa;
+  unsigned long long *rbits;
+  CHK(get_buf(h, "b_soa", (size_t)6 * ntot, &soa));
+  CHK(get_buf(h, "b_rbits", (size_t)ncat, &rbits));
+  fill_kernel<<<1, 128, 0, st>>>(reinterpret_cast<double *>(rbits), ncat, 0.0);
+  HIPCHK(hipGetLastError());
+  std::vector<int64_t> pbase(ncat);
+  {
+    int64_t o = 0;
+    for (int c = 0; c < ncat; ++c) {
+      pbase[c] = o;
+      const int64_t n = cats[c].n_points;
+      prep_points_kernel<<<cdiv(n, 256), 256, 0, st>>>(cats[c].d_points, n, soa + o, soa + ntot + o,
+                                                       soa + 2 * ntot + o, soa + 3 * ntot + o, soa + 4 * ntot + o,
+                                                       soa + 5 * ntot + o, derr, rbits + c);
+      HIPCHK(hipGetLastError());
+      o += n;
+    }
+  }
+  std::vector<double> rdata(ncat);
+  HIPCHK(hipMemcpyAsync(rdata.data(), rbits, sizeof(double) * ncat, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  PointsSoA pts{soa, soa + ntot, soa + 2 * ntot, soa + 3 * ntot, soa + 4 * ntot, soa + 5 * ntot, ntot};
+  if (mixed && ntot > 1) {
+    // sort by (catalogue, radius): each catalogue stays in its range, ordered by r
+    unsigned long long *k0, *k1;
+    int32_t *i0, *i1;
+    double *sb;
+    CHK(get_buf(h, "sort_k0", (size_t)ntot, &k0));
+    CHK(get_buf(h, "sort_k1", (size_t)ntot, &k1));
+    CHK(get_buf(h, "sort_i0", (size_t)ntot, &i0));
+    CHK(get_buf(h, "sort_i1", (size_t)ntot, &i1));
+    CHK(get_buf(h, "b_soa_sorted", (size_t)6 * ntot, &sb));
+    for (int c = 0; c < ncat; ++c) {
+      radius_keys_cat_kernel<<<cdiv(cats[c].n_points, 256), 256, 0, st>>>(pts.r, cats[c].n_points, pbase[c], c, k0,
+                                                                         i0);
+      HIPCHK(hipGetLastError());
+    }
+    size_t tmp_bytes = 0;
+    HIPCHK(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_bytes, k0, k1, i0, i1, (int)ntot, 0, 64, st));
+    char *tmp;
+    CHK(get_buf(h, "sort_tmp", tmp_bytes, &tmp));
+    HIPCHK(hipcub::DeviceRadixSort::SortPairs(tmp, tmp_bytes, k0, k1, i0, i1, (int)ntot, 0, 64, st));
+    gather_points_kernel<<<cdiv(ntot, 256), 256, 0, st>>>(pts, i1, ntot, sb, sb + ntot, sb + 2 * ntot, sb + 3 * ntot,
+                                                         sb + 4 * ntot, sb + 5 * ntot);
+    HIPCHK(hipGetLastError());
+    pts = PointsSoA{sb, sb + ntot, sb + 2 * ntot, sb + 3 * ntot, sb + 4 * ntot, sb + 5 * ntot, ntot};
+  }
+
+  // k_ln, n_l, f_lmn offsets per catalogue
+  std::vector<double> kcat;
+  std::vector<int32_t> nlcat;
+  std::vector<int64_t> foffcat;
+  std::vector<int64_t> kbase(ncat), nlbase(ncat), fbase(ncat), foffbase(ncat);
+  std::vector<std::vector<int32_t>> nl(ncat);
+  int64_t ftot = 0;
+  for (int c = 0; c < ncat; ++c) {
+    const int Lc = cats[c].lmax, Nc = cats[c].nmax;
+    const double kmax = cats[c].k_max > 0 ? cats[c].k_max : 0.0;
+    kbase[c] = (int64_t)kcat.size();
+    nlbase[c] = (int64_t)nlcat.size();
+    foffbase[c] = (int64_t)foffcat.size();
+    fbase[c] = ftot;
+    nl[c].assign(std::max(Lc, 1), 0);
+    for (int l = 0; l < Lc; ++l) {
+      int cnt = 0;
+      for (int n = 0; n < Nc; ++n) {
+        const double z = zeros[(size_t)l * NZ + n];
+        kcat.push_back(z / cats[c].rmax);
+        if (kmax <= 0.0 || z < kmax * cats[c].rmax) cnt = n + 1;
+      }
+      nl[c][l] = cnt;
+    }
+    nlcat.insert(nlcat.end(), nl[c].begin(), nl[c].end());
+    std::vector<int64_t> fo = flmn_offsets(Lc, Nc);
+    foffcat.insert(foffcat.end(), fo.begin(), fo.end());
+    ftot += (int64_t)sfbhip_flmn_count(Lc, Nc);
+  }
+  double *dk;
+  int32_t *dnl;
+  int64_t *dfoff;
+  CHK(get_buf(h, "b_kln", std::max<size_t>(kcat.size(), 1), &dk));
+  CHK(get_buf(h, "b_nl", std::max<size_t>(nlcat.size(), 1), &dnl));
+  CHK(get_buf(h, "b_foff", std::max<size_t>(foffcat.size(), 1), &dfoff));
+  if (!kcat.empty()) HIPCHK(hipMemcpyAsync(dk, kcat.data(), sizeof(double) * kcat.size(), hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(dnl, nlcat.data(), sizeof(int32_t) * nlcat.size(), hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(dfoff, foffcat.data(), sizeof(int64_t) * foffcat.size(), hipMemcpyHostToDevice, st));
+
+  // tables shared by all catalogues: Legendre constants for l < L; Bessel
+  // tables per l long enough for the largest k_{l, n_l - 1} max(rmax, max r)
+  LegendreTables lt;
+  CHK(ensure_legendre(h, std::max(L, 1), st, &lt));
+  const int lsplit = mixed ? std::min(L, std::max(0, p->mixed_fp64_below)) : L;
+  std::vector<double> xmax(std::max(L, 1), 0.0);
+  for (int c = 0; c < ncat; ++c)
+    for (int l = 0; l < cats[c].lmax; ++l)
+      if (nl[c][l] > 0)
+        xmax[l] = std::max(xmax[l], kcat[kbase[c] + (int64_t)l * cats[c].nmax + nl[c][l] - 1] *
+                                        std::max(cats[c].rmax, rdata[c]));
+  BesselTableRef tab{};
+  HermiteTables ht{nullptr, nullptr, nullptr};
+  std::vector<int32_t> hnint(std::max(L, 1), 1);
+  if (mixed && lsplit > 0) {
+    std::vector<int32_t> off(lsplit), ni(lsplit);
+    int64_t o = 0;
+    int32_t mx = 1;
+    for (int l = 0; l < lsplit; ++l) {
+      ni[l] = (int32_t)std::floor(xmax[l] * Q5_HINV) + 2;
+      off[l] = (int32_t)o;
+      o += ni[l];
+      mx = std::max(mx, ni[l]);
+    }
+    double *q5;
+    int32_t *doff, *dni;
+    CHK(get_buf(h, "b_q5_tab", (size_t)6 * o, &q5));
+    CHK(get_buf(h, "b_q5_off", off.size(), &doff));
+    CHK(get_buf(h, "b_q5_nint", ni.size(), &dni));
+    HIPCHK(hipMemcpyAsync(doff, off.data(), sizeof(int32_t) * off.size(), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(dni, ni.data(), sizeof(int32_t) * ni.size(), hipMemcpyHostToDevice, st));
+    quintic_table_kernel<<<dim3((unsigned)cdiv(mx, 256), (unsigned)lsplit), 256, 0, st>>>(lsplit, doff, dni, q5);
+    HIPCHK(hipGetLastError());
+    tab.q5 = q5;
+    tab.q5_off = doff;
+    tab.q5_nint = dni;
+    HIPCHK(hipStreamSynchronize(st));  // host vectors go out of scope
+  }
+  if (mixed && lsplit < L) {
+    std::vector<int32_t> off(L, 0);
+    int64_t o = 0;
+    int32_t mx = 1;
+    for (int l = 0; l < L; ++l) {
+      const int64_t n = (int64_t)std::floor(xmax[l] / MX_H) + 2;
+      if (n > MX_TAB_MAX)
+        return fail(SFBHIP_NOT_SUPPORTED, "mixed path: largest Bessel argument k r exceeds " +
+                                              std::to_string(MX_TAB_MAX * MX_H) + "; use precision FP64");
+      hnint[l] = (int32_t)n;
+      off[l] = (int32_t)o;
+      o += n;
+      mx = std::max<int32_t>(mx, (int32_t)n);
+      if (o > INT32_MAX) return fail(SFBHIP_NOT_SUPPORTED, "mixed path: Bessel tables too large");
+    }
+    float4 *tb;
+    int32_t *doff, *dni;
+    CHK(get_buf(h, "b_herm_tab", (size_t)o, &tb));
+    CHK(get_buf(h, "b_herm_off", off.size(), &doff));
+    CHK(get_buf(h, "b_herm_nint", hnint.size(), &dni));
+    HIPCHK(hipMemcpyAsync(doff, off.data(), sizeof(int32_t) * off.size(), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(dni, hnint.data(), sizeof(int32_t) * hnint.size(), hipMemcpyHostToDevice, st));
+    hermite_table_kernel<<<dim3((unsigned)cdiv(mx, 256), (unsigned)L), 256, 0, st>>>(L, doff, dni, tb);
+    HIPCHK(hipGetLastError());
+    ht = HermiteTables{tb, doff, dni};
+    HIPCHK(hipStreamSynchronize(st));
+  }
+
+  // one plan for all catalogues: work items in proportion to each catalogue's work
+  double wtot = 0.0;
+  std::vector<double> work(ncat, 0.0);
+  for (int c = 0; c < ncat; ++c) {
+    for (int l = 0; l < cats[c].lmax; ++l) work[c] += (double)(l + 1) * nl[c][l];
+    work[c] *= (double)cats[c].n_points;
+    wtot += work[c];
+  }
+  const int llow = mixed ? std::min(lsplit, LW_LMAX) : 0;
+  DirectPlan pllw, pl64, plmx;
+  auto ctx = [&](int c) {
+    PlanCtx cx;
+    cx.nmax = cats[c].nmax;
+    cx.norm = norm_factor(&cp[c], (double)cats[c].n_points);
+    cx.kbase = kbase[c];
+    cx.fbase = fbase[c];
+    cx.pbase = pbase[c];
+    cx.cat = c;
+    cx.kln = kcat.data() + kbase[c];
+    cx.R = std::max(cats[c].rmax, rdata[c]);
+    return cx;
+  };
+  const double ttot = (double)std::min<int64_t>(16384, std::max<int64_t>(4096, ntot / 2048));
+  auto target = [&](int c) { return std::max<int64_t>(16, (int64_t)(ttot * work[c] / std::max(wtot, 1.0))); };
+  for (int c = 0; c < ncat; ++c)
+    if (llow > 0)
+      plan_direct(nl[c], 0, std::min(llow, cats[c].lmax), cats[c].n_points, LW_ROWS, LW_STEP, F64_MC, 1, target(c),
+                  ctx(c), &pllw);
+  pl64.partial_size = pllw.partial_size;
+  for (int c = 0; c < ncat; ++c)
+    plan_direct(nl[c], std::min(llow, cats[c].lmax), std::min(lsplit, cats[c].lmax), cats[c].n_points, F64_NT,
+                F64_TP, F64_MC, 16, target(c), ctx(c), &pl64);
+  plmx.partial_size = pl64.partial_size;
+  if (mixed) {
+    for (int c = 0; c < ncat; ++c)
+      plan_direct(nl[c], std::min(lsplit, cats[c].lmax), cats[c].lmax, cats[c].n_points, MX_NT, MX_TP, MX_MC,
+                  p->mixed_chunks ? p->mixed_chunks : 4, target(c), ctx(c), &plmx);
+  }
+  std::vector<ReduceGroup> groups = pllw.groups;
+  groups.insert(groups.end(), pl64.groups.begin(), pl64.groups.end());
+  groups.insert(groups.end(), plmx.groups.begin(), plmx.groups.end());
+  double *partial = nullptr;
+  CHK(get_buf(h, "partial", (size_t)std::max<int64_t>(plmx.partial_size, 1), &partial));
+  std::vector<DirectItem> all;
+  for (auto &v : pllw.items) all.insert(all.end(), v.begin(), v.end());
+  for (auto &v : pl64.items) all.insert(all.end(), v.begin(), v.end());
+  for (auto &v : plmx.items) all.insert(all.end(), v.begin(), v.end());
+  DirectItem *ditems;
+  CHK(get_buf(h, "items", std::max<size_t>(all.size(), 1), &ditems));
+  if (!all.empty())
+    HIPCHK(hipMemcpyAsync(ditems, all.data(), sizeof(DirectItem) * all.size(), hipMemcpyHostToDevice, st));
+
+  clear_timings(h);
+  HIPCHK(hipEventRecord(h->ev[0], st));
+  size_t off = 0;
+  if (!pllw.items[0].empty()) {
+    const int n = (int)pllw.items[0].size();
+    sfb_lowl_f64_kernel<<<n, 256, 0, st>>>(pts, ditems + off, dk, 0, tab, partial);
+    HIPCHK(hipGetLastError());
+    off += n;
+  }
+  CHK(launch_plan_f64(pl64, ditems, &off, st, pts, nullptr, dk, 0, lt, mixed ? JL_QUINTIC : JL_EXACT, tab, partial,
+                      derr));
+  HIPCHK(hipEventRecord(h->ev[1], st));
+  if (mixed) CHK(launch_plan_mixed(plmx, hnint, ditems, &off, st, pts, dk, 0, lt, ht, p->mixed_operands, partial));
+  HIPCHK(hipEventRecord(h->ev[2], st));
+
+  // f_lmn of all catalogues in one workspace: zero, reduce, then C_l, k_ln and f_lmn per catalogue
+  double *f;
+  CHK(get_buf(h, "b_flmn", (size_t)std::max<int64_t>(2 * ftot, 1), &f));
+  if (ftot) {
+    fill_kernel<<<cdiv(2 * ftot, 256), 256, 0, st>>>(f, 2 * ftot, 0.0);
+    HIPCHK(hipGetLastError());
+  }
+  ReduceGroup *dg;
+  CHK(get_buf(h, "groups", std::max<size_t>(groups.size(), 1), &dg));
+  if (!groups.empty()) {
+    HIPCHK(hipMemcpyAsync(dg, groups.data(), sizeof(ReduceGroup) * groups.size(), hipMemcpyHostToDevice, st));
+    int chunks = 1;
+    for (const ReduceGroup &g : groups)
+      chunks = std::max<int>(chunks, (int)cdiv((int64_t)g.nrows * 2 * (g.l + 1), 256));
+    reduce_partials_kernel<<<dim3((unsigned)groups.size(), (unsigned)std::min(chunks, 64)), 256, 0, st>>>(dg, partial,
+                                                                                                      dk, f);
+    HIPCHK(hipGetLastError());
+  }
+  for (int c = 0; c < ncat; ++c) {
+    const int Lc = cats[c].lmax, Nc = cats[c].nmax;
+    if (Lc == 0) continue;
+    cln_kernel<<<cdiv((int64_t)Lc * Nc, 256), 256, 0, st>>>(Lc, Nc, dnl + nlbase[c], dfoff + foffbase[c],
+                                                           f + 2 * fbase[c], cats[c].d_c_ln);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(cats[c].d_k_ln, dk + kbase[c], sizeof(double) * Lc * Nc, hipMemcpyDeviceToDevice, st));
+    if (cats[c].d_f_lmn)
+      HIPCHK(hipMemcpyAsync(cats[c].d_f_lmn, f + 2 * fbase[c], sizeof(double) * 2 * sfbhip_flmn_count(Lc, Nc),
+                            hipMemcpyDeviceToDevice, st));
+  }
+  HIPCHK(hipEventRecord(h->ev[3], st));
+  int e = 0;
+  CHK(read_err(h, st, derr, &e));
+  CHK(read_timings(h, mixed && lsplit < L ? 1 : 0));
   if (e & ERR_BADPOINT) return fail(SFBHIP_DOMAIN_ERROR, "point with r < 0 or a non-finite coordinate");
   if (e & ERR_DOMAIN) return fail(SFBHIP_DOMAIN_ERROR, "argument (z) outside of defined range.");
   return SFBHIP_OK;
@@ -930,7 +1257,10 @@ sfbhip_status sfbhip_healpix(sfbhip_handle *h, const int64_t *d_offsets, const d
   BesselTableRef tab{nullptr, nullptr, nullptr, 0};
   if (use_table) CHK(ensure_table(h, p, st, &tab));
   DirectPlan pl;
-  plan_direct(h->kc.nl, 0, p->lmax, npix, F64_NT, F64_TP, F64_MC, 16, 4096, &pl);
+  PlanCtx cx;
+  cx.nmax = p->nmax;
+  cx.norm = norm;
+  plan_direct(h->kc.nl, 0, p->lmax, npix, F64_NT, F64_TP, F64_MC, 16, 4096, cx, &pl);
   double *partial;
   CHK(get_buf(h, "partial", (size_t)std::max<int64_t>(pl.partial_size, 1), &partial));
   std::vector<DirectItem> all;

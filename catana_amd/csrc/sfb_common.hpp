@@ -42,13 +42,18 @@ struct DirectItem {
   int32_t n0;
   int32_t nrows;     // valid rows (<= tile height)
   int32_t qlo, qhi;  // m chunks [qlo, qhi) handled by this item
+  int32_t cat;       // catalogue of a batched call (0 otherwise)
+  int32_t tab;       // mixed path: Bessel-table intervals the item's arguments reach
   int32_t pad;
-  int64_t p0, p1;
+  int64_t p0, p1;    // point (or pixel) range
   int64_t out;       // offset (in doubles) of this item's partial block
+  int64_t kofs;      // offset of k_{l, n0} in the k_ln array of the call
 };
 
-// One (l, row-tile) group: its items are `nsplit` consecutive point ranges
-// whose partial blocks sit `stride` doubles apart starting at `base`.
+// One (catalogue, l, row-tile) group: its items are `nsplit` consecutive point
+// ranges whose partial blocks sit `stride` doubles apart starting at `base`.
+// The sum lands in f (complex, interleaved) at ((fofs + m nmax + n) * 2 + re/im),
+// scaled by norm * k_ln with k_ln at kofs + row.
 struct ReduceGroup {
   int32_t l;
   int32_t n0;
@@ -56,6 +61,11 @@ struct ReduceGroup {
   int32_t nsplit;
   int64_t base;
   int64_t stride;
+  int64_t fofs;
+  int64_t kofs;
+  double norm;
+  int32_t nmax;
+  int32_t pad;
 };
 
 // Per-l constants for the Legendre rows:

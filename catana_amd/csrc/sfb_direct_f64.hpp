@@ -133,7 +133,7 @@ __global__ __launch_bounds__(256) void sfb_direct_f64_kernel(
   const int wave = tid >> 6;
   const int rt = wave >> 1, ct = wave & 1;
 
-  const double *kl = kln + (size_t)l * nmax + it.n0;
+  const double *kl = kln + it.kofs;
   const double2 sect = leg.sect[l];
   const double2 *coef = leg.coef + leg.coef_off[l];
 

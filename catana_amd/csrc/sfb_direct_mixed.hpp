@@ -161,13 +161,14 @@ __device__ __forceinline__ void emit_y(const YlmDown &y, float *Yf, _Float16 *Yh
   }
 }
 
-// TL: the Bessel table of the item's l is copied into LDS (dynamic shared
-// memory, nint + 1 nodes) when it is small enough to keep two blocks per CU;
-// otherwise it is read from global memory (L1/L2).
-template <int QMAX, int NP, bool TL>
+// The Bessel intervals an item's arguments reach (it.tab) are copied into LDS
+// (dynamic shared memory of lds_nodes nodes, sized for two blocks per CU) when
+// they fit; otherwise the item reads the table from global memory (L1/L2).
+// The choice is per block, so one launch serves both kinds of items.
+template <int QMAX, int NP>
 __global__ __launch_bounds__(256) void sfb_direct_mixed_kernel(
     PointsSoA pts, const DirectItem *__restrict__ items, const double *__restrict__ kln, int nmax,
-    LegendreTables leg, HermiteTables ht, double *__restrict__ partial) {
+    LegendreTables leg, HermiteTables ht, int lds_nodes, double *__restrict__ partial) {
   static_assert(NP == 0 || NP == 2 || NP == 3, "operand pieces");
   // row pitch of the Y* chunk (elements): a column starts 4 banks after the
   // previous one, so the 16-byte B-fragment reads of 8 consecutive columns
@@ -199,20 +200,16 @@ __global__ __launch_bounds__(256) void sfb_direct_mixed_kernel(
 
   const int nint = ht.nint[l];
   const float4 *__restrict__ T = ht.tab + ht.off[l];
-  extern __shared__ float2 TN[];  // TL: node table
-  if constexpr (TL) {
-    for (int i = tid; i <= nint; i += 256) {
-      const float4 q = T[i < nint ? i : nint - 1];
-      TN[i] = i < nint ? make_float2(q.x, q.y) : make_float2(q.z, q.w);
+  extern __shared__ float2 TN[];  // node table of the intervals this item's arguments reach
+  const int ntab = min(nint, it.tab);
+  const bool tl = ntab + 1 <= lds_nodes;  // block-uniform
+  if (tl) {
+    for (int i = tid; i <= ntab; i += 256) {
+      const float4 q = T[i < ntab ? i : ntab - 1];
+      TN[i] = i < ntab ? make_float2(q.x, q.y) : make_float2(q.z, q.w);
     }
   }
-  auto jl = [&](double t) -> float {
-    if constexpr (TL)
-      return jl_hermite_lds(TN, nint, t);
-    else
-      return jl_hermite(T, nint, t);
-  };
-  const double khr = rowok ? kln[(size_t)l * nmax + it.n0 + row] * (1.0 / MX_H) : 0.0;
+  const double khr = rowok ? kln[it.kofs + row] * (1.0 / MX_H) : 0.0;
   const double2 sect = leg.sect[l];
   {
     const double2 *gc = leg.coef + leg.coef_off[l];
@@ -235,31 +232,61 @@ __global__ __launch_bounds__(256) void sfb_direct_mixed_kernel(
     float af[NP ? 1 : MX_TP / 4];
     h8 ah[NA][NP ? MX_TP / 32 : 1];
     if (wave_rows) {
-      if constexpr (NP) {
+      if (tl) {
+        if constexpr (NP) {
 #pragma unroll
-        for (int s = 0; s < MX_TP / 32; ++s) {
-          h2 pc[3][4];
+          for (int s = 0; s < MX_TP / 32; ++s) {
+            h2 pc[3][4];
 #pragma unroll
-          for (int e = 0; e < 8; e += 2) {
-            const double r0 = rs[32 * s + 8 * g + e], r1 = rs[32 * s + 8 * g + e + 1];
-            const float v0 = (rowok && r0 >= 0.0) ? jl(r0 * khr) * MX_SCALE : 0.f;
-            const float v1 = (rowok && r1 >= 0.0) ? jl(r1 * khr) * MX_SCALE : 0.f;
-            if constexpr (NP == 2)
-              split_pair(v0, v1, pc[0][e / 2], pc[1][e / 2]);
-            else
-              split3_pair(v0, v1, pc[0][e / 2], pc[1][e / 2], pc[2][e / 2]);
+            for (int e = 0; e < 8; e += 2) {
+              const double r0 = rs[32 * s + 8 * g + e], r1 = rs[32 * s + 8 * g + e + 1];
+              const float v0 = (rowok && r0 >= 0.0) ? jl_hermite_lds(TN, ntab, r0 * khr) * MX_SCALE : 0.f;
+              const float v1 = (rowok && r1 >= 0.0) ? jl_hermite_lds(TN, ntab, r1 * khr) * MX_SCALE : 0.f;
+              if constexpr (NP == 2)
+                split_pair(v0, v1, pc[0][e / 2], pc[1][e / 2]);
+              else
+                split3_pair(v0, v1, pc[0][e / 2], pc[1][e / 2], pc[2][e / 2]);
+            }
+#pragma unroll
+            for (int k = 0; k < NA; ++k)
+              ah[k][s] = __builtin_shufflevector(__builtin_shufflevector(pc[k][0], pc[k][1], 0, 1, 2, 3),
+                                                 __builtin_shufflevector(pc[k][2], pc[k][3], 0, 1, 2, 3), 0, 1, 2,
+                                                 3, 4, 5, 6, 7);
           }
+        } else {
 #pragma unroll
-          for (int k = 0; k < NA; ++k)
-            ah[k][s] = __builtin_shufflevector(__builtin_shufflevector(pc[k][0], pc[k][1], 0, 1, 2, 3),
-                                               __builtin_shufflevector(pc[k][2], pc[k][3], 0, 1, 2, 3), 0, 1, 2,
-                                               3, 4, 5, 6, 7);
+          for (int s = 0; s < MX_TP / 4; ++s) {
+            const double r = rs[4 * s + g];
+            af[s] = (rowok && r >= 0.0) ? jl_hermite_lds(TN, ntab, r * khr) : 0.f;
+          }
         }
       } else {
+        if constexpr (NP) {
 #pragma unroll
-        for (int s = 0; s < MX_TP / 4; ++s) {
-          const double r = rs[4 * s + g];
-          af[s] = (rowok && r >= 0.0) ? jl(r * khr) : 0.f;
+          for (int s = 0; s < MX_TP / 32; ++s) {
+            h2 pc[3][4];
+#pragma unroll
+            for (int e = 0; e < 8; e += 2) {
+              const double r0 = rs[32 * s + 8 * g + e], r1 = rs[32 * s + 8 * g + e + 1];
+              const float v0 = (rowok && r0 >= 0.0) ? jl_hermite(T, nint, r0 * khr) * MX_SCALE : 0.f;
+              const float v1 = (rowok && r1 >= 0.0) ? jl_hermite(T, nint, r1 * khr) * MX_SCALE : 0.f;
+              if constexpr (NP == 2)
+                split_pair(v0, v1, pc[0][e / 2], pc[1][e / 2]);
+              else
+                split3_pair(v0, v1, pc[0][e / 2], pc[1][e / 2], pc[2][e / 2]);
+            }
+#pragma unroll
+            for (int k = 0; k < NA; ++k)
+              ah[k][s] = __builtin_shufflevector(__builtin_shufflevector(pc[k][0], pc[k][1], 0, 1, 2, 3),
+                                                 __builtin_shufflevector(pc[k][2], pc[k][3], 0, 1, 2, 3), 0, 1, 2,
+                                                 3, 4, 5, 6, 7);
+          }
+        } else {
+#pragma unroll
+          for (int s = 0; s < MX_TP / 4; ++s) {
+            const double r = rs[4 * s + g];
+            af[s] = (rowok && r >= 0.0) ? jl_hermite(T, nint, r * khr) : 0.f;
+          }
         }
       }
     }

@@ -52,7 +52,7 @@ __global__ __launch_bounds__(256) void sfb_lowl_f64_kernel(PointsSoA pts, const 
 
   const int col = lane & 15, m = col >> 1, kq = lane >> 4;
   const bool live_col = m <= l;
-  const double *kl = kln + (size_t)l * nmax + it.n0;
+  const double *kl = kln + it.kofs;
   double kr[4];
 #pragma unroll
   for (int t = 0; t < 4; ++t) {

@@ -51,6 +51,17 @@ __global__ void radius_keys_kernel(const double *__restrict__ r, int64_t n, unsi
   idx[i] = (int32_t)i;
 }
 
+// Keys of a batched call: catalogue index in the top 7 bits, then the top 57
+// bits of r's pattern (order within a catalogue only needs to be approximate).
+__global__ void radius_keys_cat_kernel(const double *__restrict__ r, int64_t n, int64_t base, int cat,
+                                       unsigned long long *__restrict__ keys, int32_t *__restrict__ idx) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const unsigned long long b = (unsigned long long)__double_as_longlong(fmax(r[base + i], 0.0));
+  keys[base + i] = ((unsigned long long)cat << 57) | (b >> 7);
+  idx[base + i] = (int32_t)(base + i);
+}
+
 // out[i] = in[idx[i]] for the six arrays of the point table.
 __global__ void gather_points_kernel(PointsSoA in, const int32_t *__restrict__ idx, int64_t n,
                                      double *__restrict__ r, double *__restrict__ x, double *__restrict__ s,
@@ -86,11 +97,10 @@ __global__ void bessel_table_kernel(int lmax, int P, const double *__restrict__ 
 }
 
 // Sum the partial blocks of a group in split order and scale by norm * k_ln:
-// f[l][m][n] (complex, m >= 0, n < nmax).  Block (g, c) sums elements c, c + gridDim.y*256, ... of group g.
+// f[l][m][n] (complex, m >= 0, n < nmax) of the group's catalogue.
+// Block (g, c) sums elements c, c + gridDim.y*256, ... of group g.
 __global__ void reduce_partials_kernel(const ReduceGroup *__restrict__ groups, const double *__restrict__ partial,
-                                       const double *__restrict__ kln, int nmax,
-                                       const int64_t *__restrict__ flmn_off, double norm,
-                                       double *__restrict__ flmn) {
+                                       const double *__restrict__ kln, double *__restrict__ flmn) {
   const ReduceGroup g = groups[blockIdx.x];
   const int ncol = 2 * (g.l + 1);
   const int total = g.nrows * ncol;
@@ -101,8 +111,7 @@ __global__ void reduce_partials_kernel(const ReduceGroup *__restrict__ groups, c
     for (int s = 0; s < g.nsplit; ++s) acc += src[(size_t)s * g.stride];
     const int n = g.n0 + row;
     const int m = col >> 1;
-    const double k = kln[(size_t)g.l * nmax + n];
-    flmn[(flmn_off[g.l] + (size_t)m * nmax + n) * 2 + (col & 1)] = acc * norm * k;
+    flmn[(g.fofs + (size_t)m * g.nmax + n) * 2 + (col & 1)] = acc * g.norm * kln[g.kofs + row];
   }
 }
 

@@ -107,6 +107,28 @@ sfbhip_status sfbhip_direct(sfbhip_handle *h, const double *d_points, int64_t n_
                             const sfbhip_params *p, double *d_k_ln, double *d_c_ln, double *d_f_lmn,
                             void *stream);
 
+/* One catalogue of a batched call (device buffers, shapes as for sfbhip_direct). */
+typedef struct sfbhip_catalog {
+  const double *d_points;   /* AoS (r, theta, phi), n_points records */
+  int64_t n_points;
+  int32_t lmax, nmax;       /* lmax exclusive; columns of k_ln / c_ln */
+  double rmax, k_max, window_volume;
+  double *d_k_ln, *d_c_ln;  /* lmax*nmax doubles each */
+  double *d_f_lmn;          /* NULL or 2*sfbhip_flmn_count(lmax, nmax) doubles */
+} sfbhip_catalog;
+
+/* Direct SFB of up to 128 catalogues with their own sizes, lmax, nmax, rmax,
+ * k_max and window volume, in one grouped launch per kernel: the work items of
+ * all catalogues are planned together, the Bessel and Legendre tables are
+ * shared (they depend on l only), and one reduction writes every catalogue's
+ * f_lmn.  precision, zeros, verbose and the mixed_* fields come from *p (its
+ * lmax, nmax, rmax, k_max, window_volume are not used); j_l is exact in the
+ * FP64 path.  No counterpart in the reference, where a caller loops over
+ * catana::sfb_decomposition; each catalogue's result equals that of
+ * sfbhip_direct on it alone, up to summation order. */
+sfbhip_status sfbhip_direct_batched(sfbhip_handle *h, const sfbhip_catalog *cats, int32_t ncat,
+                                    const sfbhip_params *p, void *stream);
+
 /* HEALPix RING binning of device points into CSR order by pixel: d_offsets
  * (12 nside^2 + 1 int64), d_radii (n_points doubles), radii of a pixel kept in
  * input order.  Replaces PixelizedPointContainer(nside, pc)

@@ -1,0 +1,80 @@
+"""Batched direct SFB (sfbhip_direct_batched): several catalogues with their
+own sizes, lmax, nmax, rmax, k_max and window volume in one call, each
+checked against the oracle (FP64) and against the FP64 batch (mixed)."""
+import numpy as np
+import pytest
+import torch
+
+from oracle import oracle as O
+from catana_amd import capi, workloads
+
+pytestmark = pytest.mark.gpu
+
+SPECS = [  # (N, lmax, nmax, rmax, k_max, window volume, generator)
+    (3000, 12, 8, 300.0, 0.0, 0.0, "uniform"),
+    (20000, 40, 30, 800.0, 0.12, 0.0, "clustered"),
+    (517, 9, 5, 150.0, 0.0, 2.5e7, "uniform"),
+    (9000, 33, 20, 1200.0, 0.05, 0.0, "uniform"),
+    (1, 6, 4, 50.0, 0.0, 0.0, "uniform"),
+]
+
+
+def _catalogs():
+    out = []
+    for i, (n, lmax, nmax, rmax, kmax, vol, gen) in enumerate(SPECS):
+        if gen == "uniform":
+            pts = workloads.uniform_ball(n, rmax, seed=40 + i)
+        else:
+            pts = workloads.clustered_ball(n // 40, rmax, mean_children=40, sigma=6.0, seed=40 + i)
+        out.append(dict(np=pts, points=torch.from_numpy(pts).cuda(), lmax=lmax, nmax=nmax, rmax=rmax, k_max=kmax,
+                        window_volume=vol))
+    return out
+
+
+def _flmn_err(a, b):
+    return max(np.abs(x - y).max() / max(np.abs(y).max(), 1e-300) for x, y in zip(a, b))
+
+
+def test_batched_fp64_vs_oracle(handle):
+    cats = _catalogs()
+    print("case: batched FP64, 5 catalogues", flush=True)
+    res = capi.sfb_direct_batched_torch(handle, cats, store_flmn=True, precision=capi.PREC_FP64)
+    torch.cuda.synchronize()
+    for c, (k, cl, f) in zip(cats, res):
+        nl = O.count_nl(c["lmax"], c["nmax"], c["rmax"], c["k_max"]) if c["k_max"] > 0 else None
+        ko, co, fo = O.sfb_raw(c["np"], c["lmax"], c["nmax"], c["rmax"], nl=nl, volume=c["window_volume"],
+                               store_flmn=True)
+        np.testing.assert_allclose(k.cpu().numpy(), ko, rtol=1e-14)
+        np.testing.assert_allclose(cl.cpu().numpy(), co, rtol=1e-10)
+        assert _flmn_err(capi.split_flmn(f.cpu().numpy(), c["lmax"], c["nmax"]), fo) < 1e-10
+
+
+def test_batched_mixed_vs_fp64_and_single(handle):
+    cats = _catalogs()
+    print("case: batched mixed vs batched FP64 and single calls", flush=True)
+    r64 = capi.sfb_direct_batched_torch(handle, cats, store_flmn=True, precision=capi.PREC_FP64)
+    rmx = capi.sfb_direct_batched_torch(handle, cats, store_flmn=True, precision=capi.PREC_MIXED)
+    torch.cuda.synchronize()
+    for c, (k64, c64, f64), (km, cm, fm) in zip(cats, r64, rmx):
+        a, b = cm.cpu().numpy(), c64.cpu().numpy()
+        assert np.all((a == 0) == (b == 0))
+        m = b != 0
+        assert np.max(np.abs(a[m] / b[m] - 1)) < 1e-6
+        L, N = c["lmax"], c["nmax"]
+        assert _flmn_err(capi.split_flmn(fm.cpu().numpy(), L, N), capi.split_flmn(f64.cpu().numpy(), L, N)) < 1e-6
+        # the same catalogue alone through sfbhip_direct
+        ks, cs, _ = capi.sfb_direct_torch(handle, c["points"], L, N, c["rmax"], k_max=c["k_max"],
+                                          window_volume=c["window_volume"], precision=capi.PREC_MIXED,
+                                          bessel=capi.BESSEL_EXACT)
+        s = cs.cpu().numpy()
+        assert np.max(np.abs(a[m] / s[m] - 1)) < 1e-6
+
+
+def test_batched_errors(handle):
+    cats = _catalogs()[:2]
+    with pytest.raises(ValueError, match="ncat"):
+        capi.sfb_direct_batched_torch(handle, [], precision=capi.PREC_FP64)
+    bad = dict(cats[1])
+    bad["points"] = torch.zeros((0, 3), dtype=torch.float64, device="cuda")
+    with pytest.raises(ValueError, match="catalogue 1"):
+        capi.sfb_direct_batched_torch(handle, [cats[0], bad], precision=capi.PREC_FP64)
