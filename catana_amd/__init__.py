@@ -13,8 +13,8 @@ __version__ = "0.1.0"
 
 _API = (
     "Point", "PointContainer", "PixelizedPointContainer", "KClkk", "sfb_decomposition",
-    "sfb_decomposition_reverse", "sfb_decomposition_healpix", "compute_kln", "_sfb_raw", "_sfb_reverse",
-    "_sfb_reverse_fft",
+    "sfb_decomposition_reverse", "sfb_decomposition_reverse_batched", "sfb_decomposition_healpix",
+    "compute_kln", "_sfb_raw", "_sfb_reverse", "_sfb_reverse_fft",
 )
 __all__ = [n for n in _API if not n.startswith("_")]
 

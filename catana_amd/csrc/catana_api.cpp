@@ -312,6 +312,75 @@ KClkk sfb_decomposition_reverse(const PointContainer &points, unsigned short lma
   return run_direct(points, lmax, nmax, rmax, store_flmn, verbose, o);
 }
 
+std::vector<KClkk> sfb_decomposition_reverse_batched(const std::vector<CatalogSpec> &cats, bool store_flmn,
+                                                     bool verbose, int precision) {
+  Context &c = ctx();
+  std::lock_guard<std::mutex> lk(c.mu);
+  // one device region for all points, one for all outputs
+  size_t npts = 0, nout = 0;
+  std::vector<size_t> poff(cats.size()), koff(cats.size()), foff(cats.size());
+  for (size_t i = 0; i < cats.size(); ++i) {
+    if (!cats[i].points) throw std::invalid_argument("catalogue " + std::to_string(i) + ": no points");
+    poff[i] = npts;
+    npts += cats[i].points->size();
+    koff[i] = nout;
+    nout += 2 * (size_t)cats[i].lmax * cats[i].nmax;  // k_ln, c_ln
+    foff[i] = nout;
+    if (store_flmn) nout += 2 * sfbhip_flmn_count(cats[i].lmax, cats[i].nmax);
+  }
+  c.pts.reserve(std::max<size_t>(npts * sizeof(Point), 24));
+  c.flmn.reserve(std::max<size_t>(nout * sizeof(double), 8));
+  std::vector<sfbhip_catalog> cc(cats.size());
+  double *obuf = static_cast<double *>(c.flmn.p);
+  for (size_t i = 0; i < cats.size(); ++i) {
+    const size_t n = cats[i].points->size();
+    if (n)
+      hip_check(hipMemcpy(static_cast<char *>(c.pts.p) + poff[i] * sizeof(Point), cats[i].points->data(),
+                          n * sizeof(Point), hipMemcpyHostToDevice),
+                "copy points");
+    const size_t k2 = (size_t)cats[i].lmax * cats[i].nmax;
+    cc[i].d_points = reinterpret_cast<const double *>(static_cast<char *>(c.pts.p) + poff[i] * sizeof(Point));
+    cc[i].n_points = (int64_t)n;
+    cc[i].lmax = cats[i].lmax;
+    cc[i].nmax = cats[i].nmax;
+    cc[i].rmax = cats[i].rmax;
+    cc[i].k_max = cats[i].k_max;
+    cc[i].window_volume = cats[i].window_volume;
+    cc[i].d_k_ln = obuf + koff[i];
+    cc[i].d_c_ln = obuf + koff[i] + k2;
+    cc[i].d_f_lmn = store_flmn ? obuf + foff[i] : nullptr;
+  }
+  Options o;
+  o.precision = precision;
+  sfbhip_params p = make_params(1, 1, 1.0, verbose, o);
+  throw_status(sfbhip_direct_batched(c.h, cc.data(), (int32_t)cc.size(), &p, nullptr),
+               "sfb_decomposition_reverse_batched");
+  std::vector<double> host(nout);
+  if (nout) hip_check(hipMemcpy(host.data(), obuf, nout * sizeof(double), hipMemcpyDeviceToHost), "copy results");
+  std::vector<KClkk> res(cats.size());
+  for (size_t i = 0; i < cats.size(); ++i) {
+    const unsigned short L = cats[i].lmax, N = cats[i].nmax;
+    const size_t k2 = (size_t)L * N;
+    res[i].k_ln = Array2D(L, N);
+    res[i].c_ln = Array2D(L, N);
+    std::copy(host.begin() + koff[i], host.begin() + koff[i] + k2, res[i].k_ln.data.begin());
+    std::copy(host.begin() + koff[i] + k2, host.begin() + koff[i] + 2 * k2, res[i].c_ln.data.begin());
+    if (store_flmn) {
+      size_t o2 = foff[i];
+      for (unsigned l = 0; l < L; ++l) {
+        ComplexBlock b;
+        b.rows = l + 1;
+        b.cols = N;
+        b.data.resize((size_t)(l + 1) * N);
+        for (size_t e = 0; e < b.data.size(); ++e) b.data[e] = {host[o2 + 2 * e], host[o2 + 2 * e + 1]};
+        o2 += 2 * b.data.size();
+        res[i].f_lmn.push_back(std::move(b));
+      }
+    }
+  }
+  return res;
+}
+
 KClkk sfb_decomposition_healpix(const PixelizedPointContainer &pix, unsigned short lmax, double k_max, double rmax,
                                 unsigned short nmax, double window_volume, bool store_flmn, bool verbose) {
   Options o;

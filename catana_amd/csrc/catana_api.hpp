@@ -119,6 +119,21 @@ KClkk sfb_decomposition_healpix(const PointContainer &points, unsigned int nside
                                 double k_max, double rmax, unsigned short nmax, double window_volume,
                                 bool store_flmn, bool verbose);
 
+// One catalogue of a batched decomposition (sfb_decomposition_reverse's arguments).
+struct CatalogSpec {
+  const PointContainer *points = nullptr;
+  unsigned short lmax = 0;
+  double k_max = 0.0;
+  double rmax = 1.0;
+  unsigned short nmax = 1;
+  double window_volume = 0.0;
+};
+
+// sfb_decomposition_reverse of several catalogues in one batched device call
+// (one grouped launch per kernel); result i belongs to cats[i].
+std::vector<KClkk> sfb_decomposition_reverse_batched(const std::vector<CatalogSpec> &cats, bool store_flmn,
+                                                     bool verbose, int precision = 1);
+
 // Zeros-based k_ln table (return_types.cpp:10-19).
 Array2D compute_kln(unsigned short lmax, unsigned short nmax, double rmax, int zeros = 0);
 

@@ -179,6 +179,31 @@ PYBIND11_MODULE(_catana, m) {
       py::call_guard<py::gil_scoped_release>());
 
   m.def(
+      "sfb_decomposition_reverse_batched",
+      [](py::list catalogs, bool store, bool verbose, const std::string &precision) {
+        const int pr = precision_code(precision);
+        std::vector<CatalogSpec> specs;
+        for (py::handle item : catalogs) {
+          py::tuple t = py::reinterpret_borrow<py::tuple>(item);
+          if (t.size() < 5 || t.size() > 6)
+            throw std::invalid_argument(
+                "each catalogue is (point_container, lmax, k_max, rmax, nmax[, window_volume])");
+          CatalogSpec c;
+          c.points = t[0].cast<const PointContainer *>();
+          c.lmax = t[1].cast<unsigned short>();
+          c.k_max = t[2].cast<double>();
+          c.rmax = t[3].cast<double>();
+          c.nmax = t[4].cast<unsigned short>();
+          c.window_volume = t.size() > 5 ? t[5].cast<double>() : 0.0;
+          specs.push_back(c);
+        }
+        py::gil_scoped_release rel;
+        return sfb_decomposition_reverse_batched(specs, store, verbose, pr);
+      },
+      py::arg("catalogs"), py::arg("store_flmn") = false, py::arg("verbose") = true, py::arg("precision") = "mixed",
+      "sfb_decomposition_reverse of several catalogues, given as (point_container, lmax, k_max, rmax, nmax"
+      "[, window_volume]) tuples, in one batched device call; returns a list of KClkk.");
+  m.def(
       "_sfb_raw",
       [](const PointContainer &pc, unsigned short lmax, unsigned short nmax, double rmax, bool store, bool verbose,
          bool parallel, bool interpolated) {

@@ -170,3 +170,24 @@ def test_error_types():
     assert np.all(np.isfinite(r.c_ln))
     with pytest.raises(ValueError, match="power of 2"):
         catana.PixelizedPointContainer(12, pc)
+
+
+def test_batched_api_matches_single_calls():
+    specs = [(workloads.uniform_ball(5000, 400.0, seed=61), 20, 0.1, 400.0, 16, 0.0),
+             (workloads.clustered_ball(300, 900.0, mean_children=30, sigma=5.0, seed=62), 36, 0.15, 900.0, 50, 4e9),
+             (workloads.uniform_ball(700, 100.0, seed=63), 8, 0.0, 100.0, 5, 0.0)]
+    cats = [(catana.PointContainer(p), L, k, r, n, v) for p, L, k, r, n, v in specs]
+    print("case: batched Python API, 3 catalogues", flush=True)
+    for prec, tol in (("fp64", 1e-10), ("mixed", 1e-6)):
+        res = catana.sfb_decomposition_reverse_batched(cats, store_flmn=True, verbose=False, precision=prec)
+        assert len(res) == 3
+        for (pc, L, k, r, n, v), b in zip(cats, res):
+            s = catana.sfb_decomposition_reverse(pc, L, k, r, n, v, True, False, prec)
+            np.testing.assert_array_equal(b.k_ln, s.k_ln)
+            nz = s.c_ln != 0
+            assert np.all((b.c_ln != 0) == nz)
+            assert np.max(np.abs(b.c_ln[nz] / s.c_ln[nz] - 1)) < tol
+            assert len(b.f_lmn) == L and b.f_lmn[-1].shape == (L, n)
+            assert _flmn_err(b.f_lmn, s.f_lmn) < tol
+    with pytest.raises(ValueError):
+        catana.sfb_decomposition_reverse_batched([(cats[0][0], 4)], verbose=False)
