@@ -105,7 +105,7 @@ def main():
                     help="matrix-core operand format of the mixed path")
     ap.add_argument("--chunks", type=int, default=4, choices=[1, 2, 4, 8],
                     help="mixed path: m chunks accumulated per work item")
-    ap.add_argument("--fp64-below", type=int, default=8, help="mixed path: l below this run in FP64")
+    ap.add_argument("--fp64-below", type=int, default=2, help="mixed path: l below this run in FP64")
     ap.add_argument("--cpu-sample", type=int, default=65536)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true")

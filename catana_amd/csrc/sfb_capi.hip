@@ -638,6 +638,17 @@ sfbhip_status launch_plan_mixed(const DirectPlan &pl, const std::vector<int32_t>
   return SFBHIP_OK;
 }
 
+// The low-l FP64 items (all QMAX class 1), j_l from the quintic table.
+sfbhip_status launch_lowl(const DirectPlan &pl, DirectItem *ditems, size_t *off, hipStream_t st,
+                          const PointsSoA &pts, const double *dk, const BesselTableRef &tab, double *partial) {
+  const int n = (int)pl.items[0].size();
+  if (!n) return SFBHIP_OK;
+  sfb_lowl_f64_kernel<<<n, 256, 0, st>>>(pts, ditems + *off, dk, tab, partial);
+  HIPCHK(hipGetLastError());
+  *off += n;
+  return SFBHIP_OK;
+}
+
 void clear_timings(sfbhip_handle *h) {
   h->last_ms = 0.f;
   for (float &t : h->phase_ms) t = 0.f;
@@ -668,7 +679,7 @@ sfbhip_status sfbhip_default_params(sfbhip_params *p) {
   p->zeros = SFBHIP_ZEROS_REFERENCE;
   p->alias_compat = 1;
   p->verbose = 0;
-  p->mixed_fp64_below = 8;
+  p->mixed_fp64_below = 2;
   p->mixed_operands = SFBHIP_OPERANDS_SPLIT_FP16;
   p->mixed_chunks = 4;
   return SFBHIP_OK;
@@ -790,7 +801,8 @@ sfbhip_status sfbhip_direct(sfbhip_handle *h, const double *d_points, int64_t n_
   cx.norm = norm;
   cx.kln = h->kc.kln.data();
   cx.R = std::max(p->rmax, rdata);
-  if (llow > 0) plan_direct(h->kc.nl, 0, llow, n_points, LW_ROWS, LW_STEP, F64_MC, 1, 4096, cx, &pllw);
+  // the low-l items: at most 256 point ranges per group, so the reduction chains stay short
+  if (llow > 0) plan_direct(h->kc.nl, 0, llow, n_points, LW_ROWS, LW_TILE, F64_MC, 1, 256 * (int64_t)llow, cx, &pllw);
   pl64.partial_size = pllw.partial_size;
   plan_direct(h->kc.nl, llow, lsplit, n_points, F64_NT, F64_TP, F64_MC, 16, 4096, cx, &pl64);
   plmx.partial_size = pl64.partial_size;
@@ -816,12 +828,7 @@ sfbhip_status sfbhip_direct(sfbhip_handle *h, const double *d_points, int64_t n_
   clear_timings(h);
   HIPCHK(hipEventRecord(h->ev[0], st));
   size_t off = 0;
-  if (!pllw.items[0].empty()) {
-    const int n = (int)pllw.items[0].size();
-    sfb_lowl_f64_kernel<<<n, 256, 0, st>>>(pts, ditems + off, dk, p->nmax, tab, partial);
-    HIPCHK(hipGetLastError());
-    off += n;
-  }
+  CHK(launch_lowl(pllw, ditems, &off, st, pts, dk, tab, partial));
   CHK(launch_plan_f64(pl64, ditems, &off, st, pts, nullptr, dk, p->nmax, lt, use_table, tab, partial, derr));
   HIPCHK(hipEventRecord(h->ev[1], st));
   if (mixed)
@@ -1065,7 +1072,7 @@ sfbhip_status sfbhip_direct_batched(sfbhip_handle *h, const sfbhip_catalog *cats
   auto target = [&](int c) { return std::max<int64_t>(16, (int64_t)(ttot * work[c] / std::max(wtot, 1.0))); };
   for (int c = 0; c < ncat; ++c)
     if (llow > 0)
-      plan_direct(nl[c], 0, std::min(llow, cats[c].lmax), cats[c].n_points, LW_ROWS, LW_STEP, F64_MC, 1, target(c),
+      plan_direct(nl[c], 0, std::min(llow, cats[c].lmax), cats[c].n_points, LW_ROWS, LW_TILE, F64_MC, 1, target(c),
                   ctx(c), &pllw);
   pl64.partial_size = pllw.partial_size;
   for (int c = 0; c < ncat; ++c)
@@ -1094,12 +1101,7 @@ sfbhip_status sfbhip_direct_batched(sfbhip_handle *h, const sfbhip_catalog *cats
   clear_timings(h);
   HIPCHK(hipEventRecord(h->ev[0], st));
   size_t off = 0;
-  if (!pllw.items[0].empty()) {
-    const int n = (int)pllw.items[0].size();
-    sfb_lowl_f64_kernel<<<n, 256, 0, st>>>(pts, ditems + off, dk, 0, tab, partial);
-    HIPCHK(hipGetLastError());
-    off += n;
-  }
+  CHK(launch_lowl(pllw, ditems, &off, st, pts, dk, tab, partial));
   CHK(launch_plan_f64(pl64, ditems, &off, st, pts, nullptr, dk, 0, lt, mixed ? JL_QUINTIC : JL_EXACT, tab, partial,
                       derr));
   HIPCHK(hipEventRecord(h->ev[1], st));

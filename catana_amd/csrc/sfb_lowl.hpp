@@ -5,8 +5,12 @@
 // c = lane%16) evaluates Y*_l,c/2 of point k (column c, real or imaginary part)
 // -- the B fragment of v_mfma_f64_16x16x4f64 -- and, for each 16-row tile of
 // n, j_l(k_ln r) of row lane%16 and point k -- the A fragment -- from the
-// quintic FP64 table.  Nothing goes through LDS until the four waves' 64 x 16
-// blocks are summed, in a fixed order, at the end of the item.
+// quintic FP64 table (error ~1e-10 of the amplitude).  The point data of each
+// 256-point tile is staged in LDS; the four waves' 64 x 16 blocks are summed,
+// in a fixed order, at the end of the item.  (The FP32 cubic Hermite table of
+// the mixed kernel is not accurate enough here: l = 0 and 1 have C_l with one
+// or three real degrees of freedom, so some c_ln are far below their noise
+// level and their relative error is magnified.)
 //
 // Y*_lm = Pbar_l^m(cos t) (cos m phi - i sin m phi) with the orthonormal
 // Pbar (Condon-Shortley phase, gsl_sf_legendre_sphPlm as at sfb_raw.cpp:73):
@@ -22,14 +26,19 @@ namespace sfb {
 
 constexpr int LW_LMAX = 8;   // multipoles handled: l < LW_LMAX
 constexpr int LW_ROWS = 64;  // rows of n per item (four 16-row MFMA tiles)
-constexpr int LW_STEP = 16;  // points per block iteration (4 waves x 4 points)
+constexpr int LW_STEP = 16;  // points per block step (4 waves x 4 points)
+constexpr int LW_TILE = 256; // points staged in LDS per block iteration
 
 __global__ __launch_bounds__(256) void sfb_lowl_f64_kernel(PointsSoA pts, const DirectItem *__restrict__ items,
-                                                           const double *__restrict__ kln, int nmax,
-                                                           BesselTableRef tab, double *__restrict__ partial) {
+                                                           const double *__restrict__ kln, BesselTableRef tab,
+                                                           double *__restrict__ partial) {
   __shared__ double2 acoef[LW_LMAX * LW_LMAX];  // [m][L]: (a_L, 1 / a_{L-1})
   __shared__ double sect[LW_LMAX];
-  __shared__ double red[3][LW_ROWS * 16];
+  // one LDS area: the tile's point data during the sweep, the waves' blocks at the end
+  __shared__ double lds[3 * LW_ROWS * 16];
+  double(*pt)[LW_TILE] = reinterpret_cast<double(*)[LW_TILE]>(lds);  // r (or -1), cos t, sin t, cos phi, sin phi
+  double(*red)[LW_ROWS * 16] = reinterpret_cast<double(*)[LW_ROWS * 16]>(lds);
+  static_assert(5 * LW_TILE <= 3 * LW_ROWS * 16, "point tile fits the reduction area");
 
   const DirectItem it = items[blockIdx.x];
   const int l = it.l;
@@ -68,47 +77,57 @@ __global__ __launch_bounds__(256) void sfb_lowl_f64_kernel(PointsSoA pts, const 
 #pragma unroll
   for (int t = 0; t < 4; ++t) acc[t] = v4d{0.0, 0.0, 0.0, 0.0};
 
-  for (int64_t base = it.p0 + 4 * wave; base < it.p1; base += LW_STEP) {
-    const int64_t p = base + kq;
-    const bool ok = p < it.p1;
-    double r = 0.0, x = 0.0, s = 0.0, cp = 1.0, sp = 0.0;
-    if (ok) {
-      r = pts.r[p];
-      x = pts.x[p];
-      s = pts.s[p];
-      cp = pts.cphi[p];
-      sp = pts.sphi[p];
+  // 256-point tiles: the block stages the tile's point data in LDS with one
+  // coalesced load per thread, then wave w takes points 64w .. 64w+63, four per step
+  for (int64_t tb0 = it.p0; tb0 < it.p1; tb0 += LW_TILE) {
+    {
+      const int64_t gp = tb0 + tid;
+      const bool ok = gp < it.p1;
+      pt[0][tid] = ok ? pts.r[gp] : -1.0;
+      pt[1][tid] = ok ? pts.x[gp] : 0.0;
+      pt[2][tid] = ok ? pts.s[gp] : 0.0;
+      pt[3][tid] = ok ? pts.cphi[gp] : 1.0;
+      pt[4][tid] = ok ? pts.sphi[gp] : 0.0;
     }
-    // B fragment: Y* of point kq, column col
-    double P = s0, c = 1.0, sn = 0.0;
-    for (int k = 0; k < m; ++k) {
-      P *= s;
-      const double c2 = c * cp - sn * sp;
-      sn = fma(sn, cp, c * sp);
-      c = c2;
-    }
-    double prev = 0.0;
-    for (int L = m + 1; L <= l; ++L) {
-      const double2 a = acoef[m * LW_LMAX + L];
-      const double nx = a.x * fma(x, P, -a.y * prev);
-      prev = P;
-      P = nx;
-    }
-    double y = (col & 1) ? -P * sn : P * c;
-    y = (ok && live_col) ? y : 0.0;
-    // A fragments: j_l(k r) of rows 16t + lane%16, point kq
+    __syncthreads();
+#pragma unroll 2
+    for (int st = 0; st < LW_TILE / 16; ++st) {
+      const int pl = 64 * wave + 4 * st + kq;
+      const double r0 = pt[0][pl];
+      const bool ok = r0 >= 0.0;
+      const double r = ok ? r0 : 0.0, x = pt[1][pl], s = pt[2][pl], cp = pt[3][pl], sp = pt[4][pl];
+      // B fragment: Y* of point kq, column col
+      double P = s0, c = 1.0, sn = 0.0;
+      for (int k = 0; k < m; ++k) {
+        P *= s;
+        const double c2 = c * cp - sn * sp;
+        sn = fma(sn, cp, c * sp);
+        c = c2;
+      }
+      double prev = 0.0;
+      for (int L = m + 1; L <= l; ++L) {
+        const double2 a = acoef[m * LW_LMAX + L];
+        const double nx = a.x * fma(x, P, -a.y * prev);
+        prev = P;
+        P = nx;
+      }
+      double y = (col & 1) ? -P * sn : P * c;
+      y = (ok && live_col) ? y : 0.0;
+      // A fragments: j_l(k r) of rows 16t + lane%16, point kq
 #pragma unroll
-    for (int t = 0; t < 4; ++t) {
-      if (t < ntile) {
-        const double u = kr[t] * r * Q5_HINV;
-        int i = (int)u;
-        i = i < last ? i : last;
-        const double w = u - (double)i;
-        const double2 a0 = tb[3 * i], a1 = tb[3 * i + 1], a2 = tb[3 * i + 2];
-        const double jv = fma(w, fma(w, fma(w, fma(w, fma(w, a2.y, a2.x), a1.y), a1.x), a0.y), a0.x);
-        acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(jv, y, acc[t], 0, 0, 0);
+      for (int t = 0; t < 4; ++t) {
+        if (t < ntile) {
+          const double u = kr[t] * r * Q5_HINV;
+          int i = (int)u;
+          i = i < last ? i : last;
+          const double w = u - (double)i;
+          const double2 a0 = tb[3 * i], a1 = tb[3 * i + 1], a2 = tb[3 * i + 2];
+          const double jv = fma(w, fma(w, fma(w, fma(w, fma(w, a2.y, a2.x), a1.y), a1.x), a0.y), a0.x);
+          acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(jv, y, acc[t], 0, 0, 0);
+        }
       }
     }
+    __syncthreads();  // the next tile overwrites pt
   }
 
   // fixed-order sum of the four waves' blocks; v_mfma_f64_16x16x4f64 result:

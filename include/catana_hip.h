@@ -73,8 +73,8 @@ typedef struct sfbhip_params {
   int32_t zeros;         /* SFBHIP_ZEROS_* */
   int32_t alias_compat;  /* HEALPix path: 1 keeps the reference's unconjugated fold (sfb_reverse_fft.cpp:202-206) */
   int32_t verbose;       /* print a one-line summary per call to stdout */
-  int32_t mixed_fp64_below; /* mixed path: multipoles l < this run in FP64 (C_l of low l can be
-                               dominated by cancellation); default 8 */
+  int32_t mixed_fp64_below; /* mixed path: multipoles l < this run in FP64; default 2 (at
+                               2^25 points l = 1 reached 1.2e-6 in the mixed kernels) */
   int32_t mixed_operands;   /* mixed path: SFBHIP_OPERANDS_* */
   int32_t mixed_chunks;     /* mixed path: largest number of 16-wide m chunks one work item
                                accumulates (1, 2, 4 or 8; 0 = 4) -- more chunks reuse each Bessel
@@ -84,7 +84,7 @@ typedef struct sfbhip_params {
 typedef struct sfbhip_handle sfbhip_handle;
 
 /* Fills defaults: precision FP64, bessel AUTO, zeros REFERENCE, alias_compat 1,
- * mixed_fp64_below 8, mixed_operands SPLIT_FP16, mixed_chunks 4. */
+ * mixed_fp64_below 2, mixed_operands SPLIT_FP16, mixed_chunks 4. */
 sfbhip_status sfbhip_default_params(sfbhip_params *p);
 
 /* Library state bound to one device: workspace and cached k_ln tables.

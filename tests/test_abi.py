@@ -39,7 +39,7 @@ def test_default_params():
     assert p.bessel == capi.BESSEL_AUTO
     assert p.zeros == capi.ZEROS_REFERENCE
     assert p.alias_compat == 1
-    assert p.mixed_fp64_below == 8
+    assert p.mixed_fp64_below == 2
     assert p.mixed_operands == capi.OPERANDS_SPLIT_FP16
     assert p.mixed_chunks == 4
     assert ctypes.sizeof(capi.Params) == 4 * 2 + 8 * 3 + 4 * 8
