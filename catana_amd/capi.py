@@ -20,8 +20,10 @@ OPERANDS_FP32, OPERANDS_SPLIT_FP16, OPERANDS_SPLIT3_FP16 = 0, 1, 2
 EXPORTED = [
     "sfbhip_default_params", "sfbhip_create", "sfbhip_destroy", "sfbhip_flmn_count", "sfbhip_kln",
     "sfbhip_direct", "sfbhip_direct_batched", "sfbhip_pixelize", "sfbhip_ang2pix_ring", "sfbhip_healpix", "sfbhip_last_kernel_ms",
-    "sfbhip_last_phase_ms", "sfbhip_last_error",
+    "sfbhip_last_phase_ms", "sfbhip_last_error", "sfbhip_bessel_zeros", "sfbhip_sbessel_nodes",
+    "sfbhip_sbessel_integrals",
 ]
+WEIGHT_NODES, WEIGHT_TABLE = 0, 1
 
 
 class Params(ctypes.Structure):
@@ -40,6 +42,13 @@ class Catalog(ctypes.Structure):
         ("nmax", ctypes.c_int32), ("rmax", ctypes.c_double), ("k_max", ctypes.c_double),
         ("window_volume", ctypes.c_double), ("d_k_ln", ctypes.c_void_p), ("d_c_ln", ctypes.c_void_p),
         ("d_f_lmn", ctypes.c_void_p),
+    ]
+
+
+class Weight(ctypes.Structure):
+    _fields_ = [
+        ("kind", ctypes.c_int32), ("r_power", ctypes.c_int32), ("n", ctypes.c_int64), ("values", ctypes.c_void_p),
+        ("x_min", ctypes.c_double), ("x_max", ctypes.c_double),
     ]
 
 
@@ -72,6 +81,9 @@ def lib():
         L.sfbhip_healpix.argtypes = [P, P, P, i32, ctypes.POINTER(Params), P, P, P, P]
         L.sfbhip_last_kernel_ms.argtypes = [P, ctypes.POINTER(ctypes.c_float)]
         L.sfbhip_last_phase_ms.argtypes = [P, ctypes.POINTER(ctypes.c_float)]
+        L.sfbhip_bessel_zeros.argtypes = [i32, i32, i32, P]
+        L.sfbhip_sbessel_nodes.argtypes = [P, i64, P, P, P, ctypes.c_double, ctypes.POINTER(i64), P]
+        L.sfbhip_sbessel_integrals.argtypes = [P, i64, P, P, P, ctypes.c_double, ctypes.POINTER(Weight), P]
         L.sfbhip_last_error.restype = ctypes.c_char_p
         L.sfbhip_last_error.argtypes = []
         _lib = L
@@ -250,3 +262,53 @@ def sfb_healpix_torch(handle, offsets, radii, nside, lmax, nmax, rmax, store_flm
                               ctypes.c_void_p(stream) if stream else None)
     check(rc, "sfbhip_healpix")
     return k, c, f
+
+
+def bessel_zeros(l, n, mode=ZEROS_REFERENCE):
+    """First n zeros of j_l (host), as SphericalBesselZeros(l)[0..n-1]."""
+    z = np.zeros(int(n))
+    check(lib().sfbhip_bessel_zeros(int(l), int(n), int(mode), z.ctypes.data), "sfbhip_bessel_zeros")
+    return z
+
+
+def _triples(l, k1, k2):
+    l, k1, k2 = np.broadcast_arrays(np.asarray(l, dtype=np.int32), np.asarray(k1, dtype=np.float64),
+                                    np.asarray(k2, dtype=np.float64))
+    shape = l.shape
+    return (shape, np.ascontiguousarray(l.ravel()), np.ascontiguousarray(k1.ravel()),
+            np.ascontiguousarray(k2.ravel()))
+
+
+def sbessel_nodes(handle, l, k1, k2, rmax):
+    """Quadrature nodes of sbessel_integrals for the same triples (host array)."""
+    _, l, k1, k2 = _triples(l, k1, k2)
+    cnt = ctypes.c_int64()
+    L = lib()
+    check(L.sfbhip_sbessel_nodes(handle.ptr, l.size, l.ctypes.data, k1.ctypes.data, k2.ctypes.data, float(rmax),
+                                 ctypes.byref(cnt), None), "sfbhip_sbessel_nodes")
+    r = np.zeros(cnt.value)
+    check(L.sfbhip_sbessel_nodes(handle.ptr, l.size, l.ctypes.data, k1.ctypes.data, k2.ctypes.data, float(rmax),
+                                 ctypes.byref(cnt), r.ctypes.data), "sfbhip_sbessel_nodes")
+    return r
+
+
+def sbessel_integrals(handle, l, k1, k2, rmax, node_values=None, table=None, r_power=0):
+    """int_0^rmax w(r) j_l(k1 r) j_l(k2 r) dr over broadcast (l, k1, k2).
+
+    w is given either as its values at sbessel_nodes(...) (node_values) or as
+    a table (samples, x_min, x_max) interpolated linearly on the device; the
+    integrand carries w(r) r^r_power."""
+    shape, l, k1, k2 = _triples(l, k1, k2)
+    if (node_values is None) == (table is None):
+        raise ValueError("give exactly one of node_values and table")
+    if table is not None:
+        v, x0, x1 = table
+        v = np.ascontiguousarray(v, dtype=np.float64)
+        w = Weight(WEIGHT_TABLE, int(r_power), v.size, v.ctypes.data, float(x0), float(x1))
+    else:
+        v = np.ascontiguousarray(node_values, dtype=np.float64)
+        w = Weight(WEIGHT_NODES, int(r_power), v.size, v.ctypes.data, 0.0, 0.0)
+    out = np.zeros(l.size)
+    check(lib().sfbhip_sbessel_integrals(handle.ptr, l.size, l.ctypes.data, k1.ctypes.data, k2.ctypes.data,
+                                         float(rmax), ctypes.byref(w), out.ctypes.data), "sfbhip_sbessel_integrals")
+    return out.reshape(shape)

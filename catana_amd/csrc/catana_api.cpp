@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <memory>
@@ -188,6 +189,205 @@ Array2D compute_kln(unsigned short lmax, unsigned short nmax, double rmax, int z
   std::vector<int32_t> nl(std::max<int>(lmax, 1));
   throw_status(sfbhip_kln(c.h, &p, k.data.data(), nl.data()), "compute_kln");
   return k;
+}
+
+// ---- FunctionInterpolator (FunctionInterpolator.cpp:7-45) ----
+
+FunctionInterpolator::FunctionInterpolator(std::function<double(double)> fct, unsigned int interpolation_points,
+                                           double x_min, double x_max, bool /*parallel*/)
+    : x_min_(x_min), x_max_(x_max) {
+  if (interpolation_points < 2 || !(x_max > x_min))
+    throw std::invalid_argument("FunctionInterpolator: needs >= 2 points and x_min < x_max");
+  const double dx = (x_max - x_min) / (interpolation_points - 1);
+  dx_inv_ = 1.0 / dx;
+  y_.resize(interpolation_points);
+  for (unsigned int i = 0; i < interpolation_points; ++i) y_[i] = fct(x_min + i * dx);
+}
+
+unsigned int FunctionInterpolator::get_index(double x) const {
+  if (x < x_min_ || x >= x_max_)
+    throw std::domain_error("argument outside range for which this FunctionInterpolator is defined");
+  return static_cast<unsigned int>((x - x_min_) * dx_inv_);
+}
+
+double FunctionInterpolator::operator()(double x) const {
+  if (x < x_min_ || x >= x_max_)
+    throw std::domain_error("argument outside range for which this FunctionInterpolator is defined");
+  const double u = (x - x_min_) * dx_inv_;
+  unsigned int i = static_cast<unsigned int>(u);
+  if (i > y_.size() - 2) i = (unsigned int)y_.size() - 2;
+  return y_[i] + (y_[i + 1] - y_[i]) * (u - i);
+}
+
+namespace besseltools {
+
+// ---- SphericalBesselZeros (SphericalBesselZeros.cpp:11-101), zeros from the C-ABI ----
+
+SphericalBesselZeros::SphericalBesselZeros(unsigned int l) : l_(l) { grow(2); }
+
+void SphericalBesselZeros::grow(size_t n) {
+  if (zeros_.size() >= n) return;
+  // the zero sequence is deterministic, so a longer prefix extends the shorter
+  size_t m = std::max<size_t>(n, 2 * zeros_.size());
+  std::vector<double> z(m);
+  throw_status(sfbhip_bessel_zeros((int32_t)l_, (int32_t)m, SFBHIP_ZEROS_REFERENCE, z.data()), "SphericalBesselZeros");
+  zeros_.swap(z);
+}
+
+double SphericalBesselZeros::operator[](unsigned int n) {
+  grow((size_t)n + 1);
+  return zeros_[n];
+}
+
+void SphericalBesselZeros::compute_up_to(double z_max) {
+  while (zeros_.back() < z_max) grow(zeros_.size() + 1);
+}
+
+unsigned int SphericalBesselZeros::next_zero(double z) {
+  compute_up_to(z);
+  return (unsigned int)(std::upper_bound(zeros_.begin(), zeros_.end(), z) - zeros_.begin());
+}
+
+namespace {
+
+void check_triples(const std::vector<int32_t> &l, const std::vector<double> &k1, const std::vector<double> &k2) {
+  if (l.size() != k1.size() || l.size() != k2.size())
+    throw std::invalid_argument("double_sbessel_integrals: l, k1, k2 must have the same length");
+}
+
+std::vector<double> run_integrals(const std::vector<int32_t> &l, const std::vector<double> &k1,
+                                  const std::vector<double> &k2, double Rmax, const sfbhip_weight &w) {
+  check_triples(l, k1, k2);
+  std::vector<double> out(l.size());
+  if (l.empty()) return out;
+  Context &c = ctx();
+  std::lock_guard<std::mutex> lk(c.mu);
+  throw_status(sfbhip_sbessel_integrals(c.h, (int64_t)l.size(), l.data(), k1.data(), k2.data(), Rmax, &w, out.data()),
+               "double_sbessel_integrator");
+  return out;
+}
+
+std::vector<double> integrals_callable(std::function<double(double)> f, int r_power, const std::vector<int32_t> &l,
+                                       const std::vector<double> &k1, const std::vector<double> &k2, double Rmax) {
+  std::vector<double> r = sbessel_nodes(l, k1, k2, Rmax);
+  for (double &x : r) x = f(x);
+  sfbhip_weight w{SFBHIP_WEIGHT_NODES, r_power, (int64_t)r.size(), r.data(), 0.0, 0.0};
+  return run_integrals(l, k1, k2, Rmax, w);
+}
+
+sfbhip_weight table_weight(const FunctionInterpolator &fi, int r_power) {
+  return sfbhip_weight{SFBHIP_WEIGHT_TABLE, r_power, (int64_t)fi.samples().size(), fi.samples().data(), fi.x_min(),
+                       fi.x_max()};
+}
+
+}  // namespace
+
+std::vector<double> sbessel_nodes(const std::vector<int32_t> &l, const std::vector<double> &k1,
+                                  const std::vector<double> &k2, double Rmax) {
+  check_triples(l, k1, k2);
+  if (l.empty()) return {};
+  Context &c = ctx();
+  std::lock_guard<std::mutex> lk(c.mu);
+  int64_t n = 0;
+  throw_status(sfbhip_sbessel_nodes(c.h, (int64_t)l.size(), l.data(), k1.data(), k2.data(), Rmax, &n, nullptr),
+               "sbessel_nodes");
+  std::vector<double> r((size_t)n);
+  throw_status(sfbhip_sbessel_nodes(c.h, (int64_t)l.size(), l.data(), k1.data(), k2.data(), Rmax, &n, r.data()),
+               "sbessel_nodes");
+  return r;
+}
+
+std::vector<double> sbessel_integrals_at_nodes(const std::vector<int32_t> &l, const std::vector<double> &k1,
+                                               const std::vector<double> &k2, double Rmax,
+                                               const std::vector<double> &f_at_nodes) {
+  sfbhip_weight w{SFBHIP_WEIGHT_NODES, 0, (int64_t)f_at_nodes.size(), f_at_nodes.data(), 0.0, 0.0};
+  return run_integrals(l, k1, k2, Rmax, w);
+}
+
+std::vector<double> double_sbessel_integrals(std::function<double(double)> f, const std::vector<int32_t> &l,
+                                             const std::vector<double> &k1, const std::vector<double> &k2,
+                                             double Rmax) {
+  return integrals_callable(std::move(f), 0, l, k1, k2, Rmax);
+}
+
+std::vector<double> double_sbessel_integrals(const FunctionInterpolator &fi, const std::vector<int32_t> &l,
+                                             const std::vector<double> &k1, const std::vector<double> &k2,
+                                             double Rmax) {
+  return run_integrals(l, k1, k2, Rmax, table_weight(fi, 0));
+}
+
+double double_sbessel_integrator(std::function<double(double)> f, const unsigned int &l, const double &Rmax, double k1,
+                                 double k2) {
+  return double_sbessel_integrals(std::move(f), {(int32_t)l}, {k1}, {k2}, Rmax)[0];
+}
+
+double double_sbessel_integrator_bz(std::function<double(double)> f, const unsigned int &l, SphericalBesselZeros &bz,
+                                    const double &Rmax, double k1, double k2) {
+  if (bz.l() != l) throw std::invalid_argument("double_sbessel_integrator_bz: zeros of another multipole");
+  return double_sbessel_integrator(std::move(f), l, Rmax, k1, k2);
+}
+
+double double_sbessel_integrator(const FunctionInterpolator &fi, const unsigned int &l, const double &Rmax, double k1,
+                                 double k2) {
+  return double_sbessel_integrals(fi, {(int32_t)l}, {k1}, {k2}, Rmax)[0];
+}
+
+}  // namespace besseltools
+
+void kept_modes(unsigned short lmax, double k_max, double rmax, unsigned short nmax, std::vector<int32_t> *l,
+                std::vector<int32_t> *n, std::vector<double> *k) {
+  Context &c = ctx();
+  std::lock_guard<std::mutex> lk(c.mu);
+  Options o;
+  o.k_max = k_max;
+  sfbhip_params p = make_params(lmax, nmax, rmax, false, o);
+  Array2D kln(lmax, nmax);
+  std::vector<int32_t> nl(std::max<int>(lmax, 1));
+  throw_status(sfbhip_kln(c.h, &p, kln.data.data(), nl.data()), "kept_modes");
+  for (int ll = 0; ll < lmax; ++ll)
+    for (int nn = 0; nn < nl[ll]; ++nn) {
+      l->push_back(ll);
+      n->push_back(nn);
+      k->push_back(kln(ll, nn));
+    }
+}
+
+namespace {
+
+Array2D shot_noise_finish(const std::vector<double> &I, const std::vector<int32_t> &l, const std::vector<int32_t> &n,
+                          const std::vector<double> &k, unsigned short lmax, unsigned short nmax, double rmax,
+                          double n_points, double window_volume) {
+  const double V = window_volume > 0 ? window_volume : 4.0 / 3.0 * M_PI * rmax * rmax * rmax;
+  Array2D out(lmax, nmax);
+  for (size_t i = 0; i < I.size(); ++i) out(l[i], n[i]) = V / n_points * (2.0 / M_PI) * k[i] * k[i] * I[i];
+  return out;
+}
+
+}  // namespace
+
+Array2D shot_noise(std::function<double(double)> w, unsigned short lmax, double k_max, double rmax,
+                   unsigned short nmax, double n_points, double window_volume) {
+  if (!(n_points > 0)) throw std::invalid_argument("shot_noise: n_points must be positive");
+  std::vector<int32_t> l, n;
+  std::vector<double> k;
+  kept_modes(lmax, k_max, rmax, nmax, &l, &n, &k);
+  if (l.empty()) return Array2D(lmax, nmax);
+  std::vector<double> r = besseltools::sbessel_nodes(l, k, k, rmax);
+  for (double &x : r) x = w(x);
+  sfbhip_weight wt{SFBHIP_WEIGHT_NODES, 2, (int64_t)r.size(), r.data(), 0.0, 0.0};
+  std::vector<double> I = besseltools::run_integrals(l, k, k, rmax, wt);
+  return shot_noise_finish(I, l, n, k, lmax, nmax, rmax, n_points, window_volume);
+}
+
+Array2D shot_noise(const FunctionInterpolator &w, unsigned short lmax, double k_max, double rmax,
+                   unsigned short nmax, double n_points, double window_volume) {
+  if (!(n_points > 0)) throw std::invalid_argument("shot_noise: n_points must be positive");
+  std::vector<int32_t> l, n;
+  std::vector<double> k;
+  kept_modes(lmax, k_max, rmax, nmax, &l, &n, &k);
+  if (l.empty()) return Array2D(lmax, nmax);
+  std::vector<double> I = besseltools::run_integrals(l, k, k, rmax, besseltools::table_weight(w, 2));
+  return shot_noise_finish(I, l, n, k, lmax, nmax, rmax, n_points, window_volume);
 }
 
 KClkk::KClkk(unsigned short lmax, unsigned short nmax, double rmax)

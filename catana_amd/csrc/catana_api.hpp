@@ -9,6 +9,7 @@
 #include <complex>
 #include <cstddef>
 #include <cstdint>
+#include <functional>
 #include <string>
 #include <vector>
 
@@ -136,5 +137,81 @@ std::vector<KClkk> sfb_decomposition_reverse_batched(const std::vector<CatalogSp
 
 // Zeros-based k_ln table (return_types.cpp:10-19).
 Array2D compute_kln(unsigned short lmax, unsigned short nmax, double rmax, int zeros = 0);
+
+// A function sampled at interpolation_points equidistant points on
+// [x_min, x_max] and interpolated linearly; arguments outside [x_min, x_max)
+// throw std::domain_error (FunctionInterpolator.cpp:7-45).
+class FunctionInterpolator {
+ public:
+  FunctionInterpolator(std::function<double(double)> fct, unsigned int interpolation_points, double x_min,
+                       double x_max, bool parallel = false);
+  double operator()(double x) const;
+  double operator[](unsigned int index) const { return y_[index]; }
+  unsigned int get_index(double x) const;
+  const std::vector<double> &samples() const { return y_; }
+  double x_min() const { return x_min_; }
+  double x_max() const { return x_max_; }
+
+ private:
+  std::vector<double> y_;
+  double x_min_, x_max_, dx_inv_;
+};
+
+namespace besseltools {
+
+// Zeros of j_l, computed on demand and kept (SphericalBesselZeros.cpp:11-101).
+class SphericalBesselZeros {
+ public:
+  explicit SphericalBesselZeros(unsigned int l);
+  double operator[](unsigned int n);
+  void compute_up_to(double z_max);     // until the last zero is >= z_max
+  unsigned int next_zero(double z);     // index of the first zero > z
+  unsigned int l() const { return l_; }
+
+ private:
+  void grow(size_t n);
+  unsigned int l_;
+  std::vector<double> zeros_;
+};
+
+// int_0^Rmax f(r) j_l(k1 r) j_l(k2 r) dr (sbessel_integrator.cpp:45-110), on the device.
+double double_sbessel_integrator(std::function<double(double)> f, const unsigned int &l, const double &Rmax, double k1,
+                                 double k2);
+double double_sbessel_integrator_bz(std::function<double(double)> f, const unsigned int &l, SphericalBesselZeros &bz,
+                                    const double &Rmax, double k1, double k2);
+double double_sbessel_integrator(const FunctionInterpolator &fi, const unsigned int &l, const double &Rmax, double k1,
+                                 double k2);
+
+// Many triples (l[i], k1[i], k2[i]) in one device call.  f is evaluated on the
+// host at every quadrature node (sbessel_nodes); the interpolator overload
+// runs entirely on the device.
+std::vector<double> double_sbessel_integrals(std::function<double(double)> f, const std::vector<int32_t> &l,
+                                             const std::vector<double> &k1, const std::vector<double> &k2,
+                                             double Rmax);
+std::vector<double> double_sbessel_integrals(const FunctionInterpolator &fi, const std::vector<int32_t> &l,
+                                             const std::vector<double> &k1, const std::vector<double> &k2,
+                                             double Rmax);
+// The two halves of the callable path, for callers that evaluate f in bulk.
+std::vector<double> sbessel_nodes(const std::vector<int32_t> &l, const std::vector<double> &k1,
+                                  const std::vector<double> &k2, double Rmax);
+std::vector<double> sbessel_integrals_at_nodes(const std::vector<int32_t> &l, const std::vector<double> &k1,
+                                               const std::vector<double> &k2, double Rmax,
+                                               const std::vector<double> &f_at_nodes);
+
+}  // namespace besseltools
+
+// (l, n, k_ln) of every mode sfb_decomposition_reverse keeps (n < n_l with
+// z_ln < k_max rmax; k_max <= 0: all nmax), in (l, n) order.
+void kept_modes(unsigned short lmax, double k_max, double rmax, unsigned short nmax, std::vector<int32_t> *l,
+                std::vector<int32_t> *n, std::vector<double> *k);
+
+// Shot-noise term of c_ln (docs/catana_usage.rst:126-134):
+//   C_l^sn(k_ln) = (V/N) (2/pi) k_ln^2 int_0^rmax r^2 w(r) j_l(k_ln r)^2 dr,
+// V = window_volume or (4/3) pi rmax^3; n modes as sfb_decomposition_reverse
+// keeps them (k_max <= 0: all nmax), the rest 0.  (lmax x nmax), row = l.
+Array2D shot_noise(std::function<double(double)> w, unsigned short lmax, double k_max, double rmax,
+                   unsigned short nmax, double n_points, double window_volume = 0.0);
+Array2D shot_noise(const FunctionInterpolator &w, unsigned short lmax, double k_max, double rmax,
+                   unsigned short nmax, double n_points, double window_volume = 0.0);
 
 }  // namespace catana

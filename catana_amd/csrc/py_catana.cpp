@@ -8,6 +8,7 @@
 #include <pybind11/pybind11.h>
 #include <pybind11/stl.h>
 
+#include <cmath>
 #include <cstring>
 #include <stdexcept>
 
@@ -58,6 +59,52 @@ PointContainer container_from_array(py::array_t<double> array, const std::string
     pc[i] = spherical ? point_from_spherical_position(c[0], c[1], c[2]) : point_from_box_position(c[0], c[1], c[2]);
   }
   return pc;
+}
+
+// w at the quadrature nodes: one vectorised call when w accepts an array
+// (a scalar result is broadcast), else one call per node.
+std::vector<double> eval_on_nodes(const py::function &w, const std::vector<double> &r) {
+  if (r.empty()) return {};
+  try {
+    py::array_t<double> ra((py::ssize_t)r.size(), r.data());
+    py::object res = w(ra);
+    auto a = py::array_t<double, py::array::c_style | py::array::forcecast>::ensure(res);
+    if (a && a.size() == (py::ssize_t)r.size()) return std::vector<double>(a.data(), a.data() + a.size());
+    if (a && a.size() == 1) return std::vector<double>(r.size(), *a.data());
+  } catch (py::error_already_set &) {
+  }
+  std::vector<double> out(r.size());
+  for (size_t i = 0; i < r.size(); ++i) out[i] = w(r[i]).cast<double>();
+  return out;
+}
+
+// numpy broadcasting of (k1, k2) as py::vectorize does (py_besseltools.cpp:45-78);
+// a float for scalar inputs
+template <class Run>
+py::object vectorized_k(py::array_t<double> k1, py::array_t<double> k2, unsigned int l, Run run) {
+  py::module_ np = py::module_::import("numpy");
+  py::tuple b = np.attr("broadcast_arrays")(k1, k2);
+  auto a1 = py::array_t<double, py::array::c_style | py::array::forcecast>::ensure(b[0]);
+  auto a2 = py::array_t<double, py::array::c_style | py::array::forcecast>::ensure(b[1]);
+  std::vector<double> v1(a1.data(), a1.data() + a1.size()), v2(a2.data(), a2.data() + a2.size());
+  std::vector<int32_t> ls(v1.size(), (int32_t)l);
+  std::vector<double> res = run(ls, v1, v2);
+  if (a1.ndim() == 0) return py::float_(res.empty() ? 0.0 : res[0]);
+  py::array_t<double> out(std::vector<py::ssize_t>(a1.shape(), a1.shape() + a1.ndim()));
+  if (!res.empty()) std::memcpy(out.mutable_data(), res.data(), res.size() * sizeof(double));
+  return std::move(out);
+}
+
+std::vector<double> integrals_py(const py::function &w, const std::vector<int32_t> &l, const std::vector<double> &k1,
+                                 const std::vector<double> &k2, double rmax) {
+  std::vector<double> r;
+  {
+    py::gil_scoped_release nogil;
+    r = besseltools::sbessel_nodes(l, k1, k2, rmax);
+  }
+  std::vector<double> wv = eval_on_nodes(w, r);
+  py::gil_scoped_release nogil;
+  return besseltools::sbessel_integrals_at_nodes(l, k1, k2, rmax, wv);
 }
 
 int precision_code(const std::string &s) {
@@ -227,6 +274,111 @@ PYBIND11_MODULE(_catana, m) {
       },
       py::arg("pixelized_point_container"), py::arg("lmax"), py::arg("nmax"), py::arg("rmax"), py::arg("store_flmn"),
       py::arg("verbose"), py::arg("parallel"), py::arg("interpolated"), py::call_guard<py::gil_scoped_release>());
+
+  py::class_<FunctionInterpolator>(m, "FunctionInterpolator",
+                                   "A function sampled at equidistant points on [x_min, x_max] and interpolated "
+                                   "linearly; x outside [x_min, x_max) raises ValueError")
+      .def(py::init([](const py::function &fct, unsigned int interpolation_points, double x_min, double x_max) {
+             return FunctionInterpolator([&fct](double x) { return fct(x).cast<double>(); }, interpolation_points,
+                                         x_min, x_max, false);
+           }),
+           py::arg("function"), py::arg("interpolation_points"), py::arg("x_min"), py::arg("x_max"))
+      .def("__call__", [](const FunctionInterpolator &fi, double x) { return fi(x); }, py::arg("x"))
+      .def("__call__", [](const FunctionInterpolator &fi, py::array_t<double> x) {
+             return py::vectorize([&fi](double v) { return fi(v); })(x);
+           },
+           py::arg("x"));
+
+  py::module_ bt = m.def_submodule("besseltools", "Spherical Bessel zeros and double Bessel integrals "
+                                                  "(python/py_besseltools.cpp:12-80)");
+  py::class_<besseltools::SphericalBesselZeros>(bt, "SphericalBesselZeros",
+                                                "Zeros of the spherical Bessel function j_l, computed on demand")
+      .def(py::init<unsigned int>(), py::arg("l"))
+      .def("__getitem__", &besseltools::SphericalBesselZeros::operator[], py::arg("n"))
+      .def("compute_up_to", &besseltools::SphericalBesselZeros::compute_up_to, py::arg("n"));
+  bt.def(
+      "double_sbessel_integrator",
+      [](const FunctionInterpolator &fi, unsigned int l, double r_max, py::array_t<double> k1, py::array_t<double> k2) {
+        return vectorized_k(k1, k2, l, [&](const std::vector<int32_t> &ls, const std::vector<double> &a,
+                                           const std::vector<double> &b) {
+          py::gil_scoped_release nogil;
+          return besseltools::double_sbessel_integrals(fi, ls, a, b, r_max);
+        });
+      },
+      "int_0^r_max w(r) j_l(k1 r) j_l(k2 r) dr with w a FunctionInterpolator (interpolated on the device); "
+      "vectorised over k1, k2",
+      py::arg("w"), py::arg("l"), py::arg("r_max"), py::arg("k1"), py::arg("k2"));
+  bt.def(
+      "double_sbessel_integrator",
+      [](const py::function &w, unsigned int l, double r_max, py::array_t<double> k1, py::array_t<double> k2) {
+        return vectorized_k(k1, k2, l, [&](const std::vector<int32_t> &ls, const std::vector<double> &a,
+                                           const std::vector<double> &b) { return integrals_py(w, ls, a, b, r_max); });
+      },
+      "int_0^r_max w(r) j_l(k1 r) j_l(k2 r) dr, w evaluated at the quadrature nodes (one vectorised call when it "
+      "accepts an array); vectorised over k1, k2",
+      py::arg("w"), py::arg("l"), py::arg("r_max"), py::arg("k1"), py::arg("k2"));
+  bt.def(
+      "double_sbessel_integrals",
+      [](const py::function &w, std::vector<int32_t> l, std::vector<double> k1, std::vector<double> k2, double r_max) {
+        return integrals_py(w, l, k1, k2, r_max);
+      },
+      "The same for arrays of triples (l[i], k1[i], k2[i]) in one device call", py::arg("w"), py::arg("l"),
+      py::arg("k1"), py::arg("k2"), py::arg("r_max"));
+
+  m.def(
+      "shot_noise",
+      [](const FunctionInterpolator &w, unsigned short lmax, double k_max, double rmax, unsigned short nmax,
+         double n_points, double window_volume) {
+        Array2D a;
+        {
+          py::gil_scoped_release nogil;
+          a = shot_noise(w, lmax, k_max, rmax, nmax, n_points, window_volume);
+        }
+        return to_numpy(a);
+      },
+      "Shot-noise term (V/N)(2/pi) k_ln^2 int_0^rmax r^2 w(r) j_l(k_ln r)^2 dr of c_ln, (lmax, nmax)",
+      py::arg("w"), py::arg("lmax"), py::arg("k_max"), py::arg("rmax"), py::arg("nmax"), py::arg("n_points"),
+      py::arg("window_volume") = 0.0);
+  m.def(
+      "shot_noise",
+      [](const py::function &w, unsigned short lmax, double k_max, double rmax, unsigned short nmax, double n_points,
+         double window_volume) {
+        if (!(n_points > 0)) throw std::invalid_argument("shot_noise: n_points must be positive");
+        std::vector<int32_t> ls, ns;
+        std::vector<double> ks;
+        kept_modes(lmax, k_max, rmax, nmax, &ls, &ns, &ks);
+        if (ls.empty()) return to_numpy(Array2D(lmax, nmax));
+        // the integrand r^2 w(r): w is evaluated at the nodes, r^2 applied here
+        py::function w2 = py::cpp_function([w](py::array_t<double> r) {
+          py::array_t<double> wr = py::array_t<double, py::array::forcecast>::ensure(w(r));
+          auto rr = r.unchecked<1>();
+          py::array_t<double> out(rr.shape(0));
+          auto o = out.mutable_unchecked<1>();
+          if (wr.size() == rr.shape(0)) {
+            const double *wp = wr.data();
+            for (py::ssize_t i = 0; i < rr.shape(0); ++i) o(i) = rr(i) * rr(i) * wp[i];
+          } else {
+            const double c = *wr.data();
+            for (py::ssize_t i = 0; i < rr.shape(0); ++i) o(i) = rr(i) * rr(i) * c;
+          }
+          return out;
+        });
+        std::vector<double> I;
+        try {
+          I = integrals_py(w2, ls, ks, ks, rmax);
+        } catch (py::error_already_set &) {
+          // w does not take arrays: per-node calls
+          py::function w1 = py::cpp_function([w](double r) { return r * r * w(r).cast<double>(); });
+          I = integrals_py(w1, ls, ks, ks, rmax);
+        }
+        const double V = window_volume > 0 ? window_volume : 4.0 / 3.0 * M_PI * rmax * rmax * rmax;
+        Array2D out(lmax, nmax);
+        for (size_t i = 0; i < I.size(); ++i) out(ls[i], ns[i]) = V / n_points * (2.0 / M_PI) * ks[i] * ks[i] * I[i];
+        return to_numpy(out);
+      },
+      "Shot-noise term (V/N)(2/pi) k_ln^2 int_0^rmax r^2 w(r) j_l(k_ln r)^2 dr of c_ln, (lmax, nmax)",
+      py::arg("w"), py::arg("lmax"), py::arg("k_max"), py::arg("rmax"), py::arg("nmax"), py::arg("n_points"),
+      py::arg("window_volume") = 0.0);
 
   m.def(
       "compute_kln",

@@ -23,6 +23,7 @@
 #include "sfb_direct_mixed.hpp"
 #include "sfb_lowl.hpp"
 #include "sfb_healpix.hpp"
+#include "sfb_integrals.hpp"
 #include "sfb_math.hpp"
 #include "sfb_support.hpp"
 
@@ -80,6 +81,7 @@ struct sfbhip_handle {
   KlnCache kc;
   int leg_lmax = -1;
   int pix_nside = -1;  // pixel tables on the device are for this nside
+  int ig_L = -1, ig_nz = -1;  // zero rows of the sbessel integrals on the device
 };
 
 namespace {
@@ -1284,6 +1286,204 @@ sfbhip_status sfbhip_healpix(sfbhip_handle *h, const int64_t *d_offsets, const d
   CHK(read_err(h, st, derr, &e));
   CHK(read_timings(h, 0));
   if (e & ERR_DOMAIN) return fail(SFBHIP_DOMAIN_ERROR, "argument (z) outside of defined range.");
+  return SFBHIP_OK;
+}
+
+sfbhip_status sfbhip_bessel_zeros(int32_t l, int32_t n, int32_t mode, double *z_host) {
+  g_last_error.clear();
+  if (l < 0 || n < 0 || (n > 0 && !z_host)) return fail(SFBHIP_INVALID_ARGUMENT, "bessel_zeros: l, n >= 0 required");
+  if (mode != SFBHIP_ZEROS_REFERENCE && mode != SFBHIP_ZEROS_CONVERGED)
+    return fail(SFBHIP_INVALID_ARGUMENT, "bessel_zeros: unknown mode");
+  bessel_zeros(l, n, mode, z_host);
+  return SFBHIP_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+// 16-point Gauss-Legendre nodes and weights on [-1, 1]: Newton on P_16 from
+// the estimate cos(pi (i + 3/4) / (n + 1/2)) (textbook), weights
+// 2 / ((1 - x^2) P_16'(x)^2).
+void gauss_legendre16(double *x, double *w) {
+  const int n = IG_NODES;
+  for (int i = 0; i < n; ++i) {
+    double t = std::cos(M_PI * (i + 0.75) / (n + 0.5)), dp = 1.0;
+    for (int it = 0; it < 100; ++it) {
+      double p0 = 1.0, p1 = t;
+      for (int k = 2; k <= n; ++k) {
+        const double p2 = ((2.0 * k - 1.0) * t * p1 - (k - 1.0) * p0) / k;
+        p0 = p1;
+        p1 = p2;
+      }
+      dp = n * (t * p1 - p0) / (t * t - 1.0);
+      const double d = p1 / dp;
+      t -= d;
+      if (std::fabs(d) < 1e-16) break;
+    }
+    x[i] = t;
+    w[i] = 2.0 / ((1.0 - t * t) * dp * dp);
+  }
+}
+
+struct IntegralCall {
+  IntegralPlan *plan = nullptr;
+  const double *zeros = nullptr;
+  const double *gl = nullptr;
+  int nz = 0;
+};
+
+// Validates the triples, sizes and computes the zero rows (cached per
+// handle), uploads the triples and runs the plan kernel.
+sfbhip_status prep_integrals(sfbhip_handle *h, int64_t n, const int32_t *l, const double *k1, const double *k2,
+                             double rmax, hipStream_t st, IntegralCall *c) {
+  if (!h) return fail(SFBHIP_INVALID_ARGUMENT, "handle is NULL");
+  if (n <= 0 || !l || !k1 || !k2) return fail(SFBHIP_INVALID_ARGUMENT, "sbessel integrals: no triples");
+  if (!(rmax > 0.0) || !std::isfinite(rmax)) return fail(SFBHIP_INVALID_ARGUMENT, "r_max must be positive");
+  int L = 0;
+  double X = 0.0;
+  for (int64_t t = 0; t < n; ++t) {
+    if (l[t] < 0 || l[t] > 100000) return fail(SFBHIP_INVALID_ARGUMENT, "multipole l out of range");
+    if (!(k1[t] >= 0.0) || !(k2[t] >= 0.0) || !std::isfinite(k1[t]) || !std::isfinite(k2[t]))
+      return fail(SFBHIP_INVALID_ARGUMENT, "k1, k2 must be finite and >= 0");
+    L = std::max(L, l[t] + 1);
+    X = std::max(X, std::max(k1[t], k2[t]) * rmax);
+  }
+  // zeros of j_l are more than pi apart, so fewer than X / pi + 1 lie below X
+  const double nzd = std::floor(X / M_PI) + 3.0;
+  if (nzd * L > (double)(1 << 27))
+    return fail(SFBHIP_NOT_SUPPORTED, "sbessel integrals: k r_max too large for the zero table");
+  const int nz = (int)nzd;
+  HIPCHK(hipSetDevice(h->device));
+  double *dz, *dgl;
+  CHK(get_buf(h, "ig_zeros", (size_t)L * nz, &dz));
+  CHK(get_buf(h, "ig_gl", 2 * IG_NODES, &dgl));
+  static thread_local std::vector<double> glh;
+  if (glh.empty()) {
+    glh.resize(2 * IG_NODES);
+    gauss_legendre16(glh.data(), glh.data() + IG_NODES);
+  }
+  HIPCHK(hipMemcpyAsync(dgl, glh.data(), sizeof(double) * 2 * IG_NODES, hipMemcpyHostToDevice, st));
+  if (h->ig_L != L || h->ig_nz != nz) {
+    bessel_zeros_kernel<<<(unsigned)cdiv(L, 64), 64, 0, st>>>(L, nz, SFBHIP_ZEROS_REFERENCE, dz);
+    HIPCHK(hipGetLastError());
+    h->ig_L = L;
+    h->ig_nz = nz;
+  }
+  int32_t *dl;
+  double *dk1, *dk2;
+  IntegralPlan *dplan;
+  CHK(get_buf(h, "ig_l", (size_t)n, &dl));
+  CHK(get_buf(h, "ig_k1", (size_t)n, &dk1));
+  CHK(get_buf(h, "ig_k2", (size_t)n, &dk2));
+  CHK(get_buf(h, "ig_plan", (size_t)n, &dplan));
+  HIPCHK(hipMemcpyAsync(dl, l, sizeof(int32_t) * n, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(dk1, k1, sizeof(double) * n, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(dk2, k2, sizeof(double) * n, hipMemcpyHostToDevice, st));
+  int *derr;
+  CHK(zero_err(h, st, &derr));
+  sbessel_plan_kernel<<<(unsigned)cdiv(n, 256), 256, 0, st>>>(n, dl, dk1, dk2, rmax, dz, nz, dplan, derr);
+  HIPCHK(hipGetLastError());
+  int e = 0;
+  CHK(read_err(h, st, derr, &e));
+  if (e) return fail(SFBHIP_HIP_ERROR, "sbessel integrals: zero table too short");
+  c->plan = dplan;
+  c->zeros = dz;
+  c->gl = dgl;
+  c->nz = nz;
+  return SFBHIP_OK;
+}
+
+// Node offsets of the triples (host prefix sum of 16 x pieces), uploaded.
+sfbhip_status integral_offsets(sfbhip_handle *h, int64_t n, const IntegralCall &c, hipStream_t st,
+                               int64_t **d_off, int64_t *total) {
+  std::vector<IntegralPlan> hp((size_t)n);
+  HIPCHK(hipMemcpyAsync(hp.data(), c.plan, sizeof(IntegralPlan) * n, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  std::vector<int64_t> off((size_t)n);
+  int64_t s = 0;
+  for (int64_t t = 0; t < n; ++t) {
+    off[t] = s;
+    s += IG_NODES * hp[t].npieces;
+  }
+  CHK(get_buf(h, "ig_off", (size_t)n, d_off));
+  HIPCHK(hipMemcpyAsync(*d_off, off.data(), sizeof(int64_t) * n, hipMemcpyHostToDevice, st));
+  HIPCHK(hipStreamSynchronize(st));  // the host vector goes out of scope
+  *total = s;
+  return SFBHIP_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+sfbhip_status sfbhip_sbessel_nodes(sfbhip_handle *h, int64_t n, const int32_t *l, const double *k1,
+                                   const double *k2, double rmax, int64_t *n_nodes, double *r_nodes) {
+  g_last_error.clear();
+  if (!n_nodes) return fail(SFBHIP_INVALID_ARGUMENT, "n_nodes is NULL");
+  hipStream_t st = nullptr;
+  IntegralCall c;
+  CHK(prep_integrals(h, n, l, k1, k2, rmax, st, &c));
+  int64_t *doff, total = 0;
+  CHK(integral_offsets(h, n, c, st, &doff, &total));
+  *n_nodes = total;
+  if (!r_nodes) return SFBHIP_OK;
+  double *dr;
+  CHK(get_buf(h, "ig_nodes", (size_t)std::max<int64_t>(total, 1), &dr));
+  sbessel_nodes_kernel<<<(unsigned)cdiv(n, 4), 256, 0, st>>>(n, c.plan, c.zeros, c.nz, rmax, c.gl, doff, dr);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(r_nodes, dr, sizeof(double) * total, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  return SFBHIP_OK;
+}
+
+sfbhip_status sfbhip_sbessel_integrals(sfbhip_handle *h, int64_t n, const int32_t *l, const double *k1,
+                                       const double *k2, double rmax, const sfbhip_weight *w, double *out) {
+  g_last_error.clear();
+  if (!w || !out) return fail(SFBHIP_INVALID_ARGUMENT, "weight or output is NULL");
+  if (w->kind != SFBHIP_WEIGHT_NODES && w->kind != SFBHIP_WEIGHT_TABLE)
+    return fail(SFBHIP_INVALID_ARGUMENT, "unknown weight kind");
+  if (w->n > 0 && !w->values) return fail(SFBHIP_INVALID_ARGUMENT, "weight values are NULL");
+  if (w->r_power < 0 || w->r_power > 8) return fail(SFBHIP_INVALID_ARGUMENT, "weight r_power must be in 0..8");
+  if (w->kind == SFBHIP_WEIGHT_TABLE) {
+    if (w->n < 2 || w->n > INT32_MAX) return fail(SFBHIP_INVALID_ARGUMENT, "weight table needs >= 2 samples");
+    if (!(w->x_max > w->x_min) || !std::isfinite(w->x_min) || !std::isfinite(w->x_max))
+      return fail(SFBHIP_INVALID_ARGUMENT, "weight table range must satisfy x_min < x_max");
+  }
+  hipStream_t st = nullptr;
+  IntegralCall c;
+  CHK(prep_integrals(h, n, l, k1, k2, rmax, st, &c));
+  WeightRef W{};
+  W.kind = w->kind == SFBHIP_WEIGHT_NODES ? W_NODES : W_TABLE;
+  W.r_power = w->r_power;
+  double *dv;
+  if (w->kind == SFBHIP_WEIGHT_NODES) {
+    int64_t *doff, total = 0;
+    CHK(integral_offsets(h, n, c, st, &doff, &total));
+    if (w->n != total)
+      return fail(SFBHIP_INVALID_ARGUMENT, "weight: " + std::to_string(w->n) + " node values given, the triples have " +
+                                               std::to_string(total) + " nodes");
+    W.node_off = doff;
+  } else {
+    W.ntab = (int32_t)w->n;
+    W.x_min = w->x_min;
+    W.x_max = w->x_max;
+    W.dx_inv = (double)(w->n - 1) / (w->x_max - w->x_min);
+  }
+  CHK(get_buf(h, "ig_w", (size_t)std::max<int64_t>(w->n, 1), &dv));
+  HIPCHK(hipMemcpyAsync(dv, w->values, sizeof(double) * w->n, hipMemcpyHostToDevice, st));
+  W.values = dv;
+  double *dout;
+  CHK(get_buf(h, "ig_out", (size_t)n, &dout));
+  int *derr;
+  CHK(zero_err(h, st, &derr));
+  sbessel_integral_kernel<<<(unsigned)cdiv(n, 4), 256, 0, st>>>(n, c.plan, c.zeros, c.nz, rmax, c.gl, W, dout, derr);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(out, dout, sizeof(double) * n, hipMemcpyDeviceToHost, st));
+  int e = 0;
+  CHK(read_err(h, st, derr, &e));
+  if (e & ERR_DOMAIN)
+    return fail(SFBHIP_DOMAIN_ERROR, "argument outside range for which this FunctionInterpolator is defined");
   return SFBHIP_OK;
 }
 

@@ -149,6 +149,42 @@ sfbhip_status sfbhip_healpix(sfbhip_handle *h, const int64_t *d_offsets, const d
                              int32_t nside, const sfbhip_params *p, double *d_k_ln, double *d_c_ln,
                              double *d_f_lmn, void *stream);
 
+/* First n positive zeros of j_l on the host (SFBHIP_ZEROS_* mode).
+ * Replaces besseltools::SphericalBesselZeros(l)[0..n-1] (SphericalBesselZeros.cpp:11-101). */
+sfbhip_status sfbhip_bessel_zeros(int32_t l, int32_t n, int32_t mode, double *z_host);
+
+/* Weight function w(r) of the double spherical-Bessel integrals. */
+enum {
+  SFBHIP_WEIGHT_NODES = 0, /* values[i]: w at node i of sfbhip_sbessel_nodes for the same triples */
+  SFBHIP_WEIGHT_TABLE = 1  /* values[0..n): samples on [x_min, x_max], linearly interpolated;
+                              an argument outside [x_min, x_max) is a domain error
+                              (FunctionInterpolator.cpp:7-30) */
+};
+typedef struct sfbhip_weight {
+  int32_t kind;          /* SFBHIP_WEIGHT_* */
+  int32_t r_power;       /* the integrand carries w(r) r^r_power (0..8; 2 for the shot-noise term) */
+  int64_t n;             /* NODES: node count of the call; TABLE: sample count (>= 2) */
+  const double *values;  /* host array of n doubles */
+  double x_min, x_max;   /* TABLE: the sampled range */
+} sfbhip_weight;
+
+/* Quadrature nodes of the batched integrals below, on the host: with
+ * r_nodes == NULL only *n_nodes is set; otherwise r_nodes (host, *n_nodes
+ * doubles) receives every node, triple by triple.  A caller with an arbitrary
+ * w evaluates it there and passes the values as SFBHIP_WEIGHT_NODES.
+ * l, k1, k2: host arrays of n triples (l >= 0, k >= 0). */
+sfbhip_status sfbhip_sbessel_nodes(sfbhip_handle *h, int64_t n, const int32_t *l, const double *k1,
+                                   const double *k2, double rmax, int64_t *n_nodes, double *r_nodes);
+
+/* out[t] = int_0^rmax w(r) j_l(k1 r) j_l(k2 r) dr for n triples (host arrays),
+ * computed on the device: ranges between the zeros of both Bessel functions,
+ * 16-point Gauss-Legendre pieces, and the reference's stop after an outer
+ * range that changes the sum by less than 1e-10 of itself.
+ * Replaces besseltools::double_sbessel_integrator / _bz (sbessel_integrator.cpp:45-110),
+ * which the reference's Python binding vectorises over k1, k2 (py_besseltools.cpp:45-78). */
+sfbhip_status sfbhip_sbessel_integrals(sfbhip_handle *h, int64_t n, const int32_t *l, const double *k1,
+                                       const double *k2, double rmax, const sfbhip_weight *w, double *out);
+
 /* Duration in ms of the dominant kernel of the last call on this handle
  * (the mixed contraction kernels when the call ran any, else the FP64
  * contraction kernels), measured with HIP events on the call's stream. */

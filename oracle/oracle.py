@@ -107,6 +107,33 @@ def interp_table(l, nmax, mode=ZEROS_REFERENCE):
     return y, zmax.value, dxi.value
 
 
+W_POWER_GAUSS = 0
+W_TABLE = 1
+
+
+def sbessel_integral(l, rmax, k1, k2, power=0.0, gauss=0.0, table=None):
+    """int_0^rmax w(r) j_l(k1 r) j_l(k2 r) dr (double_sbessel_integrator,
+    sbessel_integrator.cpp:45-110).  w = r^power exp(-(r/gauss)^2) (gauss <= 0:
+    no Gaussian), or table = (samples, xmin, xmax) linearly interpolated."""
+    L = lib()
+    if not getattr(L, "_dsb_init", False):
+        L.orc_sbessel_integral.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                           ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_void_p,
+                                           ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_void_p]
+        L._dsb_init = True
+    out = ctypes.c_double()
+    if table is None:
+        rc = L.orc_sbessel_integral(int(l), float(rmax), float(k1), float(k2), W_POWER_GAUSS, float(power),
+                                    float(gauss), None, 0, 0.0, 0.0, ctypes.byref(out))
+    else:
+        y, x0, x1 = table
+        y = np.ascontiguousarray(y, dtype=np.float64)
+        rc = L.orc_sbessel_integral(int(l), float(rmax), float(k1), float(k2), W_TABLE, 0.0, 0.0, _p(y), y.size,
+                                    float(x0), float(x1), ctypes.byref(out))
+    _check(rc, "sbessel_integral")
+    return out.value
+
+
 def _split_flmn(flat, lmax, nmax):
     out, off = [], 0
     c = flat.view(np.complex128)

@@ -706,6 +706,221 @@ int orc_sfb_reverse(const long long *offsets, const double *radii, int nside, in
 /* ------------------------------------------------------------------------ */
 /* KClkk::savetxt format, return_types.cpp:25-37                            */
 /* ------------------------------------------------------------------------ */
+/* double spherical-Bessel integral (sbessel_integrator.cpp:45-138)          */
+/* ------------------------------------------------------------------------ */
+
+typedef struct {
+  int l, kind;
+  double k1, k2, a, b, xmin, xmax, dxinv;
+  const double *tab;
+  int ntab, err;
+} orc_dsb_t;
+
+static double dsb_weight(orc_dsb_t *c, double r) {
+  if (c->kind == ORC_W_TABLE) {
+    /* FunctionInterpolator::operator() (FunctionInterpolator.cpp:21-30) */
+    if (r < c->xmin || r >= c->xmax) {
+      c->err = ORC_EDOMAIN;
+      return 0.0;
+    }
+    double u = (r - c->xmin) * c->dxinv;
+    int i = (int)u;
+    if (i > c->ntab - 2) i = c->ntab - 2;
+    return c->tab[i] + (c->tab[i + 1] - c->tab[i]) * (u - i);
+  }
+  double w = c->a == 0.0 ? 1.0 : pow(r, c->a);
+  if (c->b > 0.0) w *= exp(-(r / c->b) * (r / c->b));
+  return w;
+}
+
+static double dsb_f(orc_dsb_t *c, double r) {
+  return dsb_weight(c, r) * orc_sbessel_jl(c->l, c->k1 * r) * orc_sbessel_jl(c->l, c->k2 * r);
+}
+
+/* Gauss-Legendre nodes/weights on [-1, 1] by Newton on P_n (textbook) */
+#define DSB_GL 10
+static double dsb_x[DSB_GL], dsb_w[DSB_GL];
+static int dsb_init_done = 0;
+static void dsb_init(void) {
+  if (dsb_init_done) return;
+  const int n = DSB_GL;
+  for (int i = 0; i < n; ++i) {
+    double x = cos(ORC_PI * (i + 0.75) / (n + 0.5)), dp = 1.0;
+    for (int it = 0; it < 100; ++it) {
+      double p0 = 1.0, p1 = x;
+      for (int k = 2; k <= n; ++k) {
+        double p2 = ((2.0 * k - 1.0) * x * p1 - (k - 1.0) * p0) / k;
+        p0 = p1;
+        p1 = p2;
+      }
+      dp = n * (x * p1 - p0) / (x * x - 1.0);
+      double dx = p1 / dp;
+      x -= dx;
+      if (fabs(dx) < 1e-16) break;
+    }
+    dsb_x[i] = x;
+    dsb_w[i] = 2.0 / ((1.0 - x * x) * dp * dp);
+  }
+  dsb_init_done = 1;
+}
+
+/* Gauss-Legendre on [a, b]; *abs_out gets the same rule applied to |f| */
+static double dsb_gl(orc_dsb_t *c, double a, double b, double *abs_out) {
+  double h = 0.5 * (b - a), m = 0.5 * (a + b), s = 0.0, sa = 0.0;
+  for (int i = 0; i < DSB_GL; ++i) {
+    double f = dsb_f(c, m + h * dsb_x[i]);
+    s += dsb_w[i] * f;
+    sa += dsb_w[i] * fabs(f);
+  }
+  if (abs_out) *abs_out = sa * fabs(h);
+  return s * h;
+}
+
+/* bisection until the two halves agree with the whole to tol (halved with
+ * each bisection): 1e-13 of the piece's integral of |f|, plus 1e-14 of the
+ * width times the envelope |w| min(1, 1/(k1 r)) min(1, 1/(k2 r)) -- near the
+ * zeros of j_l the integrand's own rounding sits ~1e-16 of that envelope, not
+ * of the (small) local |f| */
+static double dsb_adapt(orc_dsb_t *c, double a, double b, double whole, double tol, int depth) {
+  double m = 0.5 * (a + b), L = dsb_gl(c, a, m, NULL), R = dsb_gl(c, m, b, NULL);
+  if (depth >= 18 || fabs(L + R - whole) <= tol) return L + R;
+  return dsb_adapt(c, a, m, L, 0.5 * tol, depth + 1) + dsb_adapt(c, m, b, R, 0.5 * tol, depth + 1);
+}
+
+static double dsb_piece(orc_dsb_t *c, double a, double b) {
+  double ga, g = dsb_gl(c, a, b, &ga);
+  const double m = 0.5 * (a + b), x1 = c->k1 * m, x2 = c->k2 * m;
+  const double env = fabs(dsb_weight(c, m)) * (x1 > 1.0 ? 1.0 / x1 : 1.0) * (x2 > 1.0 ? 1.0 / x2 : 1.0);
+  return dsb_adapt(c, a, b, g, 1e-13 * ga + 1e-14 * (b - a) * env + 1e-300, 0);
+}
+
+/* one range; a table weight is split at its sample points */
+static double dsb_range(orc_dsb_t *c, double a, double b) {
+  if (!(b > a)) return 0.0;
+  if (c->kind == ORC_W_TABLE) {
+    double s = 0.0, lo = a;
+    int i = (int)floor((a - c->xmin) * c->dxinv) + 1;
+    for (;; ++i) {
+      double x = c->xmin + i / c->dxinv, hi = x < b ? x : b;
+      if (hi > lo) s += dsb_piece(c, lo, hi);
+      lo = hi;
+      if (!(x < b)) break;
+    }
+    return s;
+  }
+  return dsb_piece(c, a, b);
+}
+
+/* zeros of j_l as a growing array (SphericalBesselZeros: operator[], compute_up_to) */
+typedef struct {
+  int l, n, cap;
+  double *z;
+} dsb_zeros_t;
+
+static int dsb_zeros_push(dsb_zeros_t *zs) {
+  if (zs->n == zs->cap) {
+    int cap = zs->cap ? 2 * zs->cap : 64;
+    double *nz = (double *)realloc(zs->z, sizeof(double) * cap);
+    if (!nz) return ORC_ENOMEM;
+    zs->z = nz;
+    zs->cap = cap;
+  }
+  if (zs->n < 2)
+    zs->z[zs->n] = newton(zs->l, zero_guess(zs->l, zs->n), ORC_ZEROS_REFERENCE);
+  else
+    zs->z[zs->n] = newton(zs->l, 2.0 * zs->z[zs->n - 1] - zs->z[zs->n - 2], ORC_ZEROS_REFERENCE);
+  zs->n++;
+  return ORC_OK;
+}
+
+/* zeros up to index idx (operator[], SphericalBesselZeros.cpp:95-100) */
+static int dsb_zeros_index(dsb_zeros_t *zs, int idx) {
+  while (zs->n <= idx) {
+    int rc = dsb_zeros_push(zs);
+    if (rc) return rc;
+  }
+  return ORC_OK;
+}
+
+/* compute_up_to(zmax): until the last zero is >= zmax (:70-74) */
+static int dsb_zeros_reach(dsb_zeros_t *zs, double zmax) {
+  while (zs->n < 2 || zs->z[zs->n - 1] < zmax) {
+    int rc = dsb_zeros_push(zs);
+    if (rc) return rc;
+  }
+  return ORC_OK;
+}
+
+/* SBesselIntegrationRangeGenerator (sbessel_integrator.cpp:113-138) */
+typedef struct {
+  dsb_zeros_t *zs;
+  double k, rmax, low;
+  int hi;
+} dsb_gen_t;
+
+static int dsb_gen_set(dsb_gen_t *g, double rmin, double rmax) {
+  g->rmax = rmax;
+  g->low = rmin;
+  /* next_zero(rmin k): index of the first zero above rmin k (:76-80) */
+  int rc = dsb_zeros_reach(g->zs, rmin * g->k);
+  if (rc) return rc;
+  int ix = 0;
+  while (ix < g->zs->n && !(g->zs->z[ix] > rmin * g->k)) ++ix;
+  rc = dsb_zeros_index(g->zs, ix);
+  if (rc) return rc;
+  g->hi = (g->zs->z[ix] / g->k > rmin) ? ix : ix + 1;
+  return ORC_OK;
+}
+
+static int dsb_gen_next(dsb_gen_t *g, double *lo, double *hi) {
+  int rc = dsb_zeros_index(g->zs, g->hi);
+  if (rc) return rc;
+  double z = g->zs->z[g->hi] / g->k;
+  *lo = g->low;
+  *hi = z < g->rmax ? z : g->rmax;
+  g->low = *hi;
+  g->hi++;
+  return ORC_OK;
+}
+
+int orc_sbessel_integral(int l, double rmax, double k1, double k2, int wkind, double a, double b,
+                         const double *tab, int ntab, double xmin, double xmax, double *out) {
+  if (l < 0 || !(rmax > 0.0) || !(k1 >= 0.0) || !(k2 >= 0.0) || !out) return ORC_EINVAL;
+  if (wkind == ORC_W_TABLE && (!tab || ntab < 2 || !(xmax > xmin))) return ORC_EINVAL;
+  dsb_init();
+  orc_dsb_t c = {l, wkind, k1, k2, a, b, xmin, xmax, (ntab - 1) / (xmax - xmin), tab, ntab, 0};
+  if (k1 > k2) {
+    double t = k1;
+    k1 = k2;
+    k2 = t;
+  }
+  dsb_zeros_t zs = {l, 0, 0, NULL};
+  dsb_gen_t gl = {&zs, k1, 0, 0, 0}, gs = {&zs, k2, 0, 0, 0};
+  double result = 0.0, lo, hi, slo, shi;
+  int rc = dsb_gen_set(&gl, 0.0, rmax);
+  if (!rc) rc = dsb_gen_next(&gl, &lo, &hi);
+  while (!rc && lo != hi) {
+    double inter = 0.0;
+    rc = dsb_gen_set(&gs, lo, hi);
+    if (!rc) rc = dsb_gen_next(&gs, &slo, &shi);
+    while (!rc && slo != shi) {
+      inter += dsb_range(&c, slo, shi);
+      rc = dsb_gen_next(&gs, &slo, &shi);
+    }
+    if (rc) break;
+    double old = result;
+    result += inter;
+    if (fabs(result - old) < result * 1e-10) break;
+    rc = dsb_gen_next(&gl, &lo, &hi);
+  }
+  free(zs.z);
+  if (rc) return rc;
+  if (c.err) return c.err;
+  *out = result;
+  return ORC_OK;
+}
+
+/* ------------------------------------------------------------------------ */
 
 int orc_savetxt(const char *path, const double *a, int rows, int cols) {
   FILE *f = fopen(path, "w");

@@ -69,6 +69,24 @@ int orc_sfb_reverse(const long long *offsets, const double *radii, int nside, in
                     double rmax, const int *nl, double volume, int interpolated, int zeros_mode,
                     int nthreads, double *k_ln, double *c_ln, double *f_lmn);
 
+/* ---- double spherical-Bessel integral (the shot-noise step after the path) ----
+ * I = int_0^rmax w(r) j_l(k1 r) j_l(k2 r) dr, restating
+ * besseltools::double_sbessel_integrator (sbessel_integrator.cpp:45-110): outer
+ * ranges between zeros of j_l(k_small r), each split at the zeros of
+ * j_l(k_large r) (SBesselIntegrationRangeGenerator, :113-138), the sum stopping
+ * after an outer range that changes it by less than 1e-10 of itself (:99-100).
+ * Each range is integrated by adaptive Gauss-Legendre to ~1e-13 relative (the
+ * reference uses GSL QAG 21-point Gauss-Kronrod to 1e-7, :88).
+ * Weight kinds: ORC_W_POWER_GAUSS  w(r) = r^a * exp(-(r/b)^2)   (b <= 0: no Gaussian)
+ *               ORC_W_TABLE        linear interpolation of ntab samples on
+ *                                  [xmin, xmax] (FunctionInterpolator.cpp:7-30; a
+ *                                  range is split at every sample so the
+ *                                  integrand is smooth on each piece); an argument
+ *                                  outside [xmin, xmax) is a domain error. */
+enum { ORC_W_POWER_GAUSS = 0, ORC_W_TABLE = 1 };
+int orc_sbessel_integral(int l, double rmax, double k1, double k2, int wkind, double a, double b,
+                         const double *tab, int ntab, double xmin, double xmax, double *out);
+
 /* KClkk::savetxt text format (return_types.cpp:25-64): precision 18, scientific */
 int orc_savetxt(const char *path, const double *a, int rows, int cols);
 

@@ -74,3 +74,15 @@ def test_ang2pix_ring_rejects_bad_nside():
     for ns in (0, 3, 255, 16384):
         with pytest.raises(ValueError, match="power of 2"):
             capi.check(capi.lib().sfbhip_ang2pix_ring(ns, 0.5, 0.5, ctypes.byref(out)), "ang2pix")
+
+
+def test_bessel_zeros_host():
+    # SphericalBesselZeros known answers (SphericalBesselZeros_test.cpp:9-15,
+    # test/python/besseltools_test.py:11-19) and the oracle's zeros
+    z0 = capi.bessel_zeros(0, 3)
+    np.testing.assert_allclose(z0, [np.pi, 2 * np.pi, 3 * np.pi], rtol=1e-15)
+    assert abs(capi.bessel_zeros(5, 10)[9] - 38.8836) < 1e-4
+    for l in (0, 1, 7, 64, 255):
+        np.testing.assert_allclose(capi.bessel_zeros(l, 40), O.bessel_zeros(l, 40), rtol=1e-14)
+    with pytest.raises(ValueError):
+        capi.bessel_zeros(-1, 3)

@@ -98,3 +98,38 @@ class TestPixelizedPointContainer:
         assert q == int(O.ang2pix_ring(256, np.array([1.2]), np.array([0.4]))[0])
         t, p = O.pix2ang_ring(256, q)
         assert abs(t - 1.2) < 1e-2 * 1.2 and abs(p - 0.4) < 1e-2 * 0.4
+
+
+# ---- besseltools and FunctionInterpolator (host behaviour only) ----
+
+def test_spherical_bessel_zeros_api():
+    # test/python/besseltools_test.py:6-23
+    import catana_amd as catana
+    with pytest.raises(TypeError):
+        catana.besseltools.SphericalBesselZeros()
+    z = catana.besseltools.SphericalBesselZeros(0)
+    assert z[0] == pytest.approx(np.pi)
+    assert z[1] == pytest.approx(6.283185307179586)
+    assert z[2] == pytest.approx(9.42477796076938)
+    assert catana.besseltools.SphericalBesselZeros(5)[9] == pytest.approx(38.8836)
+    z2 = catana.besseltools.SphericalBesselZeros(2)
+    z2.compute_up_to(100)
+    assert z2[30] > 100 > z2[29] or z2[29] > 100  # zeros extend past 100
+
+
+def test_function_interpolator_api():
+    # test/python/besseltools_test.py:39-52
+    import catana_amd as catana
+    w = lambda r: np.exp(-(r / 40) ** 2) * r ** 2  # noqa: E731
+    fi = catana.FunctionInterpolator(w, 100000, 0, 100)
+    assert fi(0) == w(0)
+    assert fi(99.9) == pytest.approx(w(99.9))
+    assert fi(20) == pytest.approx(fi(20))
+    for bad in (100, -1, 120):
+        with pytest.raises(ValueError):
+            fi(bad)
+    x = np.array([1.0, 2.5, 50.0])
+    np.testing.assert_allclose(fi(x), w(x), rtol=1e-8)
+    # linear interpolation between samples: exact for a linear function
+    fl = catana.FunctionInterpolator(lambda r: 3.0 - 2.0 * r, 11, 0.0, 10.0)
+    assert fl(3.7) == pytest.approx(3.0 - 7.4, rel=1e-15)

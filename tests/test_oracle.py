@@ -258,3 +258,72 @@ def test_savetxt_format(tmp_path):
     # std::ostream with precision(18) + scientific prints what printf("%.18e") prints
     assert lines[1] == "%.18e %.18e" % (5.0e-300, 3.0)
     np.testing.assert_array_equal(np.loadtxt(p), a)
+
+
+# ---------------------------------------------------------------- double spherical-Bessel integral
+def _lommel(l, R, a, b):
+    """int_0^R r^2 j_l(a r) j_l(b r) dr, a != b (Lommel's integral)."""
+    from scipy.special import spherical_jn as J
+    jm = (lambda x: np.cos(x) / x) if l == 0 else (lambda x: J(l - 1, x))
+    return R ** 2 * (b * J(l, a * R) * jm(b * R) - a * jm(a * R) * J(l, b * R)) / (a * a - b * b)
+
+
+def _diag(l, R, k):
+    """int_0^R r^2 j_l(k r)^2 dr = R^3/2 [j_l(kR)^2 - j_{l-1}(kR) j_{l+1}(kR)]."""
+    from scipy.special import spherical_jn as J
+    x = k * R
+    jm = np.cos(x) / x if l == 0 else J(l - 1, x)
+    return R ** 3 / 2 * (J(l, x) ** 2 - jm * J(l + 1, x))
+
+
+def test_sbessel_integral_reference_known_values():
+    # Mathematica values of test/besseltools_tests/sbessel_integrator.cpp:15-19
+    # (test/python/besseltools_test.py:27-31), w = 1
+    assert abs(O.sbessel_integral(0, 100, 1, 1) - 1.56582) < 1e-5
+    assert abs(O.sbessel_integral(0, 100, 1, 2) - 0.785393) < 1e-5
+    assert abs(O.sbessel_integral(4, 100, 1, 4.2) - 0.000133933) < 1e-9
+
+
+def test_sbessel_integral_symmetric_in_k():
+    # sbessel_integrator.cpp (test) :8-13, w = exp(-(r/40)^2) r^2
+    a = O.sbessel_integral(6, 100, 2.3, 4.5, power=2, gauss=40)
+    b = O.sbessel_integral(6, 100, 4.5, 2.3, power=2, gauss=40)
+    assert abs(a - b) <= 1e-12 * abs(a)
+
+
+@pytest.mark.parametrize("l,k", [(0, 0.3), (1, 0.05), (5, 0.2), (40, 0.11), (255, 0.3), (255, 0.5)])
+def test_sbessel_integral_diagonal_closed_form(l, k):
+    R = 1000.0
+    v = O.sbessel_integral(l, R, k, k, power=2)
+    assert abs(v - _diag(l, R, k)) <= 1e-10 * R / k ** 2
+
+
+@pytest.mark.parametrize("l,a,b", [(0, 0.013, 0.037), (5, 0.2, 0.21), (17, 0.1, 0.35), (255, 0.27, 0.5)])
+def test_sbessel_integral_lommel(l, a, b):
+    R = 1000.0
+    v = O.sbessel_integral(l, R, a, b, power=2)
+    assert abs(v - _lommel(l, R, a, b)) <= 1e-10 * R / (a * b)
+
+
+def test_sbessel_integral_linear_table_is_exact():
+    # a linear w is reproduced exactly by the interpolating table
+    R, l, k1, k2 = 300.0, 3, 0.07, 0.11
+    x = np.linspace(0.0, 400.0, 801)
+    table = (2.0 + 0.5 * x, 0.0, 400.0)
+    v = O.sbessel_integral(l, R, k1, k2, table=table)
+    ref = 2.0 * O.sbessel_integral(l, R, k1, k2) + 0.5 * O.sbessel_integral(l, R, k1, k2, power=1)
+    assert abs(v - ref) <= 1e-11 * R / (k1 * k2)
+    with pytest.raises(ValueError):  # the table must cover [0, R)
+        O.sbessel_integral(l, R, k1, k2, table=(np.ones(11), 0.0, 200.0))
+
+
+def test_sbessel_integral_stopping_rule_keeps_the_value():
+    # a narrow Gaussian window: outer ranges past r ~ 300 change the sum by less
+    # than 1e-10 and the loop stops there; scipy's quad on [0, 400] agrees
+    from scipy.integrate import quad
+    from scipy.special import spherical_jn as J
+    l, k, g, R = 3, 0.1, 50.0, 1000.0
+    v = O.sbessel_integral(l, R, k, k, power=2, gauss=g)
+    ref = quad(lambda r: r * r * np.exp(-(r / g) ** 2) * J(l, k * r) ** 2, 0, 400, limit=400, epsabs=0,
+               epsrel=1e-13)[0]
+    assert abs(v - ref) <= 1e-9 * ref
