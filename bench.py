@@ -1,6 +1,7 @@
 """Benchmark of the direct (point-mode) SFB decomposition on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload clustered1m|lognormal2p25|uniform10k]
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+                    [--workload clustered1m|lognormal2p25|uniform10k|batched64|healpix4m|shotnoise2p25]
                     [--precision mixed|fp64] [--cpu-sample N | --no-cpu-baseline]
 
 A step is one full sfbhip_direct call (k_ln lookup, point preparation, the
@@ -37,6 +38,9 @@ WORKLOADS = {
     # HEALPix path: 4,000,000 points, p(r) ~ r^2 exp(-(r/400)^2) on [0, 1000], nside 256, lmax 128,
     # k_max 0.15 (FP64; the reference's public wrapper semantics: linear Bessel table for N > 1000)
     "healpix4m": dict(lmax=128, nmax=48, rmax=1000.0, k_max=0.15, nside=256, healpix=True),
+    # the shot-noise step after the path: (V/N)(2/pi) k^2 int r^2 w(r) j_l(k r)^2 dr for every kept
+    # mode of the 2^25-point configuration (lmax 256, k_max 0.5, rmax 1000: 26,102 integrals), w = tophat
+    "shotnoise2p25": dict(lmax=256, nmax=160, rmax=1000.0, k_max=0.5, shotnoise=True),
 }
 
 
@@ -58,7 +62,7 @@ def make_batch(seed=0, ncat=64):
 # rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE on this build, scaled by what the counters read
 # for a 512 MiB device-to-device copy (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0).  Counters cannot
 # be read from inside the timed run, so the figure is carried here per workload.
-PROFILED_TRAFFIC_BYTES = {("clustered1m", "mixed", "split16"): 2.579e9}
+PROFILED_TRAFFIC_BYTES = {("clustered1m", "mixed", "split16", 2, 4): 3.132e9}  # (workload, precision, operands, fp64_below, chunks)
 
 
 def make_points(name):
@@ -139,6 +143,9 @@ def main():
     if cfg.get("healpix"):
         run_healpix(args, cfg)
         return
+    if cfg.get("shotnoise"):
+        run_shotnoise(args, cfg)
+        return
     log(f"generating workload {args.workload}")
     pts = make_points(args.workload)
     log(f"{pts.shape[0]} points")
@@ -201,7 +208,8 @@ def main():
                    "wall_s_timed_region": wall},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
                      "frac": achieved / peak,
-                     "traffic": PROFILED_TRAFFIC_BYTES.get((args.workload, args.precision, args.operands)),
+                     "traffic": PROFILED_TRAFFIC_BYTES.get((args.workload, args.precision, args.operands, args.fp64_below,
+                                                              args.chunks)),
                      "kernel": "sfb_direct_%s_kernel" % ("f64" if prec == capi.PREC_FP64 else "mixed"),
                      "kernel_ms": kms, "algorithmic_flops_per_launch": flops_dom,
                      "l_range": [lo, cfg["lmax"]],
@@ -292,6 +300,73 @@ def run_healpix(args, cfg):
     # no CPU leg here: the oracle evaluates each needed ring-DFT bin directly,
     # O(npix (l+1)) per (l, n), far beyond a bounded sample at nside 256
     out["cpu_baseline"] = None
+    h.close()
+    print(json.dumps(out))
+
+
+# FP64 VALU operations per node of sbessel_integral_kernel (ADD + MUL + 2 FMA + TRANS, x64 lanes),
+# from rocprofv3 --pmc SQ_INSTS_VALU_*_F64 on the shotnoise2p25 workload (profiles/)
+PROFILED_F64_OPS_PER_NODE = {"shotnoise2p25": 1186.0}
+
+
+def run_shotnoise(args, cfg):
+    """The shot-noise workload: a step is one sfbhip_sbessel_integrals call over
+    every kept (l, n) mode, w = 1 on [0, rmax] as a two-sample table, r^2 on the
+    device."""
+    import torch
+    from catana_amd import capi
+    h = capi.Handle(0)
+    L, NM, R = cfg["lmax"], cfg["nmax"], cfg["rmax"]
+    k, nl = h.kln(capi.make_params(L, NM, R, k_max=cfg["k_max"]))
+    ls = np.concatenate([np.full(nl[l], l, np.int32) for l in range(L)])
+    ks = np.concatenate([k[l, :nl[l]] for l in range(L)])
+    n = ls.size
+    tab = (np.ones(2), 0.0, R * 1.001)
+    nodes = capi.sbessel_nodes(h, ls, ks, ks, R).size
+    for _ in range(args.warmup):
+        capi.sbessel_integrals(h, ls, ks, ks, R, table=tab, r_power=2)
+    torch.cuda.synchronize()
+    kern = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        I = capi.sbessel_integrals(h, ls, ks, ks, R, table=tab, r_power=2)
+        kern.append(h.last_kernel_ms())
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    kms = float(np.mean(kern))
+    # closed form of the tophat case: int_0^R r^2 j_l(kR)^2 = R^3/2 [j_l^2 - j_{l-1} j_{l+1}](kR)
+    from scipy.special import spherical_jn as J
+    x = ks * R
+    jm = np.where(ls == 0, np.cos(x) / x, J(np.maximum(ls - 1, 0), x))
+    ref = R ** 3 / 2 * (J(ls, x) ** 2 - jm * J(ls + 1, x))
+    rel = np.abs(I / ref - 1)
+    ops = PROFILED_F64_OPS_PER_NODE.get("shotnoise2p25")
+    roof = {"bound": "fp64-valu", "achieved": None, "peak": PEAK_TFLOPS["fp64"], "unit": "TFLOP/s", "frac": None,
+            "traffic": None, "kernel": "sbessel_integral_kernel", "kernel_ms": kms, "nodes_per_launch": int(nodes)}
+    if ops:
+        a = ops * nodes / (kms * 1e-3) / 1e12
+        roof.update(achieved=a, frac=a / PEAK_TFLOPS["fp64"], f64_ops_per_node=ops)
+    out = {
+        "metric": "double spherical-Bessel integrals/s (shot-noise term of every kept mode)",
+        "value": n / (ms * 1e-3), "unit": "integrals/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic: k_ln of the configuration, w = tophat",
+        "config": {"workload": "shotnoise2p25", "lmax": L, "k_max": cfg["k_max"], "rmax": R, "integrals": int(n),
+                   "quadrature_nodes": int(nodes)},
+        "roofline": roof,
+        "accuracy": {"vs_closed_form_max_rel": float(rel.max()), "median_rel": float(np.median(rel))},
+    }
+    if not args.no_cpu_baseline:
+        sys.path.insert(0, ROOT)
+        from oracle import oracle as O
+        idx = np.arange(0, n, 8)
+        t0 = time.perf_counter()
+        for i in idx:
+            O.sbessel_integral(int(ls[i]), R, ks[i], ks[i], power=2)
+        dt = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": idx.size / dt, "unit": "integrals/s", "cores": 1, "kind": "port",
+                               "sample": f"{idx.size} of the {n} integrals (every 8th), oracle "
+                                         "restatement of double_sbessel_integrator, adaptive Gauss-Legendre"}
     h.close()
     print(json.dumps(out))
 

@@ -1477,11 +1477,17 @@ sfbhip_status sfbhip_sbessel_integrals(sfbhip_handle *h, int64_t n, const int32_
   CHK(get_buf(h, "ig_out", (size_t)n, &dout));
   int *derr;
   CHK(zero_err(h, st, &derr));
+  clear_timings(h);
+  HIPCHK(hipEventRecord(h->ev[0], st));
   sbessel_integral_kernel<<<(unsigned)cdiv(n, 4), 256, 0, st>>>(n, c.plan, c.zeros, c.nz, rmax, c.gl, W, dout, derr);
   HIPCHK(hipGetLastError());
+  HIPCHK(hipEventRecord(h->ev[1], st));
+  HIPCHK(hipEventRecord(h->ev[2], st));
+  HIPCHK(hipEventRecord(h->ev[3], st));
   HIPCHK(hipMemcpyAsync(out, dout, sizeof(double) * n, hipMemcpyDeviceToHost, st));
   int e = 0;
   CHK(read_err(h, st, derr, &e));
+  CHK(read_timings(h, 0));
   if (e & ERR_DOMAIN)
     return fail(SFBHIP_DOMAIN_ERROR, "argument outside range for which this FunctionInterpolator is defined");
   return SFBHIP_OK;

@@ -187,7 +187,8 @@ sfbhip_status sfbhip_sbessel_integrals(sfbhip_handle *h, int64_t n, const int32_
 
 /* Duration in ms of the dominant kernel of the last call on this handle
  * (the mixed contraction kernels when the call ran any, else the FP64
- * contraction kernels), measured with HIP events on the call's stream. */
+ * contraction kernels; for sfbhip_sbessel_integrals the integral kernel),
+ * measured with HIP events on the call's stream. */
 sfbhip_status sfbhip_last_kernel_ms(sfbhip_handle *h, float *ms);
 
 /* Durations in ms of the three phases of the last call, from HIP events on
