@@ -21,8 +21,10 @@ EXPORTED = [
     "sfbhip_default_params", "sfbhip_create", "sfbhip_destroy", "sfbhip_flmn_count", "sfbhip_kln",
     "sfbhip_direct", "sfbhip_direct_batched", "sfbhip_pixelize", "sfbhip_ang2pix_ring", "sfbhip_healpix", "sfbhip_last_kernel_ms",
     "sfbhip_last_phase_ms", "sfbhip_last_error", "sfbhip_bessel_zeros", "sfbhip_sbessel_nodes",
-    "sfbhip_sbessel_integrals",
+    "sfbhip_sbessel_integrals", "sfbhip_filter_points", "sfbhip_random_subset",
 ]
+RECORD_POINT, RECORD_CARTESIAN_F32, RECORD_CARTESIAN_F64, RECORD_SPHERICAL_F32, RECORD_SPHERICAL_F64 = range(5)
+FILTER_TOPHAT, FILTER_GAUSSIAN, FILTER_WINDOW_TABLE, FILTER_WINDOW_VALUES, FILTER_ANGULAR_MASK = range(5)
 WEIGHT_NODES, WEIGHT_TABLE = 0, 1
 
 
@@ -49,6 +51,13 @@ class Weight(ctypes.Structure):
     _fields_ = [
         ("kind", ctypes.c_int32), ("r_power", ctypes.c_int32), ("n", ctypes.c_int64), ("values", ctypes.c_void_p),
         ("x_min", ctypes.c_double), ("x_max", ctypes.c_double),
+    ]
+
+
+class Filter(ctypes.Structure):
+    _fields_ = [
+        ("kind", ctypes.c_int32), ("nside", ctypes.c_int32), ("a", ctypes.c_double), ("x_min", ctypes.c_double),
+        ("x_max", ctypes.c_double), ("n", ctypes.c_int64), ("d_values", ctypes.c_void_p),
     ]
 
 
@@ -84,6 +93,9 @@ def lib():
         L.sfbhip_bessel_zeros.argtypes = [i32, i32, i32, P]
         L.sfbhip_sbessel_nodes.argtypes = [P, i64, P, P, P, ctypes.c_double, ctypes.POINTER(i64), P]
         L.sfbhip_sbessel_integrals.argtypes = [P, i64, P, P, P, ctypes.c_double, ctypes.POINTER(Weight), P]
+        L.sfbhip_filter_points.argtypes = [P, P, i64, i32, ctypes.c_double, ctypes.POINTER(Filter), i32,
+                                           ctypes.c_uint64, ctypes.c_uint64, P, ctypes.POINTER(i64), P]
+        L.sfbhip_random_subset.argtypes = [P, P, i64, i64, ctypes.c_uint64, P, P]
         L.sfbhip_last_error.restype = ctypes.c_char_p
         L.sfbhip_last_error.argtypes = []
         _lib = L
@@ -312,3 +324,45 @@ def sbessel_integrals(handle, l, k1, k2, rmax, node_values=None, table=None, r_p
     check(lib().sfbhip_sbessel_integrals(handle.ptr, l.size, l.ctypes.data, k1.ctypes.data, k2.ctypes.data,
                                          float(rmax), ctypes.byref(w), out.ctypes.data), "sfbhip_sbessel_integrals")
     return out.reshape(shape)
+
+
+def _stream_ptr(stream, device=None):
+    import torch
+    if stream is None:
+        stream = torch.cuda.current_stream(device).cuda_stream
+    return ctypes.c_void_p(stream) if stream else None
+
+
+def filter_points_torch(handle, records, record_kind=RECORD_POINT, filters=(), seed=0, index_base=0, shift=0.0,
+                        stream=None):
+    """sfbhip_filter_points on a (N, 3) device tensor of records.  filters: sequence of
+    dicts {"kind": FILTER_*, "a", "x_min", "x_max", "values" (device tensor), "nside"}.
+    Returns the kept points, (M, 3) float64 on the device, in input order."""
+    import torch
+    n = records.shape[0]
+    out = torch.empty((max(n, 1), 3), dtype=torch.float64, device=records.device)
+    arr = (Filter * max(len(filters), 1))()
+    keep = []
+    for i, f in enumerate(filters):
+        v = f.get("values")
+        keep.append(v)
+        arr[i] = Filter(int(f["kind"]), int(f.get("nside", 0)), float(f.get("a", 0.0)), float(f.get("x_min", 0.0)),
+                        float(f.get("x_max", 0.0)), int(v.numel()) if v is not None else 0,
+                        v.data_ptr() if v is not None else None)
+    cnt = ctypes.c_int64()
+    check(lib().sfbhip_filter_points(handle.ptr, records.data_ptr() if n else None, n, int(record_kind),
+                                     float(shift), arr, len(filters), int(seed) & ((1 << 64) - 1),
+                                     int(index_base), out.data_ptr(), ctypes.byref(cnt), _stream_ptr(stream, records.device)),
+          "sfbhip_filter_points")
+    return out[:cnt.value]
+
+
+def random_subset_torch(handle, points, k, seed=0, stream=None):
+    """sfbhip_random_subset: k of the (N, 3) device points, in input order."""
+    import torch
+    n = points.shape[0]
+    k = min(int(k), n)
+    out = torch.empty((max(k, 1), 3), dtype=torch.float64, device=points.device)
+    check(lib().sfbhip_random_subset(handle.ptr, points.data_ptr(), n, k, int(seed) & ((1 << 64) - 1),
+                                     out.data_ptr(), _stream_ptr(stream, points.device)), "sfbhip_random_subset")
+    return out[:k]

@@ -4,10 +4,14 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
+#include <cstring>
 #include <memory>
 #include <mutex>
+#include <random>
 #include <stdexcept>
 
 #include "catana_hip.h"
@@ -595,6 +599,621 @@ KClkk sfb_decomposition_healpix(const PointContainer &points, unsigned int nside
                                 bool verbose) {
   return sfb_decomposition_healpix(PixelizedPointContainer(nside, points), lmax, k_max, rmax, nmax, window_volume,
                                    store_flmn, verbose);
+}
+
+}  // namespace catana
+
+// ================================================================ ingestion (io)
+namespace catana {
+
+namespace {
+
+std::atomic<uint64_t> g_seed{0};
+std::atomic<uint64_t> g_draws{0};  // records filtered so far: the index base of the next draws
+
+constexpr double kPi = 3.14159265358979323846;
+
+struct Unit {
+  double x, y, z;
+};
+Unit unit_vector(const Point &p) {
+  const double st = std::sin(p.p.theta);
+  return {st * std::cos(p.p.phi), st * std::sin(p.p.phi), std::cos(p.p.theta)};
+}
+
+FILE *fh(void *p) { return static_cast<FILE *>(p); }
+
+sfbhip_filter to_c(const io::FilterSpec &s) {
+  sfbhip_filter f;
+  f.kind = s.kind;
+  f.nside = s.nside;
+  f.a = s.a;
+  f.x_min = s.x_min;
+  f.x_max = s.x_max;
+  f.n = s.n;
+  f.d_values = s.d_values;
+  return f;
+}
+
+// Device buffers of the streaming filter (one set per thread).
+struct StreamBufs {
+  DevMem chunk, out, all, sub;
+  size_t all_count = 0;
+};
+StreamBufs &stream_bufs() {
+  static thread_local StreamBufs *b = new StreamBufs;
+  return *b;
+}
+
+// Filters n host points with the given filters on the device; appends the
+// survivors to bufs.all (device) and returns their count.
+size_t filter_chunk(Context &c, StreamBufs &b, const Point *pts, size_t n, const std::vector<io::Filter *> &filters) {
+  if (n == 0) return 0;
+  std::vector<sfbhip_filter> fs;
+  for (io::Filter *f : filters) {
+    f->prepare(pts, n);
+    fs.push_back(to_c(f->spec()));
+  }
+  b.chunk.reserve(n * sizeof(Point));
+  b.out.reserve(n * sizeof(Point));
+  hip_check(hipMemcpy(b.chunk.p, pts, n * sizeof(Point), hipMemcpyHostToDevice), "copy points");
+  int64_t kept = 0;
+  const uint64_t base = g_draws.fetch_add(n);
+  throw_status(sfbhip_filter_points(c.h, b.chunk.p, (int64_t)n, SFBHIP_RECORD_POINT, 0.0, fs.data(),
+                                    (int32_t)fs.size(), g_seed.load(), base, static_cast<double *>(b.out.p), &kept,
+                                    nullptr),
+               "filter");
+  if (kept > 0) {
+    const size_t need = (b.all_count + (size_t)kept) * sizeof(Point);
+    if (need > b.all.bytes) {  // grow, keeping what was collected
+      DevMem bigger;
+      bigger.reserve(std::max(need, 2 * b.all.bytes));
+      if (b.all_count)
+        hip_check(hipMemcpy(bigger.p, b.all.p, b.all_count * sizeof(Point), hipMemcpyDeviceToDevice), "grow");
+      std::swap(b.all.p, bigger.p);
+      std::swap(b.all.bytes, bigger.bytes);
+    }
+    hip_check(hipMemcpy(static_cast<char *>(b.all.p) + b.all_count * sizeof(Point), b.out.p,
+                        (size_t)kept * sizeof(Point), hipMemcpyDeviceToDevice),
+              "append");
+    b.all_count += (size_t)kept;
+  }
+  return (size_t)kept;
+}
+
+}  // namespace
+
+void init_random(unsigned int seed) {
+  g_seed = seed;
+  g_draws = 0;
+}
+
+void init_random() { init_random(std::random_device{}()); }
+
+namespace io {
+
+// ---- sources ----
+
+PointContainer Source::get_point_container() {
+  reset();
+  PointContainer pc;
+  std::vector<Point> buf(100000);
+  long long k;
+  while ((k = read(buf.data(), buf.size())) != -1) pc.insert(pc.end(), buf.begin(), buf.begin() + k);
+  return pc;
+}
+
+PixelizedPointContainer Source::get_pixelized_point_container(unsigned int nside) {
+  return PixelizedPointContainer(nside, get_point_container());
+}
+
+long long PointContainerSource::read(Point *out, size_t n) {
+  if (pos_ >= pc_.size()) return -1;
+  const size_t k = std::min(n, pc_.size() - pos_);
+  std::copy(pc_.begin() + pos_, pc_.begin() + pos_ + k, out);
+  pos_ += k;
+  return (long long)k;
+}
+
+TextSource::TextSource(std::string filename, TextFormat format, double shift)
+    : filename_(std::move(filename)), format_(format), shift_(shift) {
+  fp_ = std::fopen(filename_.c_str(), "r");
+  if (!fp_) std::printf("WARNING: Could not open file %s\n", filename_.c_str());
+}
+
+TextSource::~TextSource() {
+  if (fp_) std::fclose(fh(fp_));
+}
+
+long long TextSource::read(Point *out, size_t n) {
+  if (!fp_) return -1;
+  size_t i = 0;
+  std::string line;
+  char buf[4096];
+  while (i < n) {
+    line.clear();
+    bool got = false;
+    while (std::fgets(buf, sizeof(buf), fh(fp_))) {
+      got = true;
+      line += buf;
+      if (!line.empty() && line.back() == '\n') break;
+    }
+    if (!got) break;
+    // three numbers; a missing one reads as 0 (std::istream >> on failure)
+    double a[3] = {0.0, 0.0, 0.0};
+    const char *s = line.c_str();
+    for (int k = 0; k < 3; ++k) {
+      char *end;
+      const double v = std::strtod(s, &end);
+      if (end == s) break;
+      a[k] = v;
+      s = end;
+    }
+    if (format_ == TextFormat::CARTESIAN)
+      out[i] = point_from_box_position(a[0], a[1], a[2], shift_);
+    else if (format_ == TextFormat::SPHERICAL)
+      out[i] = point_from_spherical_position(a[0], a[1], a[2]);
+    else
+      out[i] = point_from_spherical_position(a[2], a[1], a[0]);  // phi theta r
+    ++i;
+  }
+  return i == 0 ? -1 : (long long)i;
+}
+
+void TextSource::reset() {
+  if (fp_) std::rewind(fh(fp_));
+}
+
+namespace {
+size_t record_bytes(RecordType t) {
+  return (t == RecordType::CARTESIAN_F32 || t == RecordType::SPHERICAL_F32) ? 12 : 24;
+}
+Point record_to_point(const unsigned char *rec, RecordType t) {
+  double a[3];
+  if (record_bytes(t) == 12) {
+    float f[3];
+    std::memcpy(f, rec, 12);
+    a[0] = f[0];
+    a[1] = f[1];
+    a[2] = f[2];
+  } else {
+    std::memcpy(a, rec, 24);
+  }
+  if (t == RecordType::CARTESIAN_F32 || t == RecordType::CARTESIAN_F64) return point_from_box_position(a[0], a[1], a[2], 0.0);
+  return point_from_spherical_position(a[0], a[1], a[2]);
+}
+void point_to_record(const Point &p, RecordType t, unsigned char *rec) {
+  double a[3];
+  if (t == RecordType::CARTESIAN_F32 || t == RecordType::CARTESIAN_F64) {
+    const Unit u = unit_vector(p);
+    a[0] = p.r * u.x;
+    a[1] = p.r * u.y;
+    a[2] = p.r * u.z;
+  } else {
+    a[0] = p.r;
+    a[1] = p.p.theta;
+    a[2] = p.p.phi;
+  }
+  if (record_bytes(t) == 12) {
+    const float f[3] = {(float)a[0], (float)a[1], (float)a[2]};
+    std::memcpy(rec, f, 12);
+  } else {
+    std::memcpy(rec, a, 24);
+  }
+}
+}  // namespace
+
+RawBinarySource::RawBinarySource(std::string filename, bool verbose, RecordType type)
+    : filename_(std::move(filename)), type_(type) {
+  fp_ = std::fopen(filename_.c_str(), "rb");
+  if (!fp_) {
+    std::printf("WARNING: Could not open file %s\n", filename_.c_str());
+    return;
+  }
+  if (verbose) std::printf("Opened file %s\n", filename_.c_str());
+  std::fseek(fh(fp_), 0, SEEK_END);
+  ntot_ = (size_t)std::ftell(fh(fp_)) / record_bytes(type_);
+  std::fseek(fh(fp_), 0, SEEK_SET);
+}
+
+RawBinarySource::~RawBinarySource() {
+  if (fp_) std::fclose(fh(fp_));
+}
+
+long long RawBinarySource::read(Point *out, size_t n) {
+  if (!fp_) return -1;
+  const size_t k = std::min(n, ntot_ - pos_);
+  if (k == 0) return -1;
+  const size_t rb = record_bytes(type_);
+  std::vector<unsigned char> buf(k * rb);
+  const size_t got = std::fread(buf.data(), rb, k, fh(fp_));
+  for (size_t i = 0; i < got; ++i) out[i] = record_to_point(buf.data() + i * rb, type_);
+  pos_ += got;
+  return got == 0 ? -1 : (long long)got;
+}
+
+void RawBinarySource::reset() {
+  pos_ = 0;
+  if (fp_) std::fseek(fh(fp_), 0, SEEK_SET);
+}
+
+// Gadget format 1 (the header block of read_snapshot.c, as GadgetSource.cpp:10-24 reads it)
+namespace {
+#pragma pack(push, 1)
+struct GadgetHeader {
+  uint32_t npart[6];
+  double mass[6];
+  double time, redshift;
+  int32_t flag_sfr, flag_feedback;
+  int32_t npartTotal[6];
+  int32_t flag_cooling, num_files;
+  double BoxSize, Omega0, OmegaLambda, HubbleParam;
+  char fill[256 - 6 * 4 - 6 * 8 - 2 * 8 - 2 * 4 - 6 * 4 - 2 * 4 - 4 * 8];
+};
+#pragma pack(pop)
+static_assert(sizeof(GadgetHeader) == 256, "Gadget header is 256 bytes");
+}  // namespace
+
+GadgetSource::GadgetSource(std::string filename, bool verbose) : base_(filename), verbose_(verbose) {
+  load_file(filename, true);
+  if (files_ > 1) {
+    base_ = filename.substr(0, filename.find_last_of('.'));
+    if (verbose_) std::printf("Detected %u files.\n", files_);
+    if (current_file_ != 0) load_file(base_ + ".0", false);
+  }
+}
+
+GadgetSource::~GadgetSource() {
+  if (fp_) std::fclose(fh(fp_));
+}
+
+void GadgetSource::load_file(const std::string &fname, bool initial) {
+  if (fp_) std::fclose(fh(fp_));
+  fp_ = std::fopen(fname.c_str(), "rb");
+  if (!fp_) {
+    std::printf("WARNING: Could not open file %s\n", fname.c_str());
+    n_points_file_ = 0;
+    return;
+  }
+  if (verbose_) std::printf("Opened file %s\n", fname.c_str());
+  GadgetHeader hd;
+  std::fseek(fh(fp_), 4, SEEK_SET);  // block marker
+  if (std::fread(&hd, sizeof(hd), 1, fh(fp_)) != 1) throw std::runtime_error("GadgetSource: short header in " + fname);
+  // skip the header's end marker, the position block's start marker and the gas particles
+  std::fseek(fh(fp_), 4 + (long)hd.npart[0] * 12 + 4, SEEK_CUR);
+  current_point_file_ = 0;
+  n_points_file_ = hd.npart[1];
+  const size_t dot = fname.find_last_of('.');
+  current_file_ = dot == std::string::npos ? 0 : (unsigned)std::strtoul(fname.c_str() + dot + 1, nullptr, 10);
+  if (initial) {
+    files_ = (unsigned)std::max(1, hd.num_files);
+    n_points_ = (size_t)hd.npartTotal[1];
+    hubble_ = hd.HubbleParam;
+    box_size_ = hd.BoxSize;
+  }
+}
+
+long long GadgetSource::read(Point *out, size_t n) {
+  const size_t to_load = std::min(n, n_points_ - current_point_);
+  if (to_load == 0 || !fp_) return -1;
+  const double shift = -box_size_ / 2.0;
+  size_t loaded = 0;
+  while (loaded < to_load) {
+    float rec[3];
+    if (std::fread(rec, sizeof(rec), 1, fh(fp_)) != 1) break;
+    out[loaded++] = point_from_box_position(rec[0], rec[1], rec[2], shift);
+    ++current_point_;
+    ++current_point_file_;
+    if (current_point_file_ == n_points_file_) {
+      if (current_file_ + 1 >= files_) {
+        if (loaded < to_load)
+          std::printf("WARNING: Did not find as many points as expected (%zu instead of %zu).\n", current_point_,
+                      n_points_);
+        break;
+      }
+      load_file(base_ + "." + std::to_string(current_file_ + 1), false);
+    }
+  }
+  return loaded == 0 ? -1 : (long long)loaded;
+}
+
+void GadgetSource::reset() {
+  load_file(files_ > 1 ? base_ + ".0" : base_, false);
+  current_point_ = 0;
+}
+
+// ---- sinks ----
+
+long long PointContainerSink::write(const Point *in, size_t n) {
+  pc_.insert(pc_.end(), in, in + n);
+  return (long long)n;
+}
+
+long long PixelizedPointContainerSink::write(const Point *in, size_t n) {
+  for (size_t i = 0; i < n; ++i) pix_.add_point(in[i]);
+  return (long long)n;
+}
+
+TextSink::TextSink(std::string filename, TextFormat format, bool verbose) : format_(format) {
+  fp_ = std::fopen(filename.c_str(), "w");
+  if (!fp_)
+    std::printf("WARNING: Could not create file %s\n", filename.c_str());
+  else if (verbose)
+    std::printf("Created file %s\n", filename.c_str());
+}
+
+TextSink::~TextSink() { close(); }
+
+void TextSink::close() {
+  if (fp_) std::fclose(fh(fp_));
+  fp_ = nullptr;
+}
+
+long long TextSink::write(const Point *in, size_t n) {
+  if (!fp_) return -1;
+  for (size_t i = 0; i < n; ++i) {
+    const Point &p = in[i];
+    if (format_ == TextFormat::CARTESIAN) {
+      const Unit u = unit_vector(p);
+      std::fprintf(fh(fp_), "%.16e %.16e %.16e\n", p.r * u.x, p.r * u.y, p.r * u.z);
+    } else if (format_ == TextFormat::SPHERICAL) {
+      std::fprintf(fh(fp_), "%.16e %.16e %.16e\n", p.r, p.p.theta, p.p.phi);
+    } else {
+      std::fprintf(fh(fp_), " %18.14F %18.14F %18.12F\n", p.p.phi, p.p.theta, p.r);
+    }
+  }
+  std::fflush(fh(fp_));
+  return (long long)n;
+}
+
+RawBinarySink::RawBinarySink(std::string filename, bool verbose, bool append, RecordType type) : type_(type) {
+  fp_ = std::fopen(filename.c_str(), append ? "ab" : "wb");
+  if (!fp_)
+    std::printf("WARNING: Could not create file %s\n", filename.c_str());
+  else if (verbose)
+    std::printf("Opened RawBinarySink %s in %s-mode\n", filename.c_str(), append ? "append" : "overwrite");
+}
+
+RawBinarySink::~RawBinarySink() { close(); }
+
+void RawBinarySink::close() {
+  if (fp_) std::fclose(fh(fp_));
+  fp_ = nullptr;
+}
+
+long long RawBinarySink::write(const Point *in, size_t n) {
+  if (!fp_) return -1;
+  const size_t rb = record_bytes(type_);
+  std::vector<unsigned char> buf(n * rb);
+  for (size_t i = 0; i < n; ++i) point_to_record(in[i], type_, buf.data() + i * rb);
+  std::fwrite(buf.data(), rb, n, fh(fp_));
+  std::fflush(fh(fp_));
+  return (long long)n;
+}
+
+// ---- filters ----
+
+const void *Filter::upload(const void *host, size_t bytes) const {
+  if (bytes == 0) return nullptr;
+  if (bytes > dev_bytes_) {
+    void *p = nullptr;
+    hip_check(hipMalloc(&p, bytes), "hipMalloc (filter)");
+    dev_ = std::shared_ptr<void>(p, [](void *q) { (void)hipFree(q); });
+    dev_bytes_ = bytes;
+  }
+  hip_check(hipMemcpy(dev_.get(), host, bytes, hipMemcpyHostToDevice), "copy filter data");
+  return dev_.get();
+}
+
+size_t Filter::operator()(PointContainer &pc, bool resize) {
+  Context &c = ctx();
+  std::lock_guard<std::mutex> lk(c.mu);
+  StreamBufs &b = stream_bufs();
+  b.all_count = 0;
+  const size_t kept = filter_chunk(c, b, pc.data(), pc.size(), {this});
+  if (kept) hip_check(hipMemcpy(pc.data(), b.all.p, kept * sizeof(Point), hipMemcpyDeviceToHost), "copy points");
+  if (resize) pc.resize(kept);
+  return kept;
+}
+
+FilterSpec TophatRadialWindowFunctionFilter::spec() const {
+  FilterSpec s;
+  s.kind = SFBHIP_FILTER_TOPHAT;
+  s.a = R0_;
+  return s;
+}
+
+GenericRadialWindowFunctionFilter::GenericRadialWindowFunctionFilter(std::function<double(double)> window_function)
+    : w_(std::move(window_function)) {}
+
+GenericRadialWindowFunctionFilter::GenericRadialWindowFunctionFilter(std::function<double(double)> window_function,
+                                                                     size_t interpolation_points, double min,
+                                                                     double max, bool /*parallel_init*/)
+    : x_min_(min), x_max_(max) {
+  FunctionInterpolator fi(std::move(window_function), (unsigned)interpolation_points, min, max);
+  table_ = fi.samples();
+}
+
+void GenericRadialWindowFunctionFilter::prepare(const Point *pts, size_t n) {
+  if (!table_.empty()) return;
+  values_.resize(n);
+  for (size_t i = 0; i < n; ++i) values_[i] = w_(pts[i].r);
+}
+
+FilterSpec GenericRadialWindowFunctionFilter::spec() const {
+  FilterSpec s;
+  if (!table_.empty()) {
+    s.kind = SFBHIP_FILTER_WINDOW_TABLE;
+    s.x_min = x_min_;
+    s.x_max = x_max_;
+    s.n = (int64_t)table_.size();
+    s.d_values = upload(table_.data(), table_.size() * sizeof(double));
+  } else {
+    s.kind = SFBHIP_FILTER_WINDOW_VALUES;
+    s.n = (int64_t)values_.size();
+    s.d_values = upload(values_.data(), values_.size() * sizeof(double));
+  }
+  return s;
+}
+
+FilterSpec GaussianRadialWindowFunctionFilter::spec() const {
+  FilterSpec s;
+  s.kind = SFBHIP_FILTER_GAUSSIAN;
+  s.a = R0_;
+  return s;
+}
+
+AngularMaskFilter::AngularMaskFilter(const std::vector<float> &healpix_map) : map_(healpix_map) {
+  const unsigned nside = (unsigned)std::lround(std::sqrt(map_.size() / 12.0));
+  if ((size_t)12 * nside * nside != map_.size() || !is_power_of_2(nside))
+    throw std::runtime_error("healpix_map is not a valid map (check size!)");
+  nside_ = nside;
+}
+
+AngularMaskFilter::AngularMaskFilter(const std::string &healpix_mask_fits) {
+  throw std::runtime_error("AngularMaskFilter(" + healpix_mask_fits +
+                           "): FITS maps are not read by this build (no cfitsio); pass the RING map as an array");
+}
+
+FilterSpec AngularMaskFilter::spec() const {
+  FilterSpec s;
+  s.kind = SFBHIP_FILTER_ANGULAR_MASK;
+  s.nside = (int)nside_;
+  s.n = (int64_t)map_.size();
+  s.d_values = upload(map_.data(), map_.size() * sizeof(float));
+  return s;
+}
+
+// ---- FilterStream ----
+
+FilterStream::FilterStream(Source *source, Sink *sink, size_t buffer_size, bool verbose)
+    : source_(source), sink_(sink), buffer_size_(std::max<size_t>(buffer_size, 1)), verbose_(verbose) {}
+
+size_t FilterStream::run_device(size_t subsample_size, const double **d_points) {
+  if (!source_) throw std::invalid_argument("FilterStream: no source");
+  Context &c = ctx();
+  std::lock_guard<std::mutex> lk(c.mu);
+  StreamBufs &b = stream_bufs();
+  b.all_count = 0;
+  std::vector<Point> buf(buffer_size_);
+  long long k;
+  size_t n_read = 0;
+  while ((k = source_->read(buf.data(), buf.size())) != -1) {
+    n_read += (size_t)k;
+    const size_t kept = filter_chunk(c, b, buf.data(), (size_t)k, filters_);
+    if (verbose_)
+      std::printf("\tPoints read: %10lld (%10zu)\twritten: %10zu ( %10zu)\n", k, n_read, kept, b.all_count);
+  }
+  size_t n = b.all_count;
+  const double *out = static_cast<const double *>(b.all.p);
+  if (subsample_size != 0 && subsample_size < n) {
+    b.sub.reserve(subsample_size * sizeof(Point));
+    const uint64_t seed = g_seed.load() ^ (0xd1b54a32d192ed03ULL * (g_draws.fetch_add(1) + 1));
+    throw_status(sfbhip_random_subset(c.h, out, (int64_t)n, (int64_t)subsample_size, seed,
+                                      static_cast<double *>(b.sub.p), nullptr),
+                 "random subset");
+    n = subsample_size;
+    out = static_cast<const double *>(b.sub.p);
+  }
+  if (d_points) *d_points = out;
+  return n;
+}
+
+size_t FilterStream::run(size_t subsample_size, std::string /*temp_filename*/) {
+  const double *d = nullptr;
+  const size_t n = run_device(subsample_size, &d);
+  if (!sink_) return n;
+  std::vector<Point> host(n);
+  if (n) hip_check(hipMemcpy(host.data(), d, n * sizeof(Point), hipMemcpyDeviceToHost), "copy points");
+  const size_t chunk = buffer_size_;
+  for (size_t i = 0; i < n; i += chunk) sink_->write(host.data() + i, std::min(chunk, n - i));
+  return n;
+}
+
+size_t FilterStream::run_totemp(std::string temp_filename, bool append) {
+  const double *d = nullptr;
+  const size_t n = run_device(0, &d);
+  std::vector<Point> host(n);
+  if (n) hip_check(hipMemcpy(host.data(), d, n * sizeof(Point), hipMemcpyDeviceToHost), "copy points");
+  RawBinarySink tmp(temp_filename, verbose_, append, RecordType::SPHERICAL_F64);
+  tmp.write(host.data(), n);
+  return n;
+}
+
+size_t FilterStream::run_fromtemp(std::string temp_filename, size_t subsample_size, bool remove_temp) {
+  RawBinarySource tmp(temp_filename, verbose_, RecordType::SPHERICAL_F64);
+  if (verbose_) std::printf("Temporary file %s contains %zu points.\n", temp_filename.c_str(), tmp.get_npoints());
+  Source *keep_source = source_;
+  std::vector<Filter *> keep_filters;
+  keep_filters.swap(filters_);
+  source_ = &tmp;
+  size_t n = 0;
+  try {
+    n = run(subsample_size);
+  } catch (...) {
+    source_ = keep_source;
+    filters_.swap(keep_filters);
+    throw;
+  }
+  source_ = keep_source;
+  filters_.swap(keep_filters);
+  if (remove_temp) std::remove(temp_filename.c_str());
+  return n;
+}
+
+}  // namespace io
+
+// ---- Analyzer ----
+
+namespace {
+size_t analyzer_points(io::Source *source, const std::vector<io::Filter *> &filters, double rmax, unsigned subsample,
+                       bool verbose, const double **d) {
+  if (!source) throw std::invalid_argument("Analyzer: no source");
+  source->reset();
+  io::FilterStream fs(source, nullptr, 1000000, verbose);
+  io::TophatRadialWindowFunctionFilter tophat(rmax);
+  fs.add_filter(&tophat);
+  for (io::Filter *f : filters) fs.add_filter(f);
+  return fs.run_device(subsample, d);
+}
+}  // namespace
+
+KClkk Analyzer::compute_sfb(unsigned short lmax, unsigned short nmax, double rmax, bool store_flmn, bool verbose) {
+  const double *d = nullptr;
+  const size_t n = analyzer_points(source_, filters_, rmax, subsample_, verbose, &d);
+  Context &c = ctx();
+  std::lock_guard<std::mutex> lk(c.mu);
+  reserve_outputs(c, lmax, nmax, store_flmn);
+  Options o;  // sfb_decomposition: FP64, the reference's table when N > 1000
+  sfbhip_params p = make_params(lmax, nmax, rmax, verbose, o);
+  throw_status(sfbhip_direct(c.h, d, (int64_t)n, &p, static_cast<double *>(c.kln.p), static_cast<double *>(c.cln.p),
+                             store_flmn ? static_cast<double *>(c.flmn.p) : nullptr, nullptr),
+               "Analyzer::compute_sfb");
+  return download(c, lmax, nmax, store_flmn);
+}
+
+KClkk Analyzer::compute_sfb_pixelized(unsigned short lmax, unsigned short nmax, double rmax, unsigned int nside,
+                                      bool store_flmn, bool verbose) {
+  if (!is_power_of_2(nside)) throw std::invalid_argument("nside must be a power of 2");
+  const double *d = nullptr;
+  const size_t n = analyzer_points(source_, filters_, rmax, subsample_, verbose, &d);
+  Context &c = ctx();
+  std::lock_guard<std::mutex> lk(c.mu);
+  const size_t npix = (size_t)12 * nside * nside;
+  c.off.reserve((npix + 1) * sizeof(int64_t));
+  c.rad.reserve(std::max<size_t>(n * sizeof(double), 8));
+  throw_status(sfbhip_pixelize(c.h, d, (int64_t)n, (int32_t)nside, static_cast<int64_t *>(c.off.p),
+                               static_cast<double *>(c.rad.p), nullptr),
+               "Analyzer::compute_sfb_pixelized");
+  reserve_outputs(c, lmax, nmax, store_flmn);
+  Options o;
+  sfbhip_params p = make_params(lmax, nmax, rmax, verbose, o);
+  p.precision = SFBHIP_PREC_FP64;
+  throw_status(sfbhip_healpix(c.h, static_cast<const int64_t *>(c.off.p), static_cast<const double *>(c.rad.p),
+                              (int32_t)nside, &p, static_cast<double *>(c.kln.p), static_cast<double *>(c.cln.p),
+                              store_flmn ? static_cast<double *>(c.flmn.p) : nullptr, nullptr),
+               "Analyzer::compute_sfb_pixelized");
+  return download(c, lmax, nmax, store_flmn);
 }
 
 }  // namespace catana

@@ -10,6 +10,7 @@
 #include <cstddef>
 #include <cstdint>
 #include <functional>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -213,5 +214,276 @@ Array2D shot_noise(std::function<double(double)> w, unsigned short lmax, double 
                    unsigned short nmax, double n_points, double window_volume = 0.0);
 Array2D shot_noise(const FunctionInterpolator &w, unsigned short lmax, double k_max, double rmax,
                    unsigned short nmax, double n_points, double window_volume = 0.0);
+
+// Seed of the random point filters and subsets (tools/random.cpp init_random):
+// every draw is a hash of (seed, filter, running record count), so a seeded
+// sequence of runs is reproducible.  Without a call the seed is 0.
+void init_random(unsigned int seed);
+void init_random();  // a seed from std::random_device
+
+namespace io {
+
+// ---- sources (io/Source.hpp and io/sources/*): read() fills up to n points
+// and returns how many, or -1 when nothing is left ----
+class Source {
+ public:
+  virtual ~Source() = default;
+  virtual long long read(Point *out, size_t n) = 0;
+  virtual void reset() = 0;
+  virtual size_t get_npoints() { return 0; }
+  PointContainer get_point_container();
+  PixelizedPointContainer get_pixelized_point_container(unsigned int nside);
+};
+
+class PointContainerSource : public Source {
+ public:
+  explicit PointContainerSource(const PointContainer &pc) : pc_(pc) {}
+  long long read(Point *out, size_t n) override;
+  void reset() override { pos_ = 0; }
+  size_t get_npoints() override { return pc_.size(); }
+
+ private:
+  const PointContainer &pc_;
+  size_t pos_ = 0;
+};
+
+// Text records, one per line (TextSource.cpp:10-60): "x y z" (+ shift on
+// every axis), "r theta phi", or the 3dex order "phi theta r".
+enum class TextFormat { CARTESIAN, SPHERICAL, SPHERICAL_3DEX };
+class TextSource : public Source {
+ public:
+  TextSource(std::string filename, TextFormat format, double shift = 0.0);
+  ~TextSource() override;
+  long long read(Point *out, size_t n) override;
+  void reset() override;
+
+ private:
+  std::string filename_;
+  TextFormat format_;
+  double shift_;
+  void *fp_ = nullptr;
+};
+
+// Raw binary records (RawBinarySource.cpp:9-90): 3 floats or doubles each,
+// cartesian (x, y, z) or spherical (r, theta, phi).
+enum class RecordType { CARTESIAN_F32, CARTESIAN_F64, SPHERICAL_F32, SPHERICAL_F64 };
+class RawBinarySource : public Source {
+ public:
+  RawBinarySource(std::string filename, bool verbose = false, RecordType type = RecordType::SPHERICAL_F32);
+  ~RawBinarySource() override;
+  long long read(Point *out, size_t n) override;
+  void reset() override;
+  size_t get_npoints() override { return ntot_; }
+
+ private:
+  std::string filename_;
+  RecordType type_;
+  void *fp_ = nullptr;
+  size_t ntot_ = 0, pos_ = 0;
+};
+
+// Gadget (format 1) snapshots, one file or name.0 .. name.(N-1): the halo
+// particles' float positions shifted by -BoxSize/2 (GadgetSource.cpp:38-177).
+class GadgetSource : public Source {
+ public:
+  GadgetSource(std::string filename, bool verbose = false);
+  ~GadgetSource() override;
+  long long read(Point *out, size_t n) override;
+  void reset() override;
+  size_t get_npoints() override { return n_points_; }
+  double box_size() const { return box_size_; }
+  double hubble_parameter() const { return hubble_; }
+
+ private:
+  void load_file(const std::string &fname, bool initial);
+  std::string base_;
+  bool verbose_;
+  void *fp_ = nullptr;
+  unsigned files_ = 1, current_file_ = 0;
+  size_t n_points_ = 0, current_point_ = 0, n_points_file_ = 0, current_point_file_ = 0;
+  double box_size_ = 0.0, hubble_ = 0.0;
+};
+
+// ---- sinks (io/Sink.hpp and io/sinks/*): write() returns the count written, or -1 when closed ----
+class Sink {
+ public:
+  virtual ~Sink() = default;
+  virtual long long write(const Point *in, size_t n) = 0;
+  long long write(const PointContainer &pc) { return write(pc.data(), pc.size()); }
+};
+
+class PointContainerSink : public Sink {
+ public:
+  explicit PointContainerSink(PointContainer &pc) : pc_(pc) {}
+  long long write(const Point *in, size_t n) override;
+
+ private:
+  PointContainer &pc_;
+};
+
+class PixelizedPointContainerSink : public Sink {
+ public:
+  explicit PixelizedPointContainerSink(PixelizedPointContainer &pix) : pix_(pix) {}
+  long long write(const Point *in, size_t n) override;
+
+ private:
+  PixelizedPointContainer &pix_;
+};
+
+// TextSink.cpp: "%.18e"-style columns in the source's order (the 3dex order
+// with the reference's " %18.14F %18.14F %18.12F" format, record_type.cpp:82-86).
+class TextSink : public Sink {
+ public:
+  TextSink(std::string filename, TextFormat format, bool verbose = false);
+  ~TextSink() override;
+  long long write(const Point *in, size_t n) override;
+  void close();
+
+ private:
+  TextFormat format_;
+  void *fp_ = nullptr;
+};
+
+class RawBinarySink : public Sink {
+ public:
+  RawBinarySink(std::string filename, bool verbose = false, bool append = false,
+                RecordType type = RecordType::SPHERICAL_F32);
+  ~RawBinarySink() override;
+  long long write(const Point *in, size_t n) override;
+  void close();
+
+ private:
+  RecordType type_;
+  void *fp_ = nullptr;
+};
+
+// ---- filters (io/Filter.hpp, io/filters/*), evaluated on the device ----
+// The device description of a filter: the fields of sfbhip_filter (kind as
+// SFBHIP_FILTER_*, d_values in device memory the filter owns).
+struct FilterSpec {
+  int kind = 0, nside = 0;
+  double a = 0.0, x_min = 0.0, x_max = 0.0;
+  int64_t n = 0;
+  const void *d_values = nullptr;
+};
+
+class Filter {
+ public:
+  virtual ~Filter() = default;
+  // Filters the container in place on the device; the survivors keep their
+  // input order (the reference swaps rejected points to the end).  Returns the
+  // new size (Filter.cpp:7-23).
+  size_t operator()(PointContainer &pc, bool resize = true);
+  // host evaluation needed before the device pass (callables), per chunk
+  virtual void prepare(const Point *pts, size_t n) {}
+  virtual FilterSpec spec() const = 0;
+
+ protected:
+  // uploads host data into the filter's device buffer (grown as needed)
+  const void *upload(const void *host, size_t bytes) const;
+
+ private:
+  mutable std::shared_ptr<void> dev_;
+  mutable size_t dev_bytes_ = 0;
+};
+
+class TophatRadialWindowFunctionFilter : public Filter {
+ public:
+  explicit TophatRadialWindowFunctionFilter(double R0) : R0_(R0) {}
+  FilterSpec spec() const override;
+
+ private:
+  double R0_;
+};
+
+class GenericRadialWindowFunctionFilter : public Filter {
+ public:
+  // acceptance probability w(r), evaluated on the host at every point
+  explicit GenericRadialWindowFunctionFilter(std::function<double(double)> window_function);
+  // w sampled at interpolation_points on [min, max] and interpolated on the device
+  GenericRadialWindowFunctionFilter(std::function<double(double)> window_function, size_t interpolation_points,
+                                    double min, double max, bool parallel_init = false);
+  void prepare(const Point *pts, size_t n) override;
+  FilterSpec spec() const override;
+
+ protected:
+  GenericRadialWindowFunctionFilter() = default;
+  std::function<double(double)> w_;
+  std::vector<double> table_, values_;
+  double x_min_ = 0.0, x_max_ = 0.0;
+};
+
+class GaussianRadialWindowFunctionFilter : public GenericRadialWindowFunctionFilter {
+ public:
+  explicit GaussianRadialWindowFunctionFilter(double R0) : R0_(R0) {}
+  void prepare(const Point *, size_t) override {}
+  FilterSpec spec() const override;
+
+ private:
+  double R0_;
+};
+
+class AngularMaskFilter : public Filter {
+ public:
+  // HEALPix RING float map, 12 nside^2 pixels: 1 keeps a point, 0 removes it
+  explicit AngularMaskFilter(const std::vector<float> &healpix_map);
+  // FITS maps need cfitsio, which this build does not link: throws std::runtime_error
+  explicit AngularMaskFilter(const std::string &healpix_mask_fits);
+  FilterSpec spec() const override;
+
+ private:
+  std::vector<float> map_;
+  unsigned nside_ = 0;
+};
+
+// Reads the source in chunks of buffer_size points, filters each chunk on the
+// device and collects the survivors in device memory; a subsample of the
+// survivors is drawn on the device; the result goes to the sink (host) or
+// stays on the device (run_device).  The reference streams through a
+// temporary file for the subsample (FilterStream.cpp:57-101); HBM holds the
+// filtered catalogue instead.
+class FilterStream {
+ public:
+  FilterStream(Source *source, Sink *sink, size_t buffer_size = 1000000, bool verbose = false);
+  void set_source(Source *source) { source_ = source; }
+  void add_filter(Filter *filter) { filters_.push_back(filter); }
+  size_t run(size_t subsample_size = 0, std::string temp_filename = "");
+  // the reference's two halves of a subsampled run (FilterStream.cpp:57-91):
+  // filter into a raw file of (r, theta, phi) doubles, then read it back
+  // through the random subset into the sink
+  size_t run_totemp(std::string temp_filename, bool append);
+  size_t run_fromtemp(std::string temp_filename, size_t subsample_size, bool remove_temp);
+  // the filtered (and subsampled) points stay on the device: returns their
+  // count; *d_points is valid until the next run on this thread
+  size_t run_device(size_t subsample_size, const double **d_points);
+
+ private:
+  Source *source_;
+  Sink *sink_;
+  size_t buffer_size_;
+  bool verbose_;
+  std::vector<Filter *> filters_;
+};
+
+}  // namespace io
+
+// Source + filters (+ tophat at rmax) + subsample + decomposition on the
+// device without a host round trip of the filtered catalogue (Analyzer.cpp:10-62).
+class Analyzer {
+ public:
+  Analyzer() = default;
+  explicit Analyzer(io::Source *source) : source_(source) {}
+  void set_source(io::Source *source) { source_ = source; }
+  void set_subsample_size(unsigned int n) { subsample_ = n; }
+  void add_filter(io::Filter *filter) { filters_.push_back(filter); }
+  KClkk compute_sfb(unsigned short lmax, unsigned short nmax, double rmax, bool store_flmn, bool verbose);
+  KClkk compute_sfb_pixelized(unsigned short lmax, unsigned short nmax, double rmax, unsigned int nside,
+                              bool store_flmn, bool verbose);
+
+ private:
+  io::Source *source_ = nullptr;
+  std::vector<io::Filter *> filters_;
+  unsigned int subsample_ = 0;
+};
 
 }  // namespace catana

@@ -4,6 +4,7 @@
 // the k_max / window-volume entry points sfb_decomposition_reverse and
 // sfb_decomposition_healpix.  The GIL is released around device work.
 #include <pybind11/complex.h>
+#include <pybind11/functional.h>
 #include <pybind11/numpy.h>
 #include <pybind11/pybind11.h>
 #include <pybind11/stl.h>
@@ -106,6 +107,26 @@ std::vector<double> integrals_py(const py::function &w, const std::vector<int32_
   py::gil_scoped_release nogil;
   return besseltools::sbessel_integrals_at_nodes(l, k1, k2, rmax, wv);
 }
+
+// the reference's Python-visible record classes (python/py_io.cpp:104-250)
+struct CartesianTextSource : io::TextSource {
+  CartesianTextSource(const std::string &f, double shift) : io::TextSource(f, io::TextFormat::CARTESIAN, shift) {}
+};
+struct SphericalTextSource : io::TextSource {
+  explicit SphericalTextSource(const std::string &f) : io::TextSource(f, io::TextFormat::SPHERICAL) {}
+};
+struct SphericalTextSource3dex : io::TextSource {
+  explicit SphericalTextSource3dex(const std::string &f) : io::TextSource(f, io::TextFormat::SPHERICAL_3DEX) {}
+};
+struct CartesianTextSink : io::TextSink {
+  CartesianTextSink(const std::string &f, bool v) : io::TextSink(f, io::TextFormat::CARTESIAN, v) {}
+};
+struct SphericalTextSink : io::TextSink {
+  SphericalTextSink(const std::string &f, bool v) : io::TextSink(f, io::TextFormat::SPHERICAL, v) {}
+};
+struct SphericalTextSink3dex : io::TextSink {
+  SphericalTextSink3dex(const std::string &f, bool v) : io::TextSink(f, io::TextFormat::SPHERICAL_3DEX, v) {}
+};
 
 int precision_code(const std::string &s) {
   if (s == "fp64") return 0;
@@ -379,6 +400,94 @@ PYBIND11_MODULE(_catana, m) {
       "Shot-noise term (V/N)(2/pi) k_ln^2 int_0^rmax r^2 w(r) j_l(k_ln r)^2 dr of c_ln, (lmax, nmax)",
       py::arg("w"), py::arg("lmax"), py::arg("k_max"), py::arg("rmax"), py::arg("nmax"), py::arg("n_points"),
       py::arg("window_volume") = 0.0);
+
+  // ---- catana.io (python/py_io.cpp) ----
+  py::module_ io_m = m.def_submodule("io", "Sources, sinks and point filters (python/py_io.cpp); filtering runs on "
+                                           "the device");
+  io_m.def("init_random", py::overload_cast<unsigned int>(&init_random), py::arg("seed"));
+  io_m.def("init_random", py::overload_cast<>(&init_random));
+  m.def("init_random", py::overload_cast<unsigned int>(&init_random), py::arg("seed"));
+  m.def("init_random", py::overload_cast<>(&init_random));
+  py::class_<io::Source>(io_m, "Source", "(Abstract) parent class for Source implementations.")
+      .def("reset", &io::Source::reset)
+      .def("get_point_container", &io::Source::get_point_container)
+      .def("get_pixelized_point_container", &io::Source::get_pixelized_point_container, py::arg("nside"));
+  py::class_<io::Sink>(io_m, "Sink", "(Abstract) parent class for Sink implementations")
+      .def("write", [](io::Sink &s, const PointContainer &pc) { return s.write(pc); }, py::arg("point_container"));
+  py::class_<io::Filter>(io_m, "Filter", "(Abstract) parent class for point-filtering implementations.")
+      .def("__call__", [](io::Filter &f, PointContainer &pc) { return f(pc); }, py::arg("point_container"));
+  py::class_<io::PointContainerSource, io::Source>(io_m, "PointContainerSource")
+      .def(py::init<const PointContainer &>(), py::arg("point_container"), py::keep_alive<1, 2>());
+  py::class_<io::GadgetSource, io::Source>(io_m, "GadgetSource")
+      .def(py::init<std::string, bool>(), py::arg("filename"), py::arg("verbose") = true);
+  py::class_<CartesianTextSource, io::Source>(io_m, "CartesianTextSource")
+      .def(py::init<std::string, double>(), py::arg("filename"), py::arg("shift") = 0.0);
+  py::class_<SphericalTextSource, io::Source>(io_m, "SphericalTextSource")
+      .def(py::init<std::string>(), py::arg("filename"));
+  py::class_<SphericalTextSource3dex, io::Source>(io_m, "SphericalTextSource_3dex")
+      .def(py::init<std::string>(), py::arg("filename"));
+  py::class_<io::RawBinarySource, io::Source>(io_m, "RawBinarySource")
+      .def(py::init([](std::string f, bool v) { return new io::RawBinarySource(f, v, io::RecordType::SPHERICAL_F32); }),
+           py::arg("filename"), py::arg("verbose") = true);
+  py::class_<io::PointContainerSink, io::Sink>(io_m, "PointContainerSink")
+      .def(py::init<PointContainer &>(), py::arg("point_container"), py::keep_alive<1, 2>());
+  py::class_<io::PixelizedPointContainerSink, io::Sink>(io_m, "PixelizedPointContainerSink")
+      .def(py::init<PixelizedPointContainer &>(), py::arg("pixelized_point_container"), py::keep_alive<1, 2>());
+  py::class_<CartesianTextSink, io::Sink>(io_m, "CartesianTextSink")
+      .def(py::init<std::string, bool>(), py::arg("filename"), py::arg("verbose") = true)
+      .def("close", &io::TextSink::close);
+  py::class_<SphericalTextSink, io::Sink>(io_m, "SphericalTextSink")
+      .def(py::init<std::string, bool>(), py::arg("filename"), py::arg("verbose") = true)
+      .def("close", &io::TextSink::close);
+  py::class_<SphericalTextSink3dex, io::Sink>(io_m, "SphericalTextSink_3dex")
+      .def(py::init<std::string, bool>(), py::arg("filename"), py::arg("verbose") = true)
+      .def("close", &io::TextSink::close);
+  py::class_<io::RawBinarySink, io::Sink>(io_m, "RawBinarySink")
+      .def(py::init([](std::string f, bool v, bool a) {
+             return new io::RawBinarySink(f, v, a, io::RecordType::SPHERICAL_F32);
+           }),
+           py::arg("filename"), py::arg("verbose") = true, py::arg("append") = false)
+      .def("close", &io::RawBinarySink::close);
+  py::class_<io::TophatRadialWindowFunctionFilter, io::Filter>(io_m, "TophatRadialWindowFunctionFilter")
+      .def(py::init<double>(), py::arg("R0"));
+  py::class_<io::GenericRadialWindowFunctionFilter, io::Filter>(io_m, "GenericRadialWindowFunctionFilter")
+      .def(py::init<std::function<double(double)>>(), py::arg("function"))
+      .def(py::init([](std::function<double(double)> w, size_t n, double lo, double hi) {
+             return new io::GenericRadialWindowFunctionFilter(std::move(w), n, lo, hi, false);
+           }),
+           py::arg("function"), py::arg("interpolation_points"), py::arg("min"), py::arg("max"));
+  py::class_<io::GaussianRadialWindowFunctionFilter, io::GenericRadialWindowFunctionFilter>(
+      io_m, "GaussianRadialWindowFunctionFilter")
+      .def(py::init<double>(), py::arg("R0"));
+  py::class_<io::AngularMaskFilter, io::Filter>(io_m, "AngularMaskFilter")
+      .def(py::init<const std::string &>(), py::arg("mask_file"))
+      .def(py::init([](py::array_t<float, py::array::c_style | py::array::forcecast> a) {
+             return new io::AngularMaskFilter(std::vector<float>(a.data(), a.data() + a.size()));
+           }),
+           py::arg("map"));
+  py::class_<io::FilterStream>(io_m, "FilterStream",
+                               "Connects a Source to a Sink through Filters; the filtering runs on the device")
+      .def(py::init<io::Source *, io::Sink *, size_t, bool>(), py::arg("source"), py::arg("sink"),
+           py::arg("buffer_size") = 1000000, py::arg("verbose") = true, py::keep_alive<1, 2>(),
+           py::keep_alive<1, 3>())
+      .def("add_filter", &io::FilterStream::add_filter, py::arg("filter"), py::keep_alive<1, 2>())
+      .def("set_source", &io::FilterStream::set_source, py::arg("source"), py::keep_alive<1, 2>())
+      .def("run", &io::FilterStream::run, py::arg("subsample_size") = 0,
+           py::arg("temp_filename") = std::string("tmp.bin"))
+      .def("run_totemp", &io::FilterStream::run_totemp, py::arg("temp_filename") = std::string("tmp.bin"),
+           py::arg("append") = true)
+      .def("run_fromtemp", &io::FilterStream::run_fromtemp, py::arg("temp_filename") = std::string("tmp.bin"),
+           py::arg("subsample_size") = 0, py::arg("remove_temp") = true);
+
+  py::class_<Analyzer>(m, "Analyzer", "Source + filters + SFB decomposition, the filtered catalogue kept on the device")
+      .def(py::init<io::Source *>(), py::arg("source"), py::keep_alive<1, 2>())
+      .def("set_source", &Analyzer::set_source, py::arg("source"), py::keep_alive<1, 2>())
+      .def("set_subsample_size", &Analyzer::set_subsample_size, py::arg("n"))
+      .def("add_filter", &Analyzer::add_filter, py::arg("Filter"), py::keep_alive<1, 2>())
+      .def("compute_sfb", &Analyzer::compute_sfb, py::arg("lmax"), py::arg("nmax"), py::arg("rmax"),
+           py::arg("store_flmn") = false, py::arg("verbose") = true)
+      .def("compute_sfb_pixelized", &Analyzer::compute_sfb_pixelized, py::arg("lmax"), py::arg("nmax"),
+           py::arg("rmax"), py::arg("nside"), py::arg("store_flmn") = false, py::arg("verbose") = true);
 
   m.def(
       "compute_kln",

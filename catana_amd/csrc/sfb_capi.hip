@@ -24,6 +24,7 @@
 #include "sfb_lowl.hpp"
 #include "sfb_healpix.hpp"
 #include "sfb_integrals.hpp"
+#include "sfb_ingest.hpp"
 #include "sfb_math.hpp"
 #include "sfb_support.hpp"
 
@@ -1490,6 +1491,130 @@ sfbhip_status sfbhip_sbessel_integrals(sfbhip_handle *h, int64_t n, const int32_
   CHK(read_timings(h, 0));
   if (e & ERR_DOMAIN)
     return fail(SFBHIP_DOMAIN_ERROR, "argument outside range for which this FunctionInterpolator is defined");
+  return SFBHIP_OK;
+}
+
+}  // extern "C"
+
+extern "C" {
+
+sfbhip_status sfbhip_filter_points(sfbhip_handle *h, const void *d_records, int64_t n, int32_t record_kind,
+                                   double shift, const sfbhip_filter *filters, int32_t nfilters, uint64_t seed,
+                                   uint64_t index_base, double *d_points_out, int64_t *n_out, void *stream) {
+  g_last_error.clear();
+  if (!h || !n_out) return fail(SFBHIP_INVALID_ARGUMENT, "handle or n_out is NULL");
+  if (n < 0) return fail(SFBHIP_INVALID_ARGUMENT, "negative record count");
+  if (n > INT32_MAX) return fail(SFBHIP_NOT_SUPPORTED, "filter: more than 2^31 - 1 records per call");
+  if (record_kind < SFBHIP_RECORD_POINT || record_kind > SFBHIP_RECORD_SPHERICAL_F64)
+    return fail(SFBHIP_INVALID_ARGUMENT, "unknown record kind");
+  if (nfilters < 0 || nfilters > IN_MAX_FILTERS || (nfilters > 0 && !filters))
+    return fail(SFBHIP_INVALID_ARGUMENT, "between 0 and " + std::to_string(IN_MAX_FILTERS) + " filters");
+  *n_out = 0;
+  if (n == 0) return SFBHIP_OK;
+  if (!d_records || !d_points_out) return fail(SFBHIP_INVALID_ARGUMENT, "records or output is NULL");
+  FilterSet fs{};
+  fs.count = nfilters;
+  for (int k = 0; k < nfilters; ++k) {
+    const sfbhip_filter &F = filters[k];
+    FilterDesc &D = fs.f[k];
+    D.kind = F.kind;
+    D.nside = F.nside;
+    D.a = F.a;
+    D.x_min = F.x_min;
+    D.x_max = F.x_max;
+    D.n = F.n;
+    D.values = F.d_values;
+    switch (F.kind) {
+      case SFBHIP_FILTER_TOPHAT:
+        break;
+      case SFBHIP_FILTER_GAUSSIAN:
+        if (!(F.a > 0.0)) return fail(SFBHIP_INVALID_ARGUMENT, "Gaussian filter: R0 must be positive");
+        break;
+      case SFBHIP_FILTER_WINDOW_TABLE:
+        if (F.n < 2 || F.n > INT32_MAX || !F.d_values || !(F.x_max > F.x_min))
+          return fail(SFBHIP_INVALID_ARGUMENT, "window table: >= 2 samples on x_min < x_max required");
+        D.dx_inv = (double)(F.n - 1) / (F.x_max - F.x_min);
+        break;
+      case SFBHIP_FILTER_WINDOW_VALUES:
+        if (F.n != n || !F.d_values)
+          return fail(SFBHIP_INVALID_ARGUMENT, "window values: one probability per record required");
+        break;
+      case SFBHIP_FILTER_ANGULAR_MASK:
+        if (F.nside < 1 || (F.nside & (F.nside - 1)) || F.n != 12 * (int64_t)F.nside * F.nside || !F.d_values)
+          return fail(SFBHIP_INVALID_ARGUMENT, "healpix_map is not a valid map (check size!)");
+        break;
+      default:
+        return fail(SFBHIP_INVALID_ARGUMENT, "unknown filter kind");
+    }
+  }
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  HIPCHK(hipSetDevice(h->device));
+  double *pts;
+  int32_t *keep;
+  int64_t *pos;
+  CHK(get_buf(h, "in_pts", (size_t)3 * n, &pts));
+  CHK(get_buf(h, "in_keep", (size_t)n, &keep));
+  CHK(get_buf(h, "in_pos", (size_t)n + 1, &pos));
+  int *derr;
+  CHK(zero_err(h, st, &derr));
+  const unsigned nb = (unsigned)cdiv(n, 256);
+  ingest_filter_kernel<<<nb, 256, 0, st>>>(d_records, n, record_kind, shift, fs, seed, index_base, pts, keep, derr);
+  HIPCHK(hipGetLastError());
+  size_t tmp_bytes = 0;
+  HIPCHK(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp_bytes, keep, pos, (int)n, st));
+  unsigned char *tmp;
+  CHK(get_buf(h, "in_scan_tmp", std::max<size_t>(tmp_bytes, 1), &tmp));
+  HIPCHK(hipcub::DeviceScan::ExclusiveSum(tmp, tmp_bytes, keep, pos, (int)n, st));
+  compact_points_kernel<<<nb, 256, 0, st>>>(pts, keep, pos, n, d_points_out);
+  HIPCHK(hipGetLastError());
+  int64_t last_pos = 0;
+  int32_t last_keep = 0;
+  HIPCHK(hipMemcpyAsync(&last_pos, pos + n - 1, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipMemcpyAsync(&last_keep, keep + n - 1, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+  int e = 0;
+  CHK(read_err(h, st, derr, &e));  // synchronises the stream
+  if (e & ERR_DOMAIN)
+    return fail(SFBHIP_DOMAIN_ERROR, "argument outside range for which this FunctionInterpolator is defined");
+  *n_out = last_pos + last_keep;  // exclusive scan: the count is the last position plus the last flag
+  return SFBHIP_OK;
+}
+
+sfbhip_status sfbhip_random_subset(sfbhip_handle *h, const double *d_points, int64_t n, int64_t k, uint64_t seed,
+                                   double *d_out, void *stream) {
+  g_last_error.clear();
+  if (!h) return fail(SFBHIP_INVALID_ARGUMENT, "handle is NULL");
+  if (n < 0 || k < 0) return fail(SFBHIP_INVALID_ARGUMENT, "negative size");
+  if (n > INT32_MAX) return fail(SFBHIP_NOT_SUPPORTED, "random subset: more than 2^31 - 1 points");
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  HIPCHK(hipSetDevice(h->device));
+  if (k > n) k = n;
+  if (k == 0) return SFBHIP_OK;
+  if (!d_points || !d_out) return fail(SFBHIP_INVALID_ARGUMENT, "points or output is NULL");
+  if (k == n) {
+    HIPCHK(hipMemcpyAsync(d_out, d_points, sizeof(double) * 3 * n, hipMemcpyDeviceToDevice, st));
+    HIPCHK(hipStreamSynchronize(st));
+    return SFBHIP_OK;
+  }
+  uint64_t *k0, *k1;
+  int64_t *i0, *i1;
+  CHK(get_buf(h, "ss_k0", (size_t)n, &k0));
+  CHK(get_buf(h, "ss_k1", (size_t)n, &k1));
+  CHK(get_buf(h, "ss_i0", (size_t)n, &i0));
+  CHK(get_buf(h, "ss_i1", (size_t)n, &i1));
+  const unsigned nb = (unsigned)cdiv(n, 256);
+  subset_keys_kernel<<<nb, 256, 0, st>>>(n, seed, k0, i0);
+  HIPCHK(hipGetLastError());
+  // the k smallest keys are the subset; their indices sorted back into input order
+  size_t b1 = 0, b2 = 0;
+  HIPCHK(hipcub::DeviceRadixSort::SortPairs(nullptr, b1, k0, k1, i0, i1, (int)n, 0, 64, st));
+  HIPCHK(hipcub::DeviceRadixSort::SortKeys(nullptr, b2, i1, i0, (int)k, 0, 64, st));
+  unsigned char *tmp;
+  CHK(get_buf(h, "ss_tmp", std::max<size_t>(std::max(b1, b2), 1), &tmp));
+  HIPCHK(hipcub::DeviceRadixSort::SortPairs(tmp, b1, k0, k1, i0, i1, (int)n, 0, 64, st));
+  HIPCHK(hipcub::DeviceRadixSort::SortKeys(tmp, b2, i1, i0, (int)k, 0, 64, st));
+  gather_aos_kernel<<<(unsigned)cdiv(k, 256), 256, 0, st>>>(d_points, i0, k, d_out);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(st));
   return SFBHIP_OK;
 }
 

@@ -185,6 +185,53 @@ sfbhip_status sfbhip_sbessel_nodes(sfbhip_handle *h, int64_t n, const int32_t *l
 sfbhip_status sfbhip_sbessel_integrals(sfbhip_handle *h, int64_t n, const int32_t *l, const double *k1,
                                        const double *k2, double rmax, const sfbhip_weight *w, double *out);
 
+/* ---- ingestion before the path: record conversion, point filters, subsets ---- */
+enum {
+  SFBHIP_RECORD_POINT = 0,          /* (r, theta, phi) f64: catana::Point */
+  SFBHIP_RECORD_CARTESIAN_F32 = 1,  /* (x, y, z) f32, + shift: io::CartesianRecord<float> */
+  SFBHIP_RECORD_CARTESIAN_F64 = 2,  /* (x, y, z) f64, + shift: io::CartesianRecord<double> */
+  SFBHIP_RECORD_SPHERICAL_F32 = 3,  /* (r, theta, phi) f32: io::SphericalRecord<float> */
+  SFBHIP_RECORD_SPHERICAL_F64 = 4   /* (r, theta, phi) f64: io::SphericalRecord<double> */
+};
+enum {
+  SFBHIP_FILTER_TOPHAT = 0,         /* keep r < a (TophatRadialWindowFunctionFilter) */
+  SFBHIP_FILTER_GAUSSIAN = 1,       /* keep with probability exp(-(r/a)^2) (GaussianRadialWindowFunctionFilter) */
+  SFBHIP_FILTER_WINDOW_TABLE = 2,   /* keep with probability w(r), w linearly interpolated from n samples on
+                                       [x_min, x_max]; r outside [x_min, x_max) is a domain error
+                                       (GenericRadialWindowFunctionFilter with interpolation) */
+  SFBHIP_FILTER_WINDOW_VALUES = 3,  /* keep record i with probability d_values[i] (n = record count; the
+                                       caller evaluated w at the records: GenericRadialWindowFunctionFilter) */
+  SFBHIP_FILTER_ANGULAR_MASK = 4    /* keep if d_values[ang2pix_ring(nside, theta, phi)] == 1.0f; float map of
+                                       n = 12 nside^2 pixels (AngularMaskFilter) */
+};
+typedef struct sfbhip_filter {
+  int32_t kind;          /* SFBHIP_FILTER_* */
+  int32_t nside;         /* ANGULAR_MASK */
+  double a;              /* TOPHAT / GAUSSIAN radius */
+  double x_min, x_max;   /* WINDOW_TABLE range */
+  int64_t n;             /* WINDOW_TABLE samples, WINDOW_VALUES records, ANGULAR_MASK pixels */
+  const void *d_values;  /* device: f64 samples / f64 probabilities / f32 mask */
+} sfbhip_filter;
+
+/* Converts n device records to points, applies up to 8 filters in order and
+ * writes the points that pass all of them, in input order, to d_points_out
+ * (AoS (r, theta, phi) f64, capacity n); *n_out (host) receives their count.
+ * A random filter keeps a point when its acceptance probability exceeds a
+ * uniform draw that depends only on (seed, filter position, index_base + i).
+ * Replaces FilterStream::run_step's filter loop (FilterStream.cpp:103-126),
+ * Filter::operator() (Filter.cpp:7-34) and the filters' filter(Point&)
+ * (radial_filters.cpp:10-33, angular_filters.cpp:9-12); record conversion as
+ * CartesianRecord / SphericalRecord::point (record_types.hpp, Point.cpp:5-27). */
+sfbhip_status sfbhip_filter_points(sfbhip_handle *h, const void *d_records, int64_t n, int32_t record_kind,
+                                   double shift, const sfbhip_filter *filters, int32_t nfilters, uint64_t seed,
+                                   uint64_t index_base, double *d_points_out, int64_t *n_out, void *stream);
+
+/* A uniform random subset of k of the n device points (AoS), in input order,
+ * into d_out (capacity k).  Replaces RandomSubsetFilter (FilterStream.cpp:14-41)
+ * as FilterStream::run_fromtemp applies it; k >= n copies all points. */
+sfbhip_status sfbhip_random_subset(sfbhip_handle *h, const double *d_points, int64_t n, int64_t k, uint64_t seed,
+                                   double *d_out, void *stream);
+
 /* Duration in ms of the dominant kernel of the last call on this handle
  * (the mixed contraction kernels when the call ran any, else the FP64
  * contraction kernels; for sfbhip_sbessel_integrals the integral kernel),
