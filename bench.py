@@ -1,7 +1,7 @@
 """Benchmark of the direct (point-mode) SFB decomposition on one MI355X.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W]
-                    [--workload clustered1m|lognormal2p25|uniform10k|batched64|healpix4m|shotnoise2p25]
+                    [--workload clustered1m|lognormal2p25|uniform10k|batched64|healpix4m|shotnoise2p25|ingest2p25]
                     [--precision mixed|fp64] [--cpu-sample N | --no-cpu-baseline]
 
 A step is one full sfbhip_direct call (k_ln lookup, point preparation, the
@@ -41,6 +41,9 @@ WORKLOADS = {
     # the shot-noise step after the path: (V/N)(2/pi) k^2 int r^2 w(r) j_l(k r)^2 dr for every kept
     # mode of the 2^25-point configuration (lmax 256, k_max 0.5, rmax 1000: 26,102 integrals), w = tophat
     "shotnoise2p25": dict(lmax=256, nmax=160, rmax=1000.0, k_max=0.5, shotnoise=True),
+    # the ingestion step before the path: 2^25 float32 cartesian records (a Gadget-like box of side
+    # 2000 centred by shift -1000) through tophat(1000) + angular mask (nside 64) + Gaussian(400)
+    "ingest2p25": dict(n=2 ** 25, rmax=1000.0, nside=64, r0=400.0, ingest=True),
 }
 
 
@@ -62,7 +65,8 @@ def make_batch(seed=0, ncat=64):
 # rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE on this build, scaled by what the counters read
 # for a 512 MiB device-to-device copy (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0).  Counters cannot
 # be read from inside the timed run, so the figure is carried here per workload.
-PROFILED_TRAFFIC_BYTES = {("clustered1m", "mixed", "split16", 2, 4): 3.132e9}  # (workload, precision, operands, fp64_below, chunks)
+PROFILED_TRAFFIC_BYTES = {("clustered1m", "mixed", "split16", 2, 4): 3.132e9,  # (workload, precision, operands,
+                          ("ingest2p25",): 1.337e9}                            #  fp64_below, chunks)
 
 
 def make_points(name):
@@ -145,6 +149,9 @@ def main():
         return
     if cfg.get("shotnoise"):
         run_shotnoise(args, cfg)
+        return
+    if cfg.get("ingest"):
+        run_ingest(args, cfg)
         return
     log(f"generating workload {args.workload}")
     pts = make_points(args.workload)
@@ -369,6 +376,68 @@ def run_shotnoise(args, cfg):
                                          "restatement of double_sbessel_integrator, adaptive Gauss-Legendre"}
     h.close()
     print(json.dumps(out))
+
+
+def run_ingest(args, cfg):
+    """The ingestion workload: a step is one sfbhip_filter_points call over all
+    records already resident in HBM (conversion, three filters, compaction)."""
+    import torch
+    from catana_amd import capi
+    h = capi.Handle(0)
+    n, R, ns = cfg["n"], cfg["rmax"], cfg["nside"]
+    rng = np.random.default_rng(21)
+    rec = rng.uniform(0.0, 2 * R, (n, 3)).astype(np.float32)
+    mask = (rng.uniform(size=12 * ns * ns) < 0.7).astype(np.float32)
+    dev = torch.device("cuda:0")
+    d_rec = torch.from_numpy(rec).to(dev)
+    d_mask = torch.from_numpy(mask).to(dev)
+    filters = [dict(kind=capi.FILTER_TOPHAT, a=R), dict(kind=capi.FILTER_ANGULAR_MASK, nside=ns, values=d_mask),
+               dict(kind=capi.FILTER_GAUSSIAN, a=cfg["r0"])]
+
+    def step(i):
+        return capi.filter_points_torch(h, d_rec, capi.RECORD_CARTESIAN_F32, filters, seed=9, index_base=0,
+                                        shift=-R)
+
+    for i in range(args.warmup):
+        out = step(i)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        out = step(i)
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    kept = int(out.shape[0])
+    algo = 12.0 * n + 24.0 * kept  # records read once, kept points written once
+    achieved = algo / (ms * 1e-3) / 1e9
+    res = {
+        "metric": "records filtered/s (conversion + tophat + angular mask + Gaussian + compaction)",
+        "value": n / (ms * 1e-3), "unit": "records/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32 in, f64 out",
+        "data": "synthetic: uniform float32 cartesian records in a box of side 2000, seeded",
+        "config": {"workload": "ingest2p25", "records": n, "kept": kept, "nside": ns, "R0_gauss": cfg["r0"]},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": 8000.0, "unit": "GB/s",
+                     "frac": achieved / 8000.0, "traffic": PROFILED_TRAFFIC_BYTES.get(("ingest2p25",)),
+                     "kernel": "ingest_flag_kernel + scan + compact_convert_kernel (whole call)",
+                     "algorithmic_bytes_per_call": algo},
+    }
+    # parity on a sample: the device's kept set of the first 2^16 records equals the oracle's
+    sys.path.insert(0, ROOT)
+    from oracle import ingest as I
+    m = 1 << 16
+    sub = capi.filter_points_torch(h, d_rec[:m], capi.RECORD_CARTESIAN_F32, filters, seed=9, index_base=0,
+                                   shift=-R).cpu().numpy()
+    fo = [("tophat", R), ("mask", mask), ("gaussian", cfg["r0"])]
+    exp, _ = I.apply_filters(I.cartesian_to_points(rec[:m].astype(np.float64), -R), fo, seed=9)
+    res["accuracy"] = {"sample_records": m, "kept_equal": bool(sub.shape == exp.shape),
+                       "max_abs_diff": float(np.abs(sub - exp).max()) if sub.shape == exp.shape and sub.size else None}
+    if not args.no_cpu_baseline:
+        t0 = time.perf_counter()
+        I.apply_filters(I.cartesian_to_points(rec[:m].astype(np.float64), -R), fo, seed=9)
+        dt = time.perf_counter() - t0
+        res["cpu_baseline"] = {"value": m / dt, "unit": "records/s", "cores": 1, "kind": "port",
+                               "sample": f"{m} of {n} records, numpy restatement (oracle/ingest.py)"}
+    h.close()
+    print(json.dumps(res))
 
 
 def run_batched(args):

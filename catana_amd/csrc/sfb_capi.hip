@@ -1549,33 +1549,31 @@ sfbhip_status sfbhip_filter_points(sfbhip_handle *h, const void *d_records, int6
   }
   hipStream_t st = static_cast<hipStream_t>(stream);
   HIPCHK(hipSetDevice(h->device));
-  double *pts;
-  int32_t *keep;
+  int32_t *flag;
   int64_t *pos;
-  CHK(get_buf(h, "in_pts", (size_t)3 * n, &pts));
-  CHK(get_buf(h, "in_keep", (size_t)n, &keep));
-  CHK(get_buf(h, "in_pos", (size_t)n + 1, &pos));
+  CHK(get_buf(h, "in_flag", (size_t)n, &flag));
+  CHK(get_buf(h, "in_pos", (size_t)n, &pos));
   int *derr;
   CHK(zero_err(h, st, &derr));
   const unsigned nb = (unsigned)cdiv(n, 256);
-  ingest_filter_kernel<<<nb, 256, 0, st>>>(d_records, n, record_kind, shift, fs, seed, index_base, pts, keep, derr);
+  ingest_flag_kernel<<<nb, 256, 0, st>>>(d_records, n, record_kind, shift, fs, seed, index_base, flag, derr);
   HIPCHK(hipGetLastError());
   size_t tmp_bytes = 0;
-  HIPCHK(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp_bytes, keep, pos, (int)n, st));
+  HIPCHK(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp_bytes, flag, pos, (int)n, st));
   unsigned char *tmp;
   CHK(get_buf(h, "in_scan_tmp", std::max<size_t>(tmp_bytes, 1), &tmp));
-  HIPCHK(hipcub::DeviceScan::ExclusiveSum(tmp, tmp_bytes, keep, pos, (int)n, st));
-  compact_points_kernel<<<nb, 256, 0, st>>>(pts, keep, pos, n, d_points_out);
+  HIPCHK(hipcub::DeviceScan::ExclusiveSum(tmp, tmp_bytes, flag, pos, (int)n, st));
+  compact_convert_kernel<<<nb, 256, 0, st>>>(d_records, flag, pos, n, record_kind, shift, d_points_out);
   HIPCHK(hipGetLastError());
   int64_t last_pos = 0;
-  int32_t last_keep = 0;
+  int32_t last_flag = 0;
   HIPCHK(hipMemcpyAsync(&last_pos, pos + n - 1, sizeof(int64_t), hipMemcpyDeviceToHost, st));
-  HIPCHK(hipMemcpyAsync(&last_keep, keep + n - 1, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipMemcpyAsync(&last_flag, flag + n - 1, sizeof(int32_t), hipMemcpyDeviceToHost, st));
   int e = 0;
   CHK(read_err(h, st, derr, &e));  // synchronises the stream
   if (e & ERR_DOMAIN)
     return fail(SFBHIP_DOMAIN_ERROR, "argument outside range for which this FunctionInterpolator is defined");
-  *n_out = last_pos + last_keep;  // exclusive scan: the count is the last position plus the last flag
+  *n_out = last_pos + last_flag;  // exclusive scan: the count is the last position plus the last flag
   return SFBHIP_OK;
 }
 

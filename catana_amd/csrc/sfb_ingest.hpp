@@ -2,7 +2,10 @@
 // conversion, the reference's point filters and an order-preserving
 // compaction into the AoS (r, theta, phi) catalogue the SFB kernels read.
 //
-// Records (one thread each):
+// Three passes: a flag per record (conversion as far as the filters need it),
+// an exclusive scan of the flags (hipcub), and the conversion of the kept
+// records straight into their place in the output.
+// Records:
 //   POINT          (r, theta, phi) f64 -- catana::Point as read by a Source
 //   CARTESIAN_F32 / _F64   (x, y, z): point_from_box_position(x, y, z, shift)
 //                  (record_types.hpp CartesianRecord::point, Point.cpp:5-21):
@@ -56,54 +59,65 @@ __host__ __device__ inline double filter_uniform(uint64_t seed, int slot, uint64
   return (double)(splitmix64(s + index) >> 11) * 0x1.0p-53;
 }
 
-__device__ inline void record_point(const void *rec, int kind, int64_t i, double shift, double *r, double *th,
-                                    double *ph) {
-  double a, b, c;
-  switch (kind) {
-    case REC_CART_F32:
-    case REC_SPH_F32: {
-      const float *p = static_cast<const float *>(rec) + 3 * i;
-      a = p[0];
-      b = p[1];
-      c = p[2];
-      break;
-    }
-    default: {
-      const double *p = static_cast<const double *>(rec) + 3 * i;
-      a = p[0];
-      b = p[1];
-      c = p[2];
-    }
-  }
-  if (kind == REC_CART_F32 || kind == REC_CART_F64) {
-    const double x = a + shift, y = b + shift, z = c + shift;
-    *r = sqrt(x * x + y * y + z * z);
-    *th = atan2(sqrt(x * x + y * y), z);
-    double f = (x == 0.0 && y == 0.0) ? 0.0 : atan2(y, x);
-    if (f < 0.0) f += 6.283185307179586476925;
-    if (f >= 6.283185307179586476925) f -= 6.283185307179586476925;
-    *ph = f;
+struct Rec3F {
+  float v[3];
+};
+struct Rec3D {
+  double v[3];
+};
+
+__device__ inline void load_record(const void *rec, int kind, int64_t i, double *a, double *b, double *c) {
+  if (kind == REC_CART_F32 || kind == REC_SPH_F32) {
+    const Rec3F q = static_cast<const Rec3F *>(rec)[i];
+    *a = q.v[0];
+    *b = q.v[1];
+    *c = q.v[2];
   } else {
-    *r = a;
-    *th = b;
-    *ph = c;
+    const Rec3D q = static_cast<const Rec3D *>(rec)[i];
+    *a = q.v[0];
+    *b = q.v[1];
+    *c = q.v[2];
   }
 }
 
-// Converts record i, applies the filters, writes the point (AoS) and its keep flag.
-__global__ void ingest_filter_kernel(const void *__restrict__ rec, int64_t n, int kind, double shift, FilterSet fs,
-                                     uint64_t seed, uint64_t index_base, double *__restrict__ pts,
-                                     int32_t *__restrict__ keep, int *__restrict__ err) {
+// the direction of a shifted cartesian record (Point.cpp:5-10)
+__device__ inline void cart_angles(double x, double y, double z, double *th, double *ph) {
+  *th = atan2(sqrt(x * x + y * y), z);
+  double f = (x == 0.0 && y == 0.0) ? 0.0 : atan2(y, x);
+  if (f < 0.0) f += 6.283185307179586476925;
+  if (f >= 6.283185307179586476925) f -= 6.283185307179586476925;
+  *ph = f;
+}
+
+// Pass 1: the keep flag of every record.  The radius comes first; the
+// direction of a cartesian record is computed only when a mask needs it.
+__global__ void ingest_flag_kernel(const void *__restrict__ rec, int64_t n, int kind, double shift, FilterSet fs,
+                                   uint64_t seed, uint64_t index_base, int32_t *__restrict__ flag,
+                                   int *__restrict__ err) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  double r, th, ph;
-  record_point(rec, kind, i, shift, &r, &th, &ph);
-  bool ok = true;
+  double a, b, c;
+  load_record(rec, kind, i, &a, &b, &c);
+  const bool cart = kind == REC_CART_F32 || kind == REC_CART_F64;
+  double r, th = b, ph = c;
+  if (cart) {
+    a += shift;
+    b += shift;
+    c += shift;
+    r = sqrt(a * a + b * b + c * c);
+  } else {
+    r = a;
+  }
+  bool have_dir = !cart, ok = true;
   for (int k = 0; k < fs.count && ok; ++k) {
     const FilterDesc &F = fs.f[k];
     if (F.kind == FLT_TOPHAT) {
       ok = r < F.a;
     } else if (F.kind == FLT_ANGULAR_MASK) {
+      if (!have_dir) {
+        cart_angles(a, b, c, &th, &ph);
+        have_dir = true;
+      }
       const int64_t pix = (th == th && ph == ph) ? hp_ang2pix(F.nside, th, ph) : -1;  // NaN: no pixel
       ok = pix >= 0 && pix < F.n && static_cast<const float *>(F.values)[pix] == 1.0f;
     } else {
@@ -128,21 +142,34 @@ __global__ void ingest_filter_kernel(const void *__restrict__ rec, int64_t n, in
       ok = w > filter_uniform(seed, k, index_base + (uint64_t)i);
     }
   }
-  pts[3 * i] = r;
-  pts[3 * i + 1] = th;
-  pts[3 * i + 2] = ph;
-  keep[i] = ok ? 1 : 0;
+  flag[i] = ok ? 1 : 0;
 }
 
-// out[pos[i]] = pts[i] for kept i (pos: exclusive prefix sum of keep)
-__global__ void compact_points_kernel(const double *__restrict__ pts, const int32_t *__restrict__ keep,
-                                      const int64_t *__restrict__ pos, int64_t n, double *__restrict__ out) {
+// Pass 3: kept record i goes, converted, to out[pos[i]] (pos: exclusive
+// prefix sum of the flags); only kept records are read again.
+__global__ void compact_convert_kernel(const void *__restrict__ rec, const int32_t *__restrict__ flag,
+                                       const int64_t *__restrict__ pos, int64_t n, int kind, double shift,
+                                       double *__restrict__ out) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n || !keep[i]) return;
+  if (i >= n || !flag[i]) return;
+  double a, b, c;
+  load_record(rec, kind, i, &a, &b, &c);
+  double r, th, ph;
+  if (kind == REC_CART_F32 || kind == REC_CART_F64) {
+    a += shift;
+    b += shift;
+    c += shift;
+    r = sqrt(a * a + b * b + c * c);
+    cart_angles(a, b, c, &th, &ph);
+  } else {
+    r = a;
+    th = b;
+    ph = c;
+  }
   const int64_t o = pos[i];
-  out[3 * o] = pts[3 * i];
-  out[3 * o + 1] = pts[3 * i + 1];
-  out[3 * o + 2] = pts[3 * i + 2];
+  out[3 * o] = r;
+  out[3 * o + 1] = th;
+  out[3 * o + 2] = ph;
 }
 
 // keys of a uniform random subset: the k smallest keys win
