@@ -122,11 +122,12 @@ __device__ __forceinline__ float jl_hermite(const float4 *__restrict__ T, int ni
   return fmaf(f, fmaf(f, fmaf(f, c3, c2), q.y), q.x);
 }
 
-// The same from a node copy in LDS: node i = (j_l(i h), h j_l'(i h)), i <= nint.
-__device__ __forceinline__ float jl_hermite_lds(const float2 *T, int nint, double t) {
-  int i = (int)t;
-  i = i < nint - 1 ? i : nint - 1;
-  i = i > 0 ? i : 0;
+// The same from a node copy in LDS: node i = (j_l(i h), h j_l'(i h)).  No
+// clamping: the caller guarantees 0 <= t and that node floor(t) + 1 is staged
+// (the plan stages floor(k R / h) + 2 nodes past the item's largest argument,
+// and a padding point has r = 0).
+__device__ __forceinline__ float jl_hermite_lds(const float2 *T, double t) {
+  const int i = (int)t;
   const float f = (float)(t - (double)i);
   const float2 a = T[i], b = T[i + 1];
   const float d = b.x - a.x;
@@ -204,11 +205,14 @@ __global__ __launch_bounds__(256) void sfb_direct_mixed_kernel(
   const int ntab = min(nint, it.tab);
   const bool tl = ntab + 1 <= lds_nodes;  // block-uniform
   if (tl) {
+    // FP16 operands carry v * 2^12: the staged nodes carry the (exact) scale
+    const float sc = NP ? MX_SCALE : 1.f;
     for (int i = tid; i <= ntab; i += 256) {
       const float4 q = T[i < ntab ? i : ntab - 1];
-      TN[i] = i < ntab ? make_float2(q.x, q.y) : make_float2(q.z, q.w);
+      TN[i] = i < ntab ? make_float2(q.x * sc, q.y * sc) : make_float2(q.z * sc, q.w * sc);
     }
   }
+  // rows past nrows get k = 0, so their arguments stay at 0 (their results are not read)
   const double khr = rowok ? kln[it.kofs + row] * (1.0 / MX_H) : 0.0;
   const double2 sect = leg.sect[l];
   {
@@ -225,7 +229,9 @@ __global__ __launch_bounds__(256) void sfb_direct_mixed_kernel(
   for (int64_t base = it.p0; base < it.p1; base += MX_TP) {
     const int64_t gp = base + tid;
     const bool pok = gp < it.p1;
-    rs[tid] = pok ? pts.r[gp] : -1.0;
+    // a padding point gets r = 0: its Bessel arguments stay inside the table, and
+    // its Y* columns are zero, so it adds nothing
+    rs[tid] = pok ? pts.r[gp] : 0.0;
     __syncthreads();  // rs ready; previous tile's readers done
 
     // ---- 1. Bessel values into A-operand registers
@@ -240,8 +246,8 @@ __global__ __launch_bounds__(256) void sfb_direct_mixed_kernel(
 #pragma unroll
             for (int e = 0; e < 8; e += 2) {
               const double r0 = rs[32 * s + 8 * g + e], r1 = rs[32 * s + 8 * g + e + 1];
-              const float v0 = (rowok && r0 >= 0.0) ? jl_hermite_lds(TN, ntab, r0 * khr) * MX_SCALE : 0.f;
-              const float v1 = (rowok && r1 >= 0.0) ? jl_hermite_lds(TN, ntab, r1 * khr) * MX_SCALE : 0.f;
+              const float v0 = jl_hermite_lds(TN, r0 * khr);
+              const float v1 = jl_hermite_lds(TN, r1 * khr);
               if constexpr (NP == 2)
                 split_pair(v0, v1, pc[0][e / 2], pc[1][e / 2]);
               else
@@ -257,7 +263,7 @@ __global__ __launch_bounds__(256) void sfb_direct_mixed_kernel(
 #pragma unroll
           for (int s = 0; s < MX_TP / 4; ++s) {
             const double r = rs[4 * s + g];
-            af[s] = (rowok && r >= 0.0) ? jl_hermite_lds(TN, ntab, r * khr) : 0.f;
+            af[s] = jl_hermite_lds(TN, r * khr);
           }
         }
       } else {
@@ -268,8 +274,8 @@ __global__ __launch_bounds__(256) void sfb_direct_mixed_kernel(
 #pragma unroll
             for (int e = 0; e < 8; e += 2) {
               const double r0 = rs[32 * s + 8 * g + e], r1 = rs[32 * s + 8 * g + e + 1];
-              const float v0 = (rowok && r0 >= 0.0) ? jl_hermite(T, nint, r0 * khr) * MX_SCALE : 0.f;
-              const float v1 = (rowok && r1 >= 0.0) ? jl_hermite(T, nint, r1 * khr) * MX_SCALE : 0.f;
+              const float v0 = jl_hermite(T, nint, r0 * khr) * MX_SCALE;
+              const float v1 = jl_hermite(T, nint, r1 * khr) * MX_SCALE;
               if constexpr (NP == 2)
                 split_pair(v0, v1, pc[0][e / 2], pc[1][e / 2]);
               else
@@ -285,7 +291,7 @@ __global__ __launch_bounds__(256) void sfb_direct_mixed_kernel(
 #pragma unroll
           for (int s = 0; s < MX_TP / 4; ++s) {
             const double r = rs[4 * s + g];
-            af[s] = (rowok && r >= 0.0) ? jl_hermite(T, nint, r * khr) : 0.f;
+            af[s] = jl_hermite(T, nint, r * khr);
           }
         }
       }
