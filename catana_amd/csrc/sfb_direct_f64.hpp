@@ -192,7 +192,7 @@ __global__ __launch_bounds__(256) void sfb_direct_f64_kernel(
       } else {
         g.v = g.vp = 0.0;
         g.E = 0;
-        g.cm = g.sm = g.cot = g.cph = g.sph = 0.0;
+        g.cm = g.sm = g.cp = g.sp = g.cot = g.cph = g.sph = 0.0;
         gx.y = g;
         gx.cm2 = gx.sm2 = gx.cph2 = gx.sph2 = 0.0;
         gx.np = 1;

@@ -128,7 +128,7 @@ __device__ __forceinline__ float jl_hermite(const float4 *__restrict__ T, int ni
 // and a padding point has r = 0).
 __device__ __forceinline__ float jl_hermite_lds(const float2 *T, double t) {
   const int i = (int)t;
-  const float f = (float)(t - (double)i);
+  const float f = (float)__builtin_amdgcn_fract(t);  // t - floor(t), exact
   const float2 a = T[i], b = T[i + 1];
   const float d = b.x - a.x;
   const float c2 = 3.f * d - 2.f * a.y - b.y;
@@ -308,7 +308,7 @@ __global__ __launch_bounds__(256) void sfb_direct_mixed_kernel(
     } else {
       y.v = y.vp = 0.0;
       y.E = 0;
-      y.cm = y.sm = y.cot = y.cph = y.sph = 0.0;
+      y.cm = y.sm = y.cp = y.sp = y.cot = y.cph = y.sph = 0.0;
     }
 #pragma unroll
     for (int qq = 0; qq < QMAX; ++qq) {

@@ -22,6 +22,7 @@ struct YlmDown {
   double v, vp;  // P^m, P^{m+1} without the 2^E factor
   int E;
   double cm, sm;  // cos(m phi), sin(m phi)
+  double cp, sp;  // cos((m+1) phi), sin((m+1) phi)
   double cot, cph, sph;
 
   __device__ __forceinline__ void init(int l, double x, double s, double phi, double cph_, double sph_,
@@ -56,6 +57,8 @@ struct YlmDown {
     sincos(hi, &S, &C);
     cm = C - lo * S;
     sm = S + lo * C;
+    cp = fma(cm, cph, -sm * sph);
+    sp = fma(sm, cph, cm * sph);
   }
 
   // Y*_lm = P_l^m (cos m phi - i sin m phi)
@@ -65,13 +68,17 @@ struct YlmDown {
     im = -P * sm;
   }
 
-  // m -> m-1 with ab = (a_m, b_m)
+  // m -> m-1 with ab = (a_m, b_m); the azimuth by the Chebyshev recurrence
+  // cos((m-1) phi) = 2 cos(phi) cos(m phi) - cos((m+1) phi) (and sin alike)
   __device__ __forceinline__ void step(double2 ab) {
     const double vn = -fma(ab.x * cot, v, ab.y * vp);
     vp = v;
     v = vn;
-    const double cn = fma(cm, cph, sm * sph);
-    const double sn = fma(sm, cph, -cm * sph);
+    const double c2 = cph + cph;
+    const double cn = fma(c2, cm, -cp);
+    const double sn = fma(c2, sm, -sp);
+    cp = cm;
+    sp = sm;
     cm = cn;
     sm = sn;
   }
