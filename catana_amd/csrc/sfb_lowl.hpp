@@ -25,7 +25,8 @@
 namespace sfb {
 
 constexpr int LW_LMAX = 8;   // multipoles handled: l < LW_LMAX
-constexpr int LW_ROWS = 64;  // rows of n per item (four 16-row MFMA tiles)
+constexpr int LW_ROWS = 16;  // rows of n per item (one 16-row MFMA tile: many small items fill the chip)
+constexpr int LW_TILES = LW_ROWS / 16;
 constexpr int LW_STEP = 16;  // points per block step (4 waves x 4 points)
 constexpr int LW_TILE = 256; // points staged in LDS per block iteration
 
@@ -35,10 +36,10 @@ __global__ __launch_bounds__(256) void sfb_lowl_f64_kernel(PointsSoA pts, const 
   __shared__ double2 acoef[LW_LMAX * LW_LMAX];  // [m][L]: (a_L, 1 / a_{L-1})
   __shared__ double sect[LW_LMAX];
   // one LDS area: the tile's point data during the sweep, the waves' blocks at the end
-  __shared__ double lds[3 * LW_ROWS * 16];
+  constexpr int LW_LDS = 5 * LW_TILE > 3 * LW_ROWS * 16 ? 5 * LW_TILE : 3 * LW_ROWS * 16;
+  __shared__ double lds[LW_LDS];
   double(*pt)[LW_TILE] = reinterpret_cast<double(*)[LW_TILE]>(lds);  // r (or -1), cos t, sin t, cos phi, sin phi
   double(*red)[LW_ROWS * 16] = reinterpret_cast<double(*)[LW_ROWS * 16]>(lds);
-  static_assert(5 * LW_TILE <= 3 * LW_ROWS * 16, "point tile fits the reduction area");
 
   const DirectItem it = items[blockIdx.x];
   const int l = it.l;
@@ -62,9 +63,9 @@ __global__ __launch_bounds__(256) void sfb_lowl_f64_kernel(PointsSoA pts, const 
   const int col = lane & 15, m = col >> 1, kq = lane >> 4;
   const bool live_col = m <= l;
   const double *kl = kln + it.kofs;
-  double kr[4];
+  double kr[LW_TILES];
 #pragma unroll
-  for (int t = 0; t < 4; ++t) {
+  for (int t = 0; t < LW_TILES; ++t) {
     const int row = 16 * t + (lane & 15);
     kr[t] = row < it.nrows ? kl[row] : 0.0;
   }
@@ -73,9 +74,9 @@ __global__ __launch_bounds__(256) void sfb_lowl_f64_kernel(PointsSoA pts, const 
   const int last = tab.q5_nint[l] - 1;
   const double s0 = sect[m];
 
-  v4d acc[4];
+  v4d acc[LW_TILES];
 #pragma unroll
-  for (int t = 0; t < 4; ++t) acc[t] = v4d{0.0, 0.0, 0.0, 0.0};
+  for (int t = 0; t < LW_TILES; ++t) acc[t] = v4d{0.0, 0.0, 0.0, 0.0};
 
   // 256-point tiles: the block stages the tile's point data in LDS with one
   // coalesced load per thread, then wave w takes points 64w .. 64w+63, four per step
@@ -115,7 +116,7 @@ __global__ __launch_bounds__(256) void sfb_lowl_f64_kernel(PointsSoA pts, const 
       y = (ok && live_col) ? y : 0.0;
       // A fragments: j_l(k r) of rows 16t + lane%16, point kq
 #pragma unroll
-      for (int t = 0; t < 4; ++t) {
+      for (int t = 0; t < LW_TILES; ++t) {
         if (t < ntile) {
           const double u = kr[t] * r * Q5_HINV;
           int i = (int)u;
@@ -134,7 +135,7 @@ __global__ __launch_bounds__(256) void sfb_lowl_f64_kernel(PointsSoA pts, const 
   // element j of lane i is row i/16 + 4j, column i%16
   if (wave > 0) {
 #pragma unroll
-    for (int t = 0; t < 4; ++t)
+    for (int t = 0; t < LW_TILES; ++t)
 #pragma unroll
       for (int j = 0; j < 4; ++j) red[wave - 1][(16 * t + (lane >> 4) + 4 * j) * 16 + col] = acc[t][j];
   }
@@ -143,7 +144,7 @@ __global__ __launch_bounds__(256) void sfb_lowl_f64_kernel(PointsSoA pts, const 
     const int ncol = 2 * (l + 1);
     double *out = partial + it.out;
 #pragma unroll
-    for (int t = 0; t < 4; ++t)
+    for (int t = 0; t < LW_TILES; ++t)
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
         const int row = 16 * t + (lane >> 4) + 4 * j;
