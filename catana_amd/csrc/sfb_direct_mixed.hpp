@@ -4,16 +4,18 @@
 // over points, sfb_raw.cpp:61-80), with the operands generated cheaply:
 //   * j_l(k_ln r) by cubic Hermite interpolation of (j_l(x_i), h j_l'(x_i)) on
 //     the grid x_i = i h (h = 1/16), FP32 (error <= h^4/384 max|j_l''''| ~ 4e-8
-//     of the local amplitude).  The table lives in global memory, one float4
-//     (j_i, h j_i', j_{i+1}, h j_{i+1}') per interval, so one 16-byte load per
-//     value; the points are sorted by radius beforehand, so that the lanes of
-//     a row touch a few neighbouring intervals and the loads hit the L1/L2
-//     caches.  The argument t = r k_ln / h and its split into interval index
-//     and fraction are formed in FP64, so the phase error does not grow with
-//     k r.  The table of l reaches k_{l,n_l-1} max(rmax, max_i r_i), so every
-//     argument of the catalogue lies inside it.
-//   * Y*_lm from the same FP64 downward-m recurrence as the FP64 kernel; the
-//     recurrence coefficients are read with wave-uniform (scalar) loads.
+//     of the local amplitude).  The nodes an item's arguments reach are copied
+//     into LDS (pre-scaled by the FP16 operand scale) when they fit two blocks
+//     per CU; otherwise the item reads the global table, one float4 (j_i, h j_i',
+//     j_{i+1}, h j_{i+1}') per interval.  The points are sorted by radius
+//     beforehand, so the lanes of a row touch a few neighbouring intervals.  The
+//     argument t = r k_ln / h and its split into interval index and fraction
+//     are formed in FP64, so the phase error does not grow with k r.  The table
+//     of l reaches k_{l,n_l-1} max(rmax, max_i r_i) plus two nodes, padding
+//     points have r = 0 and padding rows k = 0, so a lookup needs no guard.
+//   * Y*_lm from the same FP64 downward-m recurrence as the FP64 kernel (the
+//     azimuth by the Chebyshev recurrence); the recurrence coefficients of l
+//     are staged in LDS.
 //   * each wave evaluates its Bessel values straight into MFMA A-operand
 //     registers (it owns 16 rows of n, so every value is computed once and
 //     never staged); the Y* chunk is staged in LDS as the B operand.

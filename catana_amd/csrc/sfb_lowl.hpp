@@ -1,13 +1,15 @@
-// sfb_lowl.hpp -- FP64 kernel for the lowest multipoles (l < 8) of the mixed path.
+// sfb_lowl.hpp -- FP64 kernel for the lowest multipoles of the mixed path
+// (l < mixed_fp64_below, at most 8; by default l = 0 and 1).
 //
 // For l < 8 the 2(l+1) real columns of Y* fit one 16-wide MFMA column tile,
 // so each wave runs independently: per step of 4 points, lane (k = lane/16,
 // c = lane%16) evaluates Y*_l,c/2 of point k (column c, real or imaginary part)
 // -- the B fragment of v_mfma_f64_16x16x4f64 -- and, for each 16-row tile of
 // n, j_l(k_ln r) of row lane%16 and point k -- the A fragment -- from the
-// quintic FP64 table (error ~1e-10 of the amplitude).  The point data of each
-// 256-point tile is staged in LDS; the four waves' 64 x 16 blocks are summed,
-// in a fixed order, at the end of the item.  (The FP32 cubic Hermite table of
+// quintic FP64 table (error ~1e-10 of the amplitude).  An item is one 16-row
+// tile (many small items keep enough waves in flight for these two or so
+// multipoles); the point data of each 256-point tile is staged in LDS, and the
+// four waves' blocks are summed, in a fixed order, at the end of the item.  (The FP32 cubic Hermite table of
 // the mixed kernel is not accurate enough here: l = 0 and 1 have C_l with one
 // or three real degrees of freedom, so some c_ln are far below their noise
 // level and their relative error is magnified.)
