@@ -646,7 +646,9 @@ StreamBufs &stream_bufs() {
 }
 
 // Filters n host points with the given filters on the device; appends the
-// survivors to bufs.all (device) and returns their count.
+// survivors to bufs.all (device) and returns their count.  The filters'
+// host work (a Python window function, say) runs before the handle's lock is
+// taken, so such a callback may itself call into this library.
 size_t filter_chunk(Context &c, StreamBufs &b, const Point *pts, size_t n, const std::vector<io::Filter *> &filters) {
   if (n == 0) return 0;
   std::vector<sfbhip_filter> fs;
@@ -654,6 +656,7 @@ size_t filter_chunk(Context &c, StreamBufs &b, const Point *pts, size_t n, const
     f->prepare(pts, n);
     fs.push_back(to_c(f->spec()));
   }
+  std::lock_guard<std::mutex> lk(c.mu);
   b.chunk.reserve(n * sizeof(Point));
   b.out.reserve(n * sizeof(Point));
   hip_check(hipMemcpy(b.chunk.p, pts, n * sizeof(Point), hipMemcpyHostToDevice), "copy points");
@@ -1007,7 +1010,6 @@ const void *Filter::upload(const void *host, size_t bytes) const {
 
 size_t Filter::operator()(PointContainer &pc, bool resize) {
   Context &c = ctx();
-  std::lock_guard<std::mutex> lk(c.mu);
   StreamBufs &b = stream_bufs();
   b.all_count = 0;
   const size_t kept = filter_chunk(c, b, pc.data(), pc.size(), {this});
@@ -1092,8 +1094,7 @@ FilterStream::FilterStream(Source *source, Sink *sink, size_t buffer_size, bool 
 size_t FilterStream::run_device(size_t subsample_size, const double **d_points) {
   if (!source_) throw std::invalid_argument("FilterStream: no source");
   Context &c = ctx();
-  std::lock_guard<std::mutex> lk(c.mu);
-  StreamBufs &b = stream_bufs();
+  StreamBufs &b = stream_bufs();  // per thread: only the handle needs the lock
   b.all_count = 0;
   std::vector<Point> buf(buffer_size_);
   long long k;
@@ -1107,6 +1108,7 @@ size_t FilterStream::run_device(size_t subsample_size, const double **d_points) 
   size_t n = b.all_count;
   const double *out = static_cast<const double *>(b.all.p);
   if (subsample_size != 0 && subsample_size < n) {
+    std::lock_guard<std::mutex> lk(c.mu);
     b.sub.reserve(subsample_size * sizeof(Point));
     const uint64_t seed = g_seed.load() ^ (0xd1b54a32d192ed03ULL * (g_draws.fetch_add(1) + 1));
     throw_status(sfbhip_random_subset(c.h, out, (int64_t)n, (int64_t)subsample_size, seed,
