@@ -124,12 +124,11 @@ __device__ __forceinline__ float jl_hermite(const float4 *__restrict__ T, int ni
   return fmaf(f, fmaf(f, fmaf(f, c3, c2), q.y), q.x);
 }
 
-// The same from a node copy in LDS: node i = (j_l(i h), h j_l'(i h)).  No
-// clamping: the caller guarantees 0 <= t and that node floor(t) + 1 is staged
-// (the plan stages floor(k R / h) + 2 nodes past the item's largest argument,
-// and a padding point has r = 0).
-__device__ __forceinline__ float jl_hermite_lds(const float2 *T, double t) {
-  const int i = (int)t;
+// The same from a window of nodes copied into LDS: T[j] = node lo + j, node
+// i = (j_l(i h), h j_l'(i h)).  No clamping: the caller guarantees that nodes
+// floor(t) and floor(t) + 1 are staged.
+__device__ __forceinline__ float jl_hermite_lds(const float2 *T, double t, int lo) {
+  const int i = (int)t - lo;
   const float f = (float)__builtin_amdgcn_fract(t);  // t - floor(t), exact
   const float2 a = T[i], b = T[i + 1];
   const float d = b.x - a.x;
@@ -203,19 +202,29 @@ __global__ __launch_bounds__(256) void sfb_direct_mixed_kernel(
 
   const int nint = ht.nint[l];
   const float4 *__restrict__ T = ht.tab + ht.off[l];
-  extern __shared__ float2 TN[];  // node table of the intervals this item's arguments reach
-  const int ntab = min(nint, it.tab);
-  const bool tl = ntab + 1 <= lds_nodes;  // block-uniform
+  extern __shared__ float2 TN[];  // window of the nodes this item's arguments reach
+  // The item's points are sorted by radius and its rows by k, so every
+  // argument k r / h lies in [k_first r_first, k_last r_last] / h: nodes lo ..
+  // hi (two past the largest argument's interval) are staged when they fit.
+  const double khr0 = kln[it.kofs] * (1.0 / MX_H);
+  const double khr1 = kln[it.kofs + it.nrows - 1] * (1.0 / MX_H);
+  const double rlo = pts.r[it.p0];
+  const int lo = (int)(rlo * khr0);
+  const int nwin = (int)(pts.r[it.p1 - 1] * khr1) + 3 - lo;
+  const bool tl = nwin <= lds_nodes;  // block-uniform
   if (tl) {
-    // FP16 operands carry v * 2^12: the staged nodes carry the (exact) scale
+    // FP16 operands carry v * 2^12: the staged nodes carry the (exact) scale;
+    // nodes past the table (never read) repeat its last node
     const float sc = NP ? MX_SCALE : 1.f;
-    for (int i = tid; i <= ntab; i += 256) {
-      const float4 q = T[i < ntab ? i : ntab - 1];
-      TN[i] = i < ntab ? make_float2(q.x * sc, q.y * sc) : make_float2(q.z * sc, q.w * sc);
+    for (int j = tid; j < nwin; j += 256) {
+      const int i = lo + j;
+      const float4 q = T[i < nint ? i : nint - 1];
+      TN[j] = i < nint ? make_float2(q.x * sc, q.y * sc) : make_float2(q.z * sc, q.w * sc);
     }
   }
-  // rows past nrows get k = 0, so their arguments stay at 0 (their results are not read)
-  const double khr = rowok ? kln[it.kofs + row] * (1.0 / MX_H) : 0.0;
+  // rows past nrows take the first row's k and padding points the first
+  // radius, so their arguments stay inside the window (their results are not read)
+  const double khr = rowok ? kln[it.kofs + row] * (1.0 / MX_H) : khr0;
   const double2 sect = leg.sect[l];
   {
     const double2 *gc = leg.coef + leg.coef_off[l];
@@ -231,9 +240,9 @@ __global__ __launch_bounds__(256) void sfb_direct_mixed_kernel(
   for (int64_t base = it.p0; base < it.p1; base += MX_TP) {
     const int64_t gp = base + tid;
     const bool pok = gp < it.p1;
-    // a padding point gets r = 0: its Bessel arguments stay inside the table, and
-    // its Y* columns are zero, so it adds nothing
-    rs[tid] = pok ? pts.r[gp] : 0.0;
+    // a padding point gets the item's first radius: its Bessel arguments stay
+    // inside the window, and its Y* columns are zero, so it adds nothing
+    rs[tid] = pok ? pts.r[gp] : rlo;
     __syncthreads();  // rs ready; previous tile's readers done
 
     // ---- 1. Bessel values into A-operand registers
@@ -248,8 +257,8 @@ __global__ __launch_bounds__(256) void sfb_direct_mixed_kernel(
 #pragma unroll
             for (int e = 0; e < 8; e += 2) {
               const double r0 = rs[32 * s + 8 * g + e], r1 = rs[32 * s + 8 * g + e + 1];
-              const float v0 = jl_hermite_lds(TN, r0 * khr);
-              const float v1 = jl_hermite_lds(TN, r1 * khr);
+              const float v0 = jl_hermite_lds(TN, r0 * khr, lo);
+              const float v1 = jl_hermite_lds(TN, r1 * khr, lo);
               if constexpr (NP == 2)
                 split_pair(v0, v1, pc[0][e / 2], pc[1][e / 2]);
               else
@@ -265,7 +274,7 @@ __global__ __launch_bounds__(256) void sfb_direct_mixed_kernel(
 #pragma unroll
           for (int s = 0; s < MX_TP / 4; ++s) {
             const double r = rs[4 * s + g];
-            af[s] = jl_hermite_lds(TN, r * khr);
+            af[s] = jl_hermite_lds(TN, r * khr, lo);
           }
         }
       } else {
