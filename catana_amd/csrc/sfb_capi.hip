@@ -782,6 +782,9 @@ sfbhip_status sfbhip_direct(sfbhip_handle *h, const double *d_points, int64_t n_
   const bool mixed = p->precision == SFBHIP_PREC_MIXED;
   if (mixed && p->lmax > MX_LMAX)
     return fail(SFBHIP_NOT_SUPPORTED, "mixed path supports lmax <= " + std::to_string(MX_LMAX) + "; use precision FP64");
+  // the mixed kernels rely on the radius order (their Bessel windows), which the sort provides up to 2^31 - 1 points
+  if (mixed && n_points > INT32_MAX)
+    return fail(SFBHIP_NOT_SUPPORTED, "mixed path: more than 2^31 - 1 points; split the catalogue or use precision FP64");
   if (mixed) CHK(sort_points_by_radius(h, pts, st, &pts));
   const int lsplit = mixed ? std::min(p->lmax, std::max(0, p->mixed_fp64_below)) : p->lmax;
   // FP64 kernel's j_l: in FP64 precision the reference's choice (exact or its
