@@ -618,8 +618,9 @@ sfbhip_status launch_plan_f64(const DirectPlan &pl, DirectItem *ditems, size_t *
 int lds_reach(const DirectItem &it, const std::vector<int32_t> &nint) { return std::min(it.tab, nint[it.l]); }
 
 // One launch per class; its dynamic LDS holds the largest node window among
-// the class's items, at most MX_LDS_NODES (an item's window, from its first
-// to its last argument, is at most its reach from 0 plus three nodes); an item
+// the class's items, at most MX_LDS_NODES (an item's window, from one node
+// below its first argument to its last, is at most its reach from 0 plus four
+// nodes); an item
 // whose window does not fit reads global memory.
 sfbhip_status launch_plan_mixed(const DirectPlan &pl, const std::vector<int32_t> &nint, DirectItem *ditems,
                                 size_t *off, hipStream_t st, const PointsSoA &pts, const double *dk, int nmax,
@@ -630,7 +631,7 @@ sfbhip_status launch_plan_mixed(const DirectPlan &pl, const std::vector<int32_t>
     if (!n) continue;
     int nodes = 0;
     for (const DirectItem &it : v)
-      nodes = std::max(nodes, std::min(lds_reach(it, nint) + 3, MX_LDS_NODES));
+      nodes = std::max(nodes, std::min(lds_reach(it, nint) + 4, MX_LDS_NODES));
     const DirectItem *di = ditems + *off;
     switch (ci) {
       case 0: CHK(launch_mixed_ops<1>(ops, n, di, st, pts, dk, nmax, lt, ht, nodes, partial)); break;

@@ -209,7 +209,9 @@ __global__ __launch_bounds__(256) void sfb_direct_mixed_kernel(
   const double khr0 = kln[it.kofs] * (1.0 / MX_H);
   const double khr1 = kln[it.kofs + it.nrows - 1] * (1.0 / MX_H);
   const double rlo = pts.r[it.p0];
-  const int lo = (int)(rlo * khr0);
+  // one node of margin below: a batched catalogue is sorted by a key that drops
+  // the lowest 7 bits of r, so the first radius may exceed the smallest by an ulp
+  const int lo = max((int)(rlo * khr0) - 1, 0);
   const int nwin = (int)(pts.r[it.p1 - 1] * khr1) + 3 - lo;
   const bool tl = nwin <= lds_nodes;  // block-uniform
   if (tl) {

@@ -47,7 +47,7 @@ __global__ void radius_keys_kernel(const double *__restrict__ r, int64_t n, unsi
                                    int32_t *__restrict__ idx) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  keys[i] = (unsigned long long)__double_as_longlong(fmax(r[i], 0.0));
+  keys[i] = (unsigned long long)__double_as_longlong(r[i] > 0.0 ? r[i] : 0.0);  // -0.0 (and NaN) key as +0.0
   idx[i] = (int32_t)i;
 }
 
@@ -57,7 +57,7 @@ __global__ void radius_keys_cat_kernel(const double *__restrict__ r, int64_t n, 
                                        unsigned long long *__restrict__ keys, int32_t *__restrict__ idx) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const unsigned long long b = (unsigned long long)__double_as_longlong(fmax(r[base + i], 0.0));
+  const unsigned long long b = (unsigned long long)__double_as_longlong(r[base + i] > 0.0 ? r[base + i] : 0.0);
   keys[base + i] = ((unsigned long long)cat << 57) | (b >> 7);
   idx[base + i] = (int32_t)(base + i);
 }

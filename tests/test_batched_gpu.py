@@ -78,3 +78,44 @@ def test_batched_errors(handle):
     bad["points"] = torch.zeros((0, 3), dtype=torch.float64, device="cuda")
     with pytest.raises(ValueError, match="catalogue 1"):
         capi.sfb_direct_batched_torch(handle, [cats[0], bad], precision=capi.PREC_FP64)
+
+
+def test_batched_mixed_first_radius_not_smallest(handle):
+    # the batched sort orders by r with its lowest 7 bits dropped, so within a
+    # key group points keep their input order.  Here the first key group holds
+    # radii on both sides of a Bessel node of the l = 4, n = 0 row (t = k r / h
+    # an integer, h = 1/16) and the input is in descending order: the item's
+    # first point then lies above the node while the group's smallest lies
+    # below.  The mixed kernel's node window must still cover the smallest.
+    lmax, nmax, rmax = 8, 6, 500.0
+    ball = workloads.uniform_ball(2000, rmax, seed=77)
+    k, _, _ = capi.sfb_direct_torch(handle, torch.from_numpy(ball).cuda(), lmax, nmax, rmax,
+                                    precision=capi.PREC_FP64)
+    khr = float(k.cpu().numpy()[4, 0]) * 16.0
+    for m in range(40, 400):
+        r = m / khr
+        while r * khr < m:
+            r = np.nextafter(r, np.inf)
+        while np.nextafter(r, -np.inf) * khr >= m:
+            r = np.nextafter(r, -np.inf)
+        b = int(np.float64(r).view(np.int64))  # smallest radius with t >= m
+        if b % 128 >= 16 and r < 0.9 * rmax:
+            break
+    g = b - b % 128
+    bits = np.arange(b + 40, g - 1, -1, dtype=np.int64)  # descending; the lowest group starts at g
+    radii = bits.view(np.float64)
+    assert radii[-1] * khr < m <= radii[0] * khr
+    rng = np.random.default_rng(5)
+    pts = np.stack([radii, rng.uniform(0.1, 3.0, radii.size), rng.uniform(0.0, 6.2, radii.size)], axis=1)
+    print(f"case: batched mixed, {radii.size} radii around node {m} of l=4 n=0", flush=True)
+    cats = [dict(points=torch.from_numpy(pts).cuda(), lmax=lmax, nmax=nmax, rmax=rmax, k_max=0.0,
+                 window_volume=0.0),
+            dict(points=torch.from_numpy(ball).cuda(), lmax=lmax, nmax=nmax, rmax=rmax, k_max=0.0,
+                 window_volume=0.0)]
+    r64 = capi.sfb_direct_batched_torch(handle, cats, store_flmn=True, precision=capi.PREC_FP64)
+    rmx = capi.sfb_direct_batched_torch(handle, cats, store_flmn=True, precision=capi.PREC_MIXED)
+    torch.cuda.synchronize()
+    for (_, _, f64), (_, _, fm) in zip(r64, rmx):
+        a = capi.split_flmn(fm.cpu().numpy(), lmax, nmax)
+        e = capi.split_flmn(f64.cpu().numpy(), lmax, nmax)
+        assert _flmn_err(a[2:], e[2:]) < 1e-6

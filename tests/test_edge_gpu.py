@@ -95,3 +95,18 @@ def test_error_paths(handle):
     # the handle stays usable after errors
     _, c, _ = capi.sfb_direct_torch(handle, ok, 4, 2, 10.0, bessel=capi.BESSEL_EXACT)
     assert torch.isfinite(c).all()
+
+
+@pytest.mark.parametrize("precision", [capi.PREC_FP64, capi.PREC_MIXED])
+def test_negative_zero_radius(handle, precision):
+    # r = -0.0 is a valid radius (it compares equal to 0): it sorts with +0.0,
+    # and the catalogue gives the coefficients of the same catalogue with +0.0
+    pts = workloads.uniform_ball(3000, 400.0, seed=8)
+    pts[::500, 0] = 0.0
+    neg = pts.copy()
+    neg[::500, 0] = -0.0
+    print(f"case: r = -0.0, precision {precision}", flush=True)
+    _, c0, f0 = _run(handle, pts, 20, 10, 400.0, precision=precision)
+    _, c1, f1 = _run(handle, neg, 20, 10, 400.0, precision=precision)
+    np.testing.assert_allclose(c1, c0, rtol=1e-13, atol=0)
+    assert _flmn_err(f1, f0) < 1e-13
