@@ -183,9 +183,12 @@ __global__ __launch_bounds__(256) void sfb_direct_f64_kernel(
             gx.rescale();
           }
         } else {
-          g.init(l, pts.x[gp], pts.s[gp], pts.phi[gp], pts.cphi[gp], pts.sphi[gp], sect);
-          for (int m = l; m >= F64_MC * it.qhi; --m) {
-            g.step(coef[m]);
+          // the azimuth starts at the top of the highest chunk; the Legendre
+          // pair alone is taken there from m = l
+          const int mstart = min(l, F64_MC * it.qhi - 1);
+          g.init(l, mstart, pts.x[gp], pts.s[gp], pts.phi[gp], pts.cphi[gp], pts.sphi[gp], sect);
+          for (int m = l; m > mstart; --m) {
+            g.step_p(coef[m]);
             g.rescale();
           }
         }

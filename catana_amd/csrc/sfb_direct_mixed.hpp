@@ -313,9 +313,16 @@ __global__ __launch_bounds__(256) void sfb_direct_mixed_kernel(
     // ---- 2. Legendre/azimuth rows of this thread's point, chunks from high m to low m
     YlmDown y;
     if (pok) {
-      y.init(l, pts.x[gp], pts.s[gp], pts.phi[gp], pts.cphi[gp], pts.sphi[gp], sect);
-      for (int m = l; m > l - MX_MC * it.qlo; --m) {  // down to the top of the first chunk
-        y.step(cf[m]);
+      // the azimuth starts at the top of the first chunk; the Legendre pair
+      // alone is taken there from m = l (16 qlo steps, renormalised every four
+      // as in the emitting loop below: four steps grow |v| by at most 2^340)
+      const int mstart = l - MX_MC * it.qlo;
+      y.init(l, mstart, pts.x[gp], pts.s[gp], pts.phi[gp], pts.cphi[gp], pts.sphi[gp], sect);
+      for (int m = l; m > mstart; m -= 4) {
+        y.step_p(cf[m]);
+        y.step_p(cf[m - 1]);
+        y.step_p(cf[m - 2]);
+        y.step_p(cf[m - 3]);
         y.rescale_nb();
       }
     } else {

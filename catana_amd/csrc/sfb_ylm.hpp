@@ -27,6 +27,13 @@ struct YlmDown {
 
   __device__ __forceinline__ void init(int l, double x, double s, double phi, double cph_, double sph_,
                                        double2 sect) {
+    init(l, l, x, s, phi, cph_, sph_, sect);
+  }
+
+  // The Legendre pair at m = l and the azimuth at m = m0 <= l: for a caller
+  // that first takes the Legendre pair alone down to m0 (step_p).
+  __device__ __forceinline__ void init(int l, int m0, double x, double s, double phi, double cph_, double sph_,
+                                       double2 sect) {
     const double se = fmax(s, 0x1p-80);
     cot = x / se;
     cph = cph_;
@@ -50,9 +57,9 @@ struct YlmDown {
     v = m * sect.x;
     E = e + (int)sect.y;
     vp = 0.0;
-    // e^{i l phi}; l*phi is split exactly into hi + lo
-    const double hi = (double)l * phi;
-    const double lo = fma((double)l, phi, -hi);
+    // e^{i m0 phi}; m0*phi is split exactly into hi + lo
+    const double hi = (double)m0 * phi;
+    const double lo = fma((double)m0, phi, -hi);
     double S, C;
     sincos(hi, &S, &C);
     cm = C - lo * S;
@@ -81,6 +88,13 @@ struct YlmDown {
     sp = sm;
     cm = cn;
     sm = sn;
+  }
+
+  // m -> m-1 for the Legendre pair alone (the azimuth stays where it is)
+  __device__ __forceinline__ void step_p(double2 ab) {
+    const double vn = -fma(ab.x * cot, v, ab.y * vp);
+    vp = v;
+    v = vn;
   }
 
   // branch-free renormalisation (same effect as rescale())
