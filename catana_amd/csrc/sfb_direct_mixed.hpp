@@ -58,12 +58,21 @@ typedef _Float16 h8 __attribute__((ext_vector_type(8)));
 typedef _Float16 h2 __attribute__((ext_vector_type(2)));
 typedef float f2 __attribute__((ext_vector_type(2)));
 
+// a - (float)h.x and b - (float)h.y, each in one v_fma_mix_f32 (the FP16
+// operand read in place; the remainder of a rounding is exact in FP32)
+__device__ __forceinline__ f2 rem_f16(float a, float b, h2 h) {
+  const unsigned hb = __builtin_bit_cast(unsigned, h);
+  float ra, rb;
+  asm("v_fma_mix_f32 %0, %1, 1.0, -%2 op_sel_hi:[0,0,1]" : "=v"(ra) : "v"(a), "v"(hb));
+  asm("v_fma_mix_f32 %0, %1, 1.0, -%2 op_sel:[0,0,1] op_sel_hi:[0,0,1]" : "=v"(rb) : "v"(b), "v"(hb));
+  return (f2){ra, rb};
+}
+
 // (a, b) -> FP16 pairs (hi, lo) with a ~= hi.x + lo.x, b ~= hi.y + lo.y: two
 // v_cvt_pk_f16_f32 (round to nearest) and the FP32 remainders.
 __device__ __forceinline__ void split_pair(float a, float b, h2 &hi, h2 &lo) {
   hi = __builtin_convertvector((f2){a, b}, h2);
-  const f2 back = __builtin_convertvector(hi, f2);
-  lo = __builtin_convertvector((f2){a - back.x, b - back.y}, h2);
+  lo = __builtin_convertvector(rem_f16(a, b, hi), h2);
 }
 
 // (a, b) -> three FP16 pairs with a = p0.x + p1.x + p2.x (and b likewise) to
@@ -118,23 +127,24 @@ __device__ __forceinline__ float jl_hermite(const float4 *__restrict__ T, int ni
   i = i > 0 ? i : 0;
   const float f = (float)(t - (double)i);
   const float4 q = T[i];
+  // p(f) = j0 + f (s0 + f (c2 + f c3)) with d = j1 - j0, c3 = s0 + s1 - 2 d,
+  // c2 = 3 d - 2 s0 - s1, and c2 + f c3 = (d - s0) + (f - 1) c3
   const float d = q.z - q.x;
-  const float c2 = 3.f * d - 2.f * q.y - q.w;
-  const float c3 = -2.f * d + q.y + q.w;
-  return fmaf(f, fmaf(f, fmaf(f, c3, c2), q.y), q.x);
+  const float c3 = fmaf(-2.f, d, q.y + q.w);
+  return fmaf(f, fmaf(f, fmaf(f - 1.f, c3, d - q.y), q.y), q.x);
 }
 
 // The same from a window of nodes copied into LDS: T[j] = node lo + j, node
-// i = (j_l(i h), h j_l'(i h)).  No clamping: the caller guarantees that nodes
-// floor(t) and floor(t) + 1 are staged.
-__device__ __forceinline__ float jl_hermite_lds(const float2 *T, double t, int lo) {
-  const int i = (int)t - lo;
+// i = (j_l(i h), h j_l'(i h)), at t = r khr - lo (one rounding).  No clamping:
+// the caller guarantees 0 <= t and that nodes floor(t) and floor(t) + 1 are staged.
+__device__ __forceinline__ float jl_hermite_lds(const float2 *T, double r, double khr, double lo) {
+  const double t = fma(r, khr, -lo);
+  const int i = (int)t;
   const float f = (float)__builtin_amdgcn_fract(t);  // t - floor(t), exact
   const float2 a = T[i], b = T[i + 1];
   const float d = b.x - a.x;
-  const float c2 = 3.f * d - 2.f * a.y - b.y;
-  const float c3 = -2.f * d + a.y + b.y;
-  return fmaf(f, fmaf(f, fmaf(f, c3, c2), a.y), a.x);
+  const float c3 = fmaf(-2.f, d, a.y + b.y);
+  return fmaf(f, fmaf(f, fmaf(f - 1.f, c3, d - a.y), a.y), a.x);
 }
 
 // Write Y*_lm of the generator's current m into local columns (2mm, 2mm+1)
@@ -212,6 +222,7 @@ __global__ __launch_bounds__(256) void sfb_direct_mixed_kernel(
   // one node of margin below: a batched catalogue is sorted by a key that drops
   // the lowest 7 bits of r, so the first radius may exceed the smallest by an ulp
   const int lo = max((int)(rlo * khr0) - 1, 0);
+  const double lod = (double)lo;
   const int nwin = (int)(pts.r[it.p1 - 1] * khr1) + 3 - lo;
   const bool tl = nwin <= lds_nodes;  // block-uniform
   if (tl) {
@@ -259,8 +270,8 @@ __global__ __launch_bounds__(256) void sfb_direct_mixed_kernel(
 #pragma unroll
             for (int e = 0; e < 8; e += 2) {
               const double r0 = rs[32 * s + 8 * g + e], r1 = rs[32 * s + 8 * g + e + 1];
-              const float v0 = jl_hermite_lds(TN, r0 * khr, lo);
-              const float v1 = jl_hermite_lds(TN, r1 * khr, lo);
+              const float v0 = jl_hermite_lds(TN, r0, khr, lod);
+              const float v1 = jl_hermite_lds(TN, r1, khr, lod);
               if constexpr (NP == 2)
                 split_pair(v0, v1, pc[0][e / 2], pc[1][e / 2]);
               else
@@ -276,7 +287,7 @@ __global__ __launch_bounds__(256) void sfb_direct_mixed_kernel(
 #pragma unroll
           for (int s = 0; s < MX_TP / 4; ++s) {
             const double r = rs[4 * s + g];
-            af[s] = jl_hermite_lds(TN, r * khr, lo);
+            af[s] = jl_hermite_lds(TN, r, khr, lod);
           }
         }
       } else {
