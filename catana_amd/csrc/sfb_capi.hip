@@ -288,7 +288,9 @@ int qclass(int q) {
 // Items of a row-tiled direct kernel: groups (l, n-tile), split over point
 // ranges, with m chunks split into passes of at most qmax_cap chunks.
 struct DirectPlan {
-  std::vector<DirectItem> items[6];  // by QMAX class 1,2,4,8,16,32
+  // by QMAX class 1,2,4,8,16,32 (index ci), and for the mixed kernel by point
+  // split KS = 1, 2, 4 (index ci + 6 * ksi)
+  std::vector<DirectItem> items[18];
   std::vector<ReduceGroup> groups;
   int64_t partial_size = 0;
 };
@@ -305,8 +307,10 @@ struct PlanCtx {
   double R = 0.0;               // max(rmax, largest radius)
 };
 
+// ks_cap > 1 (mixed kernel, 64-row items): an item of at most 16 rows goes to
+// point split KS = min(4, ks_cap), of at most 32 rows to KS = min(2, ks_cap).
 void plan_direct(const std::vector<int32_t> &nl, int lmin, int lmax, int64_t npts, int nt, int tp, int mc,
-                 int qmax_cap, int64_t target_items, const PlanCtx &cx, DirectPlan *pl) {
+                 int qmax_cap, int64_t target_items, const PlanCtx &cx, DirectPlan *pl, int ks_cap = 1) {
   int64_t ngroups = 0, passes_total = 0;
   for (int l = lmin; l < lmax; ++l) {
     const int64_t g = cdiv(nl[l], nt);
@@ -356,7 +360,12 @@ void plan_direct(const std::vector<int32_t> &nl, int lmin, int lmax, int64_t npt
           const int qc = qclass(it.qhi - it.qlo);
           int ci = 0;
           while ((1 << ci) < qc) ++ci;
-          pl->items[ci].push_back(it);
+          int ksi = 0;
+          if (ks_cap >= 4 && it.nrows <= 16)
+            ksi = 2;
+          else if (ks_cap >= 2 && it.nrows <= 32)
+            ksi = 1;
+          pl->items[ci + 6 * ksi].push_back(it);
         }
       }
     }
@@ -567,26 +576,35 @@ sfbhip_status ensure_hermite(sfbhip_handle *h, const sfbhip_params *p, double rd
 // block then still fits twice per CU.
 constexpr int MX_LDS_NODES = 4608;
 
-template <int QMAX, int NP>
+template <int QMAX, int NP, int KS = 1>
 sfbhip_status launch_mixed(int nitems, const DirectItem *items, hipStream_t st, const PointsSoA &pts,
                            const double *kln, int nmax, const LegendreTables &lt, const HermiteTables &ht,
                            int lds_nodes, double *partial) {
   const size_t shm = sizeof(float2) * (size_t)lds_nodes;
   if (shm > 0)
-    HIPCHK(hipFuncSetAttribute(reinterpret_cast<const void *>(&sfb_direct_mixed_kernel<QMAX, NP>),
+    HIPCHK(hipFuncSetAttribute(reinterpret_cast<const void *>(&sfb_direct_mixed_kernel<QMAX, NP, KS>),
                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
-  sfb_direct_mixed_kernel<QMAX, NP><<<nitems, 256, shm, st>>>(pts, items, kln, nmax, lt, ht, lds_nodes, partial);
+  sfb_direct_mixed_kernel<QMAX, NP, KS><<<nitems, 256, shm, st>>>(pts, items, kln, nmax, lt, ht, lds_nodes,
+                                                                 partial);
   HIPCHK(hipGetLastError());
   return SFBHIP_OK;
 }
 
-// operand format SFBHIP_OPERANDS_* -> number of FP16 pieces (0: FP32 operands)
+// the point split applies to the split-FP16 operands only
+int mixed_ks_cap(int ops) { return ops == SFBHIP_OPERANDS_SPLIT_FP16 ? 4 : 1; }
+
+// operand format SFBHIP_OPERANDS_* -> number of FP16 pieces (0: FP32 operands);
+// ksi: point split KS = 1 << ksi (split-FP16 only)
 template <int QMAX>
-sfbhip_status launch_mixed_ops(int ops, int nitems, const DirectItem *items, hipStream_t st, const PointsSoA &pts,
-                               const double *kln, int nmax, const LegendreTables &lt, const HermiteTables &ht,
-                               int lds_nodes, double *partial) {
-  if (ops == SFBHIP_OPERANDS_SPLIT_FP16)
+sfbhip_status launch_mixed_ops(int ops, int ksi, int nitems, const DirectItem *items, hipStream_t st,
+                               const PointsSoA &pts, const double *kln, int nmax, const LegendreTables &lt,
+                               const HermiteTables &ht, int lds_nodes, double *partial) {
+  if (ops == SFBHIP_OPERANDS_SPLIT_FP16) {
+    if (ksi == 2) return launch_mixed<QMAX, 2, 4>(nitems, items, st, pts, kln, nmax, lt, ht, lds_nodes, partial);
+    if (ksi == 1) return launch_mixed<QMAX, 2, 2>(nitems, items, st, pts, kln, nmax, lt, ht, lds_nodes, partial);
     return launch_mixed<QMAX, 2>(nitems, items, st, pts, kln, nmax, lt, ht, lds_nodes, partial);
+  }
+  if (ksi != 0) return fail(SFBHIP_NOT_SUPPORTED, "point split planned for an operand format without it");
   if (ops == SFBHIP_OPERANDS_SPLIT3_FP16)
     return launch_mixed<QMAX, 3>(nitems, items, st, pts, kln, nmax, lt, ht, lds_nodes, partial);
   return launch_mixed<QMAX, 0>(nitems, items, st, pts, kln, nmax, lt, ht, lds_nodes, partial);
@@ -625,19 +643,22 @@ int lds_reach(const DirectItem &it, const std::vector<int32_t> &nint) { return s
 sfbhip_status launch_plan_mixed(const DirectPlan &pl, const std::vector<int32_t> &nint, DirectItem *ditems,
                                 size_t *off, hipStream_t st, const PointsSoA &pts, const double *dk, int nmax,
                                 const LegendreTables &lt, const HermiteTables &ht, int ops, double *partial) {
-  for (int ci = 0; ci < 4; ++ci) {
-    const auto &v = pl.items[ci];
+  // classes in the order the items were uploaded (index ci + 6 ksi ascending)
+  for (int idx = 0; idx < 18; ++idx) {
+    const int ci = idx % 6, ksi = idx / 6;
+    const auto &v = pl.items[idx];
     const int n = (int)v.size();
     if (!n) continue;
+    if (ci > 3) return fail(SFBHIP_NOT_SUPPORTED, "mixed plan with more than 8 chunks per item");
     int nodes = 0;
     for (const DirectItem &it : v)
       nodes = std::max(nodes, std::min(lds_reach(it, nint) + 4, MX_LDS_NODES));
     const DirectItem *di = ditems + *off;
     switch (ci) {
-      case 0: CHK(launch_mixed_ops<1>(ops, n, di, st, pts, dk, nmax, lt, ht, nodes, partial)); break;
-      case 1: CHK(launch_mixed_ops<2>(ops, n, di, st, pts, dk, nmax, lt, ht, nodes, partial)); break;
-      case 2: CHK(launch_mixed_ops<4>(ops, n, di, st, pts, dk, nmax, lt, ht, nodes, partial)); break;
-      default: CHK(launch_mixed_ops<8>(ops, n, di, st, pts, dk, nmax, lt, ht, nodes, partial)); break;
+      case 0: CHK(launch_mixed_ops<1>(ops, ksi, n, di, st, pts, dk, nmax, lt, ht, nodes, partial)); break;
+      case 1: CHK(launch_mixed_ops<2>(ops, ksi, n, di, st, pts, dk, nmax, lt, ht, nodes, partial)); break;
+      case 2: CHK(launch_mixed_ops<4>(ops, ksi, n, di, st, pts, dk, nmax, lt, ht, nodes, partial)); break;
+      default: CHK(launch_mixed_ops<8>(ops, ksi, n, di, st, pts, dk, nmax, lt, ht, nodes, partial)); break;
     }
     *off += n;
   }
@@ -821,7 +842,7 @@ sfbhip_status sfbhip_direct(sfbhip_handle *h, const double *d_points, int64_t n_
     // large catalogues get more, smaller items: the launch's tail (one item) stays short
     const int64_t target = std::min<int64_t>(16384, std::max<int64_t>(4096, n_points / 2048));
     plan_direct(h->kc.nl, lsplit, p->lmax, n_points, MX_NT, MX_TP, MX_MC, p->mixed_chunks ? p->mixed_chunks : 4,
-                target, cx, &plmx);
+                target, cx, &plmx, mixed_ks_cap(p->mixed_operands));
   }
   std::vector<ReduceGroup> groups = pllw.groups;
   groups.insert(groups.end(), pl64.groups.begin(), pl64.groups.end());
@@ -1093,7 +1114,7 @@ sfbhip_status sfbhip_direct_batched(sfbhip_handle *h, const sfbhip_catalog *cats
   if (mixed) {
     for (int c = 0; c < ncat; ++c)
       plan_direct(nl[c], std::min(lsplit, cats[c].lmax), cats[c].lmax, cats[c].n_points, MX_NT, MX_TP, MX_MC,
-                  p->mixed_chunks ? p->mixed_chunks : 4, target(c), ctx(c), &plmx);
+                  p->mixed_chunks ? p->mixed_chunks : 4, target(c), ctx(c), &plmx, mixed_ks_cap(p->mixed_operands));
   }
   std::vector<ReduceGroup> groups = pllw.groups;
   groups.insert(groups.end(), pl64.groups.begin(), pl64.groups.end());

@@ -38,6 +38,11 @@
 // only the lowest chunk is partial.  All 256 threads generate the Legendre rows
 // (one point each); wave w owns rows 16w..16w+15 and both 16-column halves of
 // every 32-column chunk.  Waves whose rows are all padding skip their MFMAs.
+// An item of at most 16 (32) rows runs with KS = 4 (2): the waves then split
+// each tile's points instead of the rows -- wave w owns row block w % (4/KS)
+// and the points of part w / (4/KS) -- so that every wave evaluates Bessel
+// values and runs MFMAs; the parts' FP64 sums are added in LDS at the end, in
+// a fixed order.
 #pragma once
 
 #include "sfb_common.hpp"
@@ -177,11 +182,14 @@ __device__ __forceinline__ void emit_y(const YlmDown &y, float *Yf, _Float16 *Yh
 // (dynamic shared memory of lds_nodes nodes, sized for two blocks per CU) when
 // they fit; otherwise the item reads the table from global memory (L1/L2).
 // The choice is per block, so one launch serves both kinds of items.
-template <int QMAX, int NP>
+template <int QMAX, int NP, int KS = 1>
 __global__ __launch_bounds__(256) void sfb_direct_mixed_kernel(
     PointsSoA pts, const DirectItem *__restrict__ items, const double *__restrict__ kln, int nmax,
     LegendreTables leg, HermiteTables ht, int lds_nodes, double *__restrict__ partial) {
   static_assert(NP == 0 || NP == 2 || NP == 3, "operand pieces");
+  static_assert(KS == 1 || ((KS == 2 || KS == 4) && NP != 0), "point split: FP16 operands only");
+  constexpr int RBN = 4 / KS;                // row blocks of 16 per item
+  constexpr int SPW = MX_TP / 32 / KS;       // 32-point blocks of a tile per wave (FP16 operands)
   // row pitch of the Y* chunk (elements): a column starts 4 banks after the
   // previous one, so the 16-byte B-fragment reads of 8 consecutive columns
   // (FP16) and the 4-byte reads of 16 columns x 4 points (FP32) hit distinct banks
@@ -206,8 +214,9 @@ __global__ __launch_bounds__(256) void sfb_direct_mixed_kernel(
   const int lane = tid & 63;
   const int wave = tid >> 6;
   const int g = lane >> 4;
-  const int row = 16 * wave + (lane & 15);
-  const bool wave_rows = 16 * wave < it.nrows;  // wave-uniform
+  const int rb = wave % RBN, kp = wave / RBN;  // row block, part of the tile's points
+  const int row = 16 * rb + (lane & 15);
+  const bool wave_rows = 16 * rb < it.nrows;  // wave-uniform
   const bool rowok = row < it.nrows;
 
   const int nint = ht.nint[l];
@@ -260,16 +269,17 @@ __global__ __launch_bounds__(256) void sfb_direct_mixed_kernel(
 
     // ---- 1. Bessel values into A-operand registers
     float af[NP ? 1 : MX_TP / 4];
-    h8 ah[NA][NP ? MX_TP / 32 : 1];
+    h8 ah[NA][NP ? SPW : 1];
     if (wave_rows) {
       if (tl) {
         if constexpr (NP) {
 #pragma unroll
-          for (int s = 0; s < MX_TP / 32; ++s) {
+          for (int s = 0; s < SPW; ++s) {
+            const int sp = kp * SPW + s;
             h2 pc[3][4];
 #pragma unroll
             for (int e = 0; e < 8; e += 2) {
-              const double r0 = rs[32 * s + 8 * g + e], r1 = rs[32 * s + 8 * g + e + 1];
+              const double r0 = rs[32 * sp + 8 * g + e], r1 = rs[32 * sp + 8 * g + e + 1];
               const float v0 = jl_hermite_lds(TN, r0, khr, lod);
               const float v1 = jl_hermite_lds(TN, r1, khr, lod);
               if constexpr (NP == 2)
@@ -293,11 +303,12 @@ __global__ __launch_bounds__(256) void sfb_direct_mixed_kernel(
       } else {
         if constexpr (NP) {
 #pragma unroll
-          for (int s = 0; s < MX_TP / 32; ++s) {
+          for (int s = 0; s < SPW; ++s) {
+            const int sp = kp * SPW + s;
             h2 pc[3][4];
 #pragma unroll
             for (int e = 0; e < 8; e += 2) {
-              const double r0 = rs[32 * s + 8 * g + e], r1 = rs[32 * s + 8 * g + e + 1];
+              const double r0 = rs[32 * sp + 8 * g + e], r1 = rs[32 * sp + 8 * g + e + 1];
               const float v0 = jl_hermite(T, nint, r0 * khr) * MX_SCALE;
               const float v1 = jl_hermite(T, nint, r1 * khr) * MX_SCALE;
               if constexpr (NP == 2)
@@ -369,8 +380,8 @@ __global__ __launch_bounds__(256) void sfb_direct_mixed_kernel(
             // whose FP32 rounding is correspondingly smaller
             v4f c0 = v4f{0.f, 0.f, 0.f, 0.f}, c1 = v4f{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-            for (int s = 0; s < MX_TP / 32; ++s) {
-              const int o = col * LDP + 32 * s + 8 * g;
+            for (int s = 0; s < SPW; ++s) {
+              const int o = col * LDP + 32 * (kp * SPW + s) + 8 * g;
               h8 b0[NA], b1[NA];
 #pragma unroll
               for (int k = 0; k < NP; ++k) {
@@ -392,7 +403,7 @@ __global__ __launch_bounds__(256) void sfb_direct_mixed_kernel(
               c1 = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[1][s], b1[0], c1, 0, 0, 0);
               a0 = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[0][s], b0[0], a0, 0, 0, 0);
               a1 = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[0][s], b1[0], a1, 0, 0, 0);
-              if ((32 * (s + 1)) % FOLD == 0 && 32 * (s + 1) < MX_TP) {
+              if ((32 * (s + 1)) % FOLD == 0 && s + 1 < SPW) {
 #pragma unroll
                 for (int j = 0; j < 4; ++j) {
                   sum[qq][0][j] += (double)a0[j];
@@ -436,13 +447,44 @@ __global__ __launch_bounds__(256) void sfb_direct_mixed_kernel(
     }
   }
 
+  if constexpr (KS > 1) {
+    // parts 1 .. KS-1 of each row block hand their sums to part 0 through LDS
+    // (the Y* area), one chunk at a time; part 0 adds them in part order
+    double *red = reinterpret_cast<double *>(ybuf);
+    static_assert((KS - 1) * RBN * 64 * 8 * (int)sizeof(double) <= YBYTES, "reduction area");
+#pragma unroll
+    for (int qq = 0; qq < QMAX; ++qq) {
+      if (it.qlo + qq < it.qhi) {  // block-uniform
+        __syncthreads();  // the Y* area is free (last chunk's readers, previous chunk's adders)
+        if (kp > 0 && wave_rows) {
+          double *d = red + (size_t)(((kp - 1) * RBN + rb) * 64 + lane) * 8;
+#pragma unroll
+          for (int c = 0; c < 2; ++c)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) d[4 * c + j] = sum[qq][c][j];
+        }
+        __syncthreads();
+        if (kp == 0 && wave_rows) {
+#pragma unroll
+          for (int k = 1; k < KS; ++k) {
+            const double *d = red + (size_t)(((k - 1) * RBN + rb) * 64 + lane) * 8;
+#pragma unroll
+            for (int c = 0; c < 2; ++c)
+#pragma unroll
+              for (int j = 0; j < 4; ++j) sum[qq][c][j] += d[4 * c + j];
+          }
+        }
+      }
+    }
+  }
+
   // ---- write the item's partial block (64 rows x its columns), FP64
   const double scale = NP ? 1.0 / ((double)MX_SCALE * (double)MX_SCALE) : 1.0;
   double *out = partial + it.out;
 #pragma unroll
   for (int qq = 0; qq < QMAX; ++qq) {
     const int q = it.qlo + qq;
-    if (q < it.qhi) {
+    if (q < it.qhi && kp == 0) {
 #pragma unroll
       for (int c = 0; c < 2; ++c) {
         const int lc = 16 * c + (lane & 15);       // local column: (mm, re/im)
@@ -451,7 +493,7 @@ __global__ __launch_bounds__(256) void sfb_direct_mixed_kernel(
         // v_mfma_f32_16x16x{4f32,32_f16} result: element j of lane i is row 4(i/16) + j, column i%16
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
-          const int orow = 16 * wave + 4 * g + j;
+          const int orow = 16 * rb + 4 * g + j;
           if (col < ncol) out[(size_t)orow * ncol + col] = sum[qq][c][j] * scale;
         }
       }
