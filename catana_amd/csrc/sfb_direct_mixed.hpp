@@ -336,15 +336,13 @@ __global__ __launch_bounds__(256) void sfb_direct_mixed_kernel(
     YlmDown y;
     if (pok) {
       // the azimuth starts at the top of the first chunk; the Legendre pair
-      // alone is taken there from m = l (16 qlo steps, renormalised every four
-      // as in the emitting loop below: four steps grow |v| by at most 2^340)
+      // alone is taken there from m = l (16 qlo steps, renormalised every eight
+      // as in the emitting loop below)
       const int mstart = l - MX_MC * it.qlo;
       y.init(l, mstart, pts.x[gp], pts.s[gp], pts.phi[gp], pts.cphi[gp], pts.sphi[gp], sect);
-      for (int m = l; m > mstart; m -= 4) {
-        y.step_p(cf[m]);
-        y.step_p(cf[m - 1]);
-        y.step_p(cf[m - 2]);
-        y.step_p(cf[m - 3]);
+      for (int m = l; m > mstart; m -= 8) {
+#pragma unroll
+        for (int k = 0; k < 8; ++k) y.step_p(cf[m - k]);
         y.rescale_nb();
       }
     } else {
@@ -364,7 +362,7 @@ __global__ __launch_bounds__(256) void sfb_direct_mixed_kernel(
         for (int mm = 0; mm < MX_MC; ++mm) {
           emit_y<NP, LDP>(y, Yf, Yh, mm, tid);
           y.step(cf[mtop - mm]);
-          if ((mm & 3) == 3) y.rescale_nb();
+          if ((mm & 7) == 7) y.rescale_nb();
         }
         __syncthreads();
         if (wave_rows) {

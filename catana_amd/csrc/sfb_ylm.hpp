@@ -9,7 +9,8 @@
 // Downward in m the wanted solution dominates in the evanescent region
 // (m > l sin t) and is oscillatory below it, so the recurrence is stable.
 // sin^l t underflows FP64 near the poles, so the state carries a binary
-// exponent E (value = v * 2^E) and is renormalised when |v| exceeds 2^600.
+// exponent E (value = v * 2^E) and is renormalised when |v| exceeds 2^600
+// (2^300 in the branch-free form that runs every eight steps).
 // sin t is clamped to >= 2^-80: the rows of such points differ from the
 // pole values by O(l^2 2^-80).
 #pragma once
@@ -97,13 +98,16 @@ struct YlmDown {
     v = vn;
   }
 
-  // branch-free renormalisation (same effect as rescale())
+  // Branch-free renormalisation for callers that renormalise every eight
+  // steps: one step grows max(|v|, |vp|) by at most |a_m cot| + |b_m| < 2^85
+  // (|cot| <= 2^80, a_m <= sqrt(2m) <= 2^5, b_m <= 1), so from |v| <= 2^300
+  // eight steps stay below 2^981.
   __device__ __forceinline__ void rescale_nb() {
-    const bool big = fabs(v) > 0x1p+600;
-    const double f = big ? 0x1p-600 : 1.0;
+    const bool big = fabs(v) > 0x1p+300;
+    const double f = big ? 0x1p-300 : 1.0;
     v *= f;
     vp *= f;
-    E += big ? 600 : 0;
+    E += big ? 300 : 0;
   }
 
   __device__ __forceinline__ void rescale() {
