@@ -80,6 +80,10 @@ def make_points(name):
     raise ValueError(name)
 
 
+# default CPU-baseline subsample per workload (points), about 10-25 s on 16 threads
+CPU_SAMPLE = {"clustered1m": 1 << 19, "lognormal2p25": 16384, "uniform10k": 10000, "batched64": 1 << 17}
+
+
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
@@ -114,7 +118,9 @@ def main():
     ap.add_argument("--chunks", type=int, default=4, choices=[1, 2, 4, 8],
                     help="mixed path: m chunks accumulated per work item")
     ap.add_argument("--fp64-below", type=int, default=2, help="mixed path: l below this run in FP64")
-    ap.add_argument("--cpu-sample", type=int, default=65536)
+    ap.add_argument("--cpu-sample", type=int, default=0,
+                    help="points of the CPU baseline's seeded subsample (0: the workload's default, "
+                         "about 10-25 s of CPU work on 16 threads)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -251,7 +257,8 @@ def main():
     if not args.no_cpu_baseline:
         log("CPU baseline")
         threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-        out["cpu_baseline"] = cpu_baseline(pts, cfg, nl, args.cpu_sample, threads)
+        out["cpu_baseline"] = cpu_baseline(pts, cfg, nl, args.cpu_sample or CPU_SAMPLE.get(args.workload, 65536),
+                                           threads)
     h.close()
     print(json.dumps(out))
 
@@ -304,9 +311,28 @@ def run_healpix(args, cfg):
                      "kernel": "sfb_direct_f64_kernel (pixel mode)", "kernel_ms": kms,
                      "algorithmic_flops_per_launch": flops},
     }
-    # no CPU leg here: the oracle evaluates each needed ring-DFT bin directly,
-    # O(npix (l+1)) per (l, n), far beyond a bounded sample at nside 256
-    out["cpu_baseline"] = None
+    if args.no_cpu_baseline:
+        out["cpu_baseline"] = None
+    else:
+        # the oracle's restatement of _sfb_reverse_fft (every needed ring-DFT bin
+        # evaluated directly) over all points and pixels, for every 16th multipole
+        # from l = 8; pairs/s counts the sampled multipoles' pairs
+        log("CPU baseline")
+        sys.path.insert(0, ROOT)
+        from oracle import oracle as O
+        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        off_h, rad_h = O.pixelize(pts, ns)
+        sample = list(range(8, L, 16))
+        nls = np.zeros_like(nl)
+        nls[sample] = nl[sample]
+        t0 = time.perf_counter()
+        O.sfb_reverse_fft(off_h, rad_h, ns, L, NM, R, nl=nls, interpolated=N > 1000, nthreads=threads)
+        dt = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": N * int(((2 * l + 1) * nls).sum()) / dt,
+                               "unit": "pairs/s (point-mode equivalent)", "cores": threads, "kind": "port",
+                               "sample": f"all {N} points, nside {ns}, multipoles l = {sample} only "
+                                         f"(n_l = {[int(nl[x]) for x in sample]}), reference linear table, "
+                                         f"{dt:.2f} s"}
     h.close()
     print(json.dumps(out))
 
@@ -532,7 +558,7 @@ def run_batched(args):
         c = max(cats, key=lambda c: c["np"].shape[0])
         _, nl = h.kln(capi.make_params(c["lmax"], c["nmax"], c["rmax"], k_max=c["k_max"]))
         out["cpu_baseline"] = cpu_baseline(c["np"], dict(lmax=c["lmax"], nmax=c["nmax"], rmax=c["rmax"]), nl,
-                                           args.cpu_sample, threads)
+                                           args.cpu_sample or CPU_SAMPLE["batched64"], threads)
         out["cpu_baseline"]["sample"] += " (largest catalogue of the batch)"
     h.close()
     print(json.dumps(out))
