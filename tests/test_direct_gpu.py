@@ -121,6 +121,24 @@ def test_mixed_vs_fp64_clustered_high_l(handle):
     assert _flmn_err(fm, f64) < 1e-6
 
 
+def test_mixed_vs_fp64_two_row_groups(handle):
+    # n_l up to 76 > 64: multipoles with a second 64-row group (its 12 rows run
+    # with the tile's points split four ways) and, for l >= 64, items whose m
+    # chunks start below m = l
+    pts = workloads.uniform_ball(3000, 600.0, seed=26)
+    print("case: mixed vs fp64, uniform N=3000 lmax=80 k_max=0.4 rmax=600 (n_l up to 76)", flush=True)
+    kw = dict(k_max=0.4, bessel=capi.BESSEL_EXACT)
+    k64, c64, f64 = _run(handle, pts, 80, 80, 600.0, precision=capi.PREC_FP64, **kw)
+    km, cm, fm = _run(handle, pts, 80, 80, 600.0, precision=capi.PREC_MIXED, **kw)
+    assert np.count_nonzero(c64[:, 64]) > 0  # some l has a 65th mode
+    np.testing.assert_array_equal(km, k64)
+    rel = _rel(cm, c64)
+    print("max rel c_ln", rel.max(), "median", np.median(rel))
+    assert rel.max() < 1e-6
+    assert _flmn_err(fm, f64) < 1e-6
+    assert np.all((cm == 0) == (c64 == 0))
+
+
 def test_mixed_fp64_multipoles_tables(handle):
     # mixed_fp64_below = lmax: every l in FP64 MFMA kernels with the quintic j_l
     # table (error ~1e-10 of the amplitude) -- l < 8 in the low-l kernel,
