@@ -110,3 +110,38 @@ def test_negative_zero_radius(handle, precision):
     _, c1, f1 = _run(handle, neg, 20, 10, 400.0, precision=precision)
     np.testing.assert_allclose(c1, c0, rtol=1e-13, atol=0)
     assert _flmn_err(f1, f0) < 1e-13
+
+
+@pytest.mark.parametrize("precision", [capi.PREC_FP64, capi.PREC_MIXED])
+def test_deterministic_run_to_run(handle, precision):
+    # partial blocks are summed in a fixed order (no atomics): repeated calls,
+    # and a second handle, give bit-identical k_ln, c_ln and f_lmn
+    pts = workloads.clustered_ball(300, 700.0, mean_children=30, sigma=6.0, seed=31)
+    print(f"case: determinism, N={pts.shape[0]}, precision {precision}", flush=True)
+    kw = dict(k_max=0.25, precision=precision)
+    k0, c0, f0 = _run(handle, pts, 70, 70, 700.0, **kw)
+    k1, c1, f1 = _run(handle, pts, 70, 70, 700.0, **kw)
+    h2 = capi.Handle(0)
+    try:
+        k2, c2, f2 = _run(h2, pts, 70, 70, 700.0, **kw)
+    finally:
+        h2.close()
+    for k, c, f in ((k1, c1, f1), (k2, c2, f2)):
+        np.testing.assert_array_equal(k, k0)
+        np.testing.assert_array_equal(c, c0)
+        for a, b in zip(f, f0):
+            np.testing.assert_array_equal(a, b)
+
+
+def test_batched_deterministic(handle):
+    cats = []
+    for i, (n, L, R) in enumerate([(4000, 30, 400.0), (900, 50, 900.0)]):
+        p = workloads.uniform_ball(n, R, seed=60 + i)
+        cats.append(dict(points=torch.from_numpy(p).cuda(), lmax=L, nmax=40, rmax=R, k_max=0.12))
+    print("case: batched determinism", flush=True)
+    r0 = capi.sfb_direct_batched_torch(handle, cats, store_flmn=True, precision=capi.PREC_MIXED)
+    r1 = capi.sfb_direct_batched_torch(handle, cats, store_flmn=True, precision=capi.PREC_MIXED)
+    torch.cuda.synchronize()
+    for a, b in zip(r0, r1):
+        for x, y in zip(a, b):
+            assert torch.equal(x, y)
