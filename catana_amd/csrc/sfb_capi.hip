@@ -1283,6 +1283,26 @@ sfbhip_status sfbhip_healpix(sfbhip_handle *h, const int64_t *d_offsets, const d
   pix.off = d_offsets;
   pix.radii = d_radii;
   pix.n = npix;
+  {
+    // contraction order of the pixels: by point count (stable sort), so that
+    // the lanes of a wave, one pixel each, loop over about equally many points
+    uint32_t *ck0, *ck1;
+    int32_t *pv0, *pv1;
+    CHK(get_buf(h, "pix_ck0", (size_t)npix, &ck0));
+    CHK(get_buf(h, "pix_ck1", (size_t)npix, &ck1));
+    CHK(get_buf(h, "pix_pv0", (size_t)npix, &pv0));
+    CHK(get_buf(h, "pix_pv1", (size_t)npix, &pv1));
+    pixel_count_keys_kernel<<<cdiv(npix, 256), 256, 0, st>>>(d_offsets, npix, ck0, pv0);
+    HIPCHK(hipGetLastError());
+    int end_bit = 1;
+    while (end_bit < 32 && (1LL << end_bit) <= npts) ++end_bit;
+    size_t tmp_bytes = 0;
+    HIPCHK(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_bytes, ck0, ck1, pv0, pv1, (int)npix, 0, end_bit, st));
+    char *tmp;
+    CHK(get_buf(h, "pix_ptmp", std::max<size_t>(tmp_bytes, 1), &tmp));
+    HIPCHK(hipcub::DeviceRadixSort::SortPairs(tmp, tmp_bytes, ck0, ck1, pv0, pv1, (int)npix, 0, end_bit, st));
+    pix.perm = pv1;
+  }
   pix.alias_compat = p->alias_compat;
   const double norm = norm_factor(p, (double)npts);
   double *dk;

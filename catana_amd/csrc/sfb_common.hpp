@@ -31,6 +31,7 @@ struct PixelsSoA {
   const int32_t *np;
   const int64_t *off;
   const double *radii;
+  const int32_t *perm;  // contraction order: slot i holds pixel perm[i]
   int64_t n;  // pixels
   int32_t alias_compat;
 };

@@ -8,6 +8,9 @@
 // product over pixels with F_l(p, n) = sum_{r in p} j_l(k_ln r) in place of the
 // Bessel value and the pixel's Y* (YlmDownPix) in place of the point's.
 //
+// Pixel slots are taken in the order pix.perm (pixels sorted by point count):
+// a lane loops over its pixel's points, so the 64 pixels of a wave should
+// hold about equally many.
 // A work item is (l, 32 rows of n, a range of points or pixels).  Per 128-wide
 // tile the workgroup
 //   1. evaluates the 32 x 128 Bessel (or pixel-sum) tile into LDS (all 256 threads),
@@ -153,8 +156,9 @@ __global__ __launch_bounds__(256) void sfb_direct_f64_kernel(
       if (gp < it.p1) {
         int64_t q0 = gp, q1 = gp + 1;
         if constexpr (PIX) {
-          q0 = pix.off[gp];
-          q1 = pix.off[gp + 1];
+          const int64_t pp = pix.perm[gp];
+          q0 = pix.off[pp];
+          q1 = pix.off[pp + 1];
         }
         for (int64_t q = q0; q < q1; ++q) {
           const double r = PIX ? pix.radii[q] : pts.r[q];
@@ -176,8 +180,9 @@ __global__ __launch_bounds__(256) void sfb_direct_f64_kernel(
     if (ygen) {
       if (gp < it.p1) {
         if constexpr (PIX) {
-          gx.init(l, pix.x[gp], pix.s[gp], pix.phi[gp], pix.cphi[gp], pix.sphi[gp], pix.phi2[gp], pix.cphi2[gp],
-                  pix.sphi2[gp], pix.np[gp], pix.alias_compat != 0, sect);
+          const int64_t pp = pix.perm[gp];
+          gx.init(l, pix.x[pp], pix.s[pp], pix.phi[pp], pix.cphi[pp], pix.sphi[pp], pix.phi2[pp], pix.cphi2[pp],
+                  pix.sphi2[pp], pix.np[pp], pix.alias_compat != 0, sect);
           for (int m = l; m >= F64_MC * it.qhi; --m) {
             gx.step(coef[m]);
             gx.rescale();

@@ -147,4 +147,17 @@ __global__ void csr_offsets_kernel(const uint32_t *__restrict__ keys, int64_t n,
   off[p] = lo;
 }
 
+// Sort keys of the pixelised contraction's pixel order: the point count of
+// pixel p (a stable sort keeps ties in pixel order).  Each lane of the
+// contraction loops over its pixel's points, so a wave of pixels with about
+// equal counts does not wait on its fullest pixel.
+__global__ void pixel_count_keys_kernel(const int64_t *__restrict__ off, int64_t npix, uint32_t *__restrict__ keys,
+                                        int32_t *__restrict__ idx) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= npix) return;
+  const int64_t c = off[p + 1] - off[p];
+  keys[p] = (uint32_t)(c < 0xffffffffLL ? c : 0xffffffffLL);
+  idx[p] = (int32_t)p;
+}
+
 }  // namespace sfb
