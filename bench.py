@@ -130,8 +130,8 @@ def main():
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
-        print("bench.py: this build runs on one GPU only (the SFB path shards by points with one "
-              "all-reduce of f_lmn; see DESIGN.md); --gpus must be 1", file=sys.stderr)
+        print("bench.py: runs on one GPU only (across GPUs the SFB path would shard the points "
+              "and sum f_lmn with one all-reduce); --gpus must be 1", file=sys.stderr)
         sys.exit(2)
 
     import torch

@@ -130,7 +130,7 @@ def test_reverse_paths_vs_oracle():
         assert _flmn_err(r.f_lmn, fo) < 1e-10
 
 
-def test_north_star_entry_points():
+def test_k_max_entry_points():
     pts = workloads.uniform_ball(20000, 500.0, seed=9)
     pc = catana.PointContainer(pts)
     lmax, k_max, rmax, nmax, vol = 16, 0.1, 500.0, 16, 3.1e8
