@@ -61,7 +61,7 @@ struct Context {
 };
 
 // Never destroyed: at process exit the HIP runtime may already be gone, and
-// the driver releases the device memory anyway.
+// the operating system's GPU driver releases the device memory anyway.
 Context &ctx() {
   static Context *c = new Context;
   return *c;
