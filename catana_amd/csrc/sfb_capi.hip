@@ -599,6 +599,15 @@ int64_t mixed_target_items(int64_t n_points) {
   return std::min<int64_t>(131072, std::max<int64_t>(16384, n_points / 256));
 }
 
+// The same for the FP64 kernel: its items cost in proportion to l + 1 (all m
+// chunks in one item), so many short items keep the l = lmax - 1 items from
+// forming the tail.  Measured: 1M points 43.6 ms (4096 items) -> 37.3 ms
+// (16384); HEALPix nside 256 (786,432 pixels) 196 -> 126 (16384) -> 107 ms
+// (65536 and more).
+int64_t f64_target_items(int64_t n_points, bool pixels) {
+  return std::min<int64_t>(131072, std::max<int64_t>(16384, pixels ? n_points / 8 : n_points / 64));
+}
+
 // the point split applies to the split-FP16 operands only
 int mixed_ks_cap(int ops) { return ops == SFBHIP_OPERANDS_SPLIT_FP16 ? 4 : 1; }
 
@@ -845,7 +854,8 @@ sfbhip_status sfbhip_direct(sfbhip_handle *h, const double *d_points, int64_t n_
     plan_direct(h->kc.nl, 0, llow, n_points, LW_ROWS, LW_TILE, F64_MC, 1, 256 * (int64_t)llow * (64 / LW_ROWS), cx,
                 &pllw);
   pl64.partial_size = pllw.partial_size;
-  plan_direct(h->kc.nl, llow, lsplit, n_points, F64_NT, F64_TP, F64_MC, 16, 4096, cx, &pl64);
+  plan_direct(h->kc.nl, llow, lsplit, n_points, F64_NT, F64_TP, F64_MC, 16, f64_target_items(n_points, false), cx,
+              &pl64);
   plmx.partial_size = pl64.partial_size;
   if (mixed) {
     const int64_t target = mixed_target_items(n_points);
@@ -1322,7 +1332,7 @@ sfbhip_status sfbhip_healpix(sfbhip_handle *h, const int64_t *d_offsets, const d
   PlanCtx cx;
   cx.nmax = p->nmax;
   cx.norm = norm;
-  plan_direct(h->kc.nl, 0, p->lmax, npix, F64_NT, F64_TP, F64_MC, 16, 4096, cx, &pl);
+  plan_direct(h->kc.nl, 0, p->lmax, npix, F64_NT, F64_TP, F64_MC, 16, f64_target_items(npix, true), cx, &pl);
   double *partial;
   CHK(get_buf(h, "partial", (size_t)std::max<int64_t>(pl.partial_size, 1), &partial));
   std::vector<DirectItem> all;
