@@ -65,7 +65,7 @@ def make_batch(seed=0, ncat=64):
 # rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE on this build, scaled by what the counters read
 # for a 512 MiB device-to-device copy (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0).  Counters cannot
 # be read from inside the timed run, so the figure is carried here per workload.
-PROFILED_TRAFFIC_BYTES = {("clustered1m", "mixed", "split16", 2, 4): 3.132e9,  # (workload, precision, operands,
+PROFILED_TRAFFIC_BYTES = {("clustered1m", "mixed", "split16", 2, 4): 3.551e9,  # (workload, precision, operands,
                           ("ingest2p25",): 1.337e9}                            #  fp64_below, chunks)
 
 
