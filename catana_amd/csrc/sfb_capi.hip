@@ -849,9 +849,10 @@ sfbhip_status sfbhip_direct(sfbhip_handle *h, const double *d_points, int64_t n_
   cx.norm = norm;
   cx.kln = h->kc.kln.data();
   cx.R = std::max(p->rmax, rdata);
-  // the low-l items: 256 point ranges per (l, 16-row group), so the reduction chains stay short
+  // the low-l items: 1024 point ranges per (l, 16-row group) (the partial
+  // reduction sums them in eight chains; 256 ranges: 0.04 ms slower at 1M points)
   if (llow > 0)
-    plan_direct(h->kc.nl, 0, llow, n_points, LW_ROWS, LW_TILE, F64_MC, 1, 256 * (int64_t)llow * (64 / LW_ROWS), cx,
+    plan_direct(h->kc.nl, 0, llow, n_points, LW_ROWS, LW_TILE, F64_MC, 1, 1024 * (int64_t)llow * (64 / LW_ROWS), cx,
                 &pllw);
   pl64.partial_size = pllw.partial_size;
   plan_direct(h->kc.nl, llow, lsplit, n_points, F64_NT, F64_TP, F64_MC, 16, f64_target_items(n_points, false), cx,

@@ -96,9 +96,13 @@ __global__ void bessel_table_kernel(int lmax, int P, const double *__restrict__ 
   y[idx] = sbessel_jl(l, 0.0 + i * dx);
 }
 
-// Sum the partial blocks of a group in split order and scale by norm * k_ln:
-// f[l][m][n] (complex, m >= 0, n < nmax) of the group's catalogue.
-// Block (g, c) sums elements c, c + gridDim.y*256, ... of group g.
+// Sum the partial blocks of a group and scale by norm * k_ln: f[l][m][n]
+// (complex, m >= 0, n < nmax) of the group's catalogue.  Block (g, c) sums
+// elements c, c + gridDim.y*256, ... of group g.  The splits are summed in
+// eight interleaved chains (split s into chain s mod 8, in split order),
+// added pairwise in a fixed order: the result depends only on the plan, and
+// eight loads are in flight per element (a group of few elements and many
+// splits is otherwise one long dependent chain per thread).
 __global__ void reduce_partials_kernel(const ReduceGroup *__restrict__ groups, const double *__restrict__ partial,
                                        const double *__restrict__ kln, double *__restrict__ flmn) {
   const ReduceGroup g = groups[blockIdx.x];
@@ -107,8 +111,16 @@ __global__ void reduce_partials_kernel(const ReduceGroup *__restrict__ groups, c
   for (int e = blockIdx.y * blockDim.x + threadIdx.x; e < total; e += gridDim.y * blockDim.x) {
     const int row = e / ncol, col = e - row * ncol;
     const double *src = partial + g.base + (size_t)row * ncol + col;
-    double acc = 0.0;
-    for (int s = 0; s < g.nsplit; ++s) acc += src[(size_t)s * g.stride];
+    double c[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    int s = 0;
+    for (; s + 8 <= g.nsplit; s += 8) {
+#pragma unroll
+      for (int k = 0; k < 8; ++k) c[k] += src[(size_t)(s + k) * g.stride];
+    }
+#pragma unroll
+    for (int k = 0; k < 7; ++k)
+      if (s + k < g.nsplit) c[k] += src[(size_t)(s + k) * g.stride];
+    const double acc = ((c[0] + c[1]) + (c[2] + c[3])) + ((c[4] + c[5]) + (c[6] + c[7]));
     const int n = g.n0 + row;
     const int m = col >> 1;
     flmn[(g.fofs + (size_t)m * g.nmax + n) * 2 + (col & 1)] = acc * g.norm * kln[g.kofs + row];
