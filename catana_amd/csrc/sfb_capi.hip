@@ -1547,8 +1547,26 @@ sfbhip_status sfbhip_sbessel_integrals(sfbhip_handle *h, int64_t n, const int32_
   int *derr;
   CHK(zero_err(h, st, &derr));
   clear_timings(h);
+  // the triples in order of decreasing piece count (stable radix sort)
+  int32_t *order = nullptr;
+  if (n > 1 && n <= INT32_MAX) {
+    uint32_t *k0, *k1;
+    int32_t *i0;
+    CHK(get_buf(h, "ig_ok0", (size_t)n, &k0));
+    CHK(get_buf(h, "ig_ok1", (size_t)n, &k1));
+    CHK(get_buf(h, "ig_oi0", (size_t)n, &i0));
+    CHK(get_buf(h, "ig_oi1", (size_t)n, &order));
+    sbessel_order_keys_kernel<<<(unsigned)cdiv(n, 256), 256, 0, st>>>(n, c.plan, k0, i0);
+    HIPCHK(hipGetLastError());
+    size_t tmp_bytes = 0;
+    HIPCHK(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_bytes, k0, k1, i0, order, (int)n, 0, 32, st));
+    char *tmp;
+    CHK(get_buf(h, "ig_otmp", std::max<size_t>(tmp_bytes, 1), &tmp));
+    HIPCHK(hipcub::DeviceRadixSort::SortPairs(tmp, tmp_bytes, k0, k1, i0, order, (int)n, 0, 32, st));
+  }
   HIPCHK(hipEventRecord(h->ev[0], st));
-  sbessel_integral_kernel<<<(unsigned)cdiv(n, 4), 256, 0, st>>>(n, c.plan, c.zeros, c.nz, rmax, c.gl, W, dout, derr);
+  sbessel_integral_kernel<<<(unsigned)cdiv(n, 4), 256, 0, st>>>(n, c.plan, c.zeros, c.nz, rmax, c.gl, W, order, dout,
+                                                               derr);
   HIPCHK(hipGetLastError());
   HIPCHK(hipEventRecord(h->ev[1], st));
   HIPCHK(hipEventRecord(h->ev[2], st));

@@ -182,12 +182,27 @@ __global__ __launch_bounds__(256) void sbessel_nodes_kernel(int64_t n, const Int
   }
 }
 
+// Sort keys of the order in which the integral kernel takes the triples:
+// most pieces first (ascending keys), so the long triples do not form the tail.
+__global__ void sbessel_order_keys_kernel(int64_t n, const IntegralPlan *__restrict__ plan, uint32_t *__restrict__ keys,
+                                          int32_t *__restrict__ idx) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const int64_t np = plan[t].npieces;
+  keys[t] = 0xffffffffu - (uint32_t)(np < 0xffffffffLL ? np : 0xffffffffLL);
+  idx[t] = (int32_t)t;
+}
+
+// order (nullable): wave slot s integrates triple order[s]; out[t] is written
+// for the triple t itself.
 __global__ __launch_bounds__(256) void sbessel_integral_kernel(int64_t n, const IntegralPlan *__restrict__ plan,
                                                                const double *__restrict__ zeros, int nz, double R,
                                                                const double *__restrict__ gl, WeightRef W,
+                                                               const int32_t *__restrict__ order,
                                                                double *__restrict__ out, int *__restrict__ err) {
-  const int64_t t = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
-  if (t >= n) return;  // whole waves leave together
+  const int64_t slot = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (slot >= n) return;  // whole waves leave together
+  const int64_t t = order ? (int64_t)order[slot] : slot;
   const int lane = threadIdx.x & 63, q = lane & 15, grp = lane >> 4;
   const IntegralPlan P = plan[t];
   const double *z = zeros + (size_t)P.l * nz;
