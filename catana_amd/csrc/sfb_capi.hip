@@ -77,6 +77,11 @@ struct sfbhip_handle {
   std::map<std::string, DevBuf> bufs;
   int *h_err = nullptr;  // pinned: [0] error bits, [2..3] largest radius of the last call (double)
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};  // phase boundaries of the last call
+  // the mixed kernels' classes run on the call's stream and these two (default
+  // priority), forked and joined by events, so each launch's tail overlaps
+  // another class's work
+  hipStream_t mx_side[2] = {nullptr, nullptr};
+  hipEvent_t mx_fork = nullptr, mx_join[2] = {nullptr, nullptr};
   float phase_ms[3] = {0.f, 0.f, 0.f};                      // FP64 contraction, mixed contraction, finalize
   float last_ms = 0.f;
   KlnCache kc;
@@ -658,10 +663,14 @@ int lds_reach(const DirectItem &it, const std::vector<int32_t> &nint) { return s
 // below its first argument to its last, is at most its reach from 0 plus four
 // nodes); an item
 // whose window does not fit reads global memory.
-sfbhip_status launch_plan_mixed(const DirectPlan &pl, const std::vector<int32_t> &nint, DirectItem *ditems,
-                                size_t *off, hipStream_t st, const PointsSoA &pts, const double *dk, int nmax,
-                                const LegendreTables &lt, const HermiteTables &ht, int ops, double *partial) {
-  // classes in the order the items were uploaded (index ci + 6 ksi ascending)
+sfbhip_status launch_mixed_classes(sfbhip_handle *h, const DirectPlan &pl, const std::vector<int32_t> &nint,
+                                   DirectItem *ditems, size_t *off, hipStream_t st, const PointsSoA &pts,
+                                   const double *dk, int nmax, const LegendreTables &lt, const HermiteTables &ht,
+                                   int ops, double *partial, bool *used) {
+  // classes in the order the items were uploaded (index ci + 6 ksi ascending),
+  // dealt in turn to the call's stream and the two side streams
+  const hipStream_t streams[3] = {st, h->mx_side[0], h->mx_side[1]};
+  int next = 0;
   for (int idx = 0; idx < 18; ++idx) {
     const int ci = idx % 6, ksi = idx / 6;
     const auto &v = pl.items[idx];
@@ -672,14 +681,42 @@ sfbhip_status launch_plan_mixed(const DirectPlan &pl, const std::vector<int32_t>
     for (const DirectItem &it : v)
       nodes = std::max(nodes, std::min(lds_reach(it, nint) + 4, MX_LDS_NODES));
     const DirectItem *di = ditems + *off;
+    const int si = next;
+    next = (next + 1) % 3;
+    const hipStream_t s = streams[si];
+    if (si > 0) used[si - 1] = true;
     switch (ci) {
-      case 0: CHK(launch_mixed_ops<1>(ops, ksi, n, di, st, pts, dk, nmax, lt, ht, nodes, partial)); break;
-      case 1: CHK(launch_mixed_ops<2>(ops, ksi, n, di, st, pts, dk, nmax, lt, ht, nodes, partial)); break;
-      case 2: CHK(launch_mixed_ops<4>(ops, ksi, n, di, st, pts, dk, nmax, lt, ht, nodes, partial)); break;
-      default: CHK(launch_mixed_ops<8>(ops, ksi, n, di, st, pts, dk, nmax, lt, ht, nodes, partial)); break;
+      case 0: CHK(launch_mixed_ops<1>(ops, ksi, n, di, s, pts, dk, nmax, lt, ht, nodes, partial)); break;
+      case 1: CHK(launch_mixed_ops<2>(ops, ksi, n, di, s, pts, dk, nmax, lt, ht, nodes, partial)); break;
+      case 2: CHK(launch_mixed_ops<4>(ops, ksi, n, di, s, pts, dk, nmax, lt, ht, nodes, partial)); break;
+      default: CHK(launch_mixed_ops<8>(ops, ksi, n, di, s, pts, dk, nmax, lt, ht, nodes, partial)); break;
     }
     *off += n;
   }
+  return SFBHIP_OK;
+}
+
+// All classes of the mixed plan, forked from st and joined back to it: st's
+// later work (and its phase events) follows every class.  On a failure the
+// side streams are drained before returning, so no launched work outlives the call.
+sfbhip_status launch_plan_mixed(sfbhip_handle *h, const DirectPlan &pl, const std::vector<int32_t> &nint,
+                                DirectItem *ditems, size_t *off, hipStream_t st, const PointsSoA &pts,
+                                const double *dk, int nmax, const LegendreTables &lt, const HermiteTables &ht, int ops,
+                                double *partial) {
+  HIPCHK(hipEventRecord(h->mx_fork, st));
+  for (int k = 0; k < 2; ++k) HIPCHK(hipStreamWaitEvent(h->mx_side[k], h->mx_fork, 0));
+  bool used[2] = {false, false};
+  const sfbhip_status rc =
+      launch_mixed_classes(h, pl, nint, ditems, off, st, pts, dk, nmax, lt, ht, ops, partial, used);
+  if (rc != SFBHIP_OK) {
+    for (int k = 0; k < 2; ++k) (void)hipStreamSynchronize(h->mx_side[k]);
+    return rc;
+  }
+  for (int k = 0; k < 2; ++k)
+    if (used[k]) {
+      HIPCHK(hipEventRecord(h->mx_join[k], h->mx_side[k]));
+      HIPCHK(hipStreamWaitEvent(st, h->mx_join[k], 0));
+    }
   return SFBHIP_OK;
 }
 
@@ -744,6 +781,11 @@ sfbhip_status sfbhip_create(sfbhip_handle **out, int device) {
     return fail(SFBHIP_HIP_ERROR, std::string("hipHostMalloc: ") + hipGetErrorString(e));
   }
   for (auto &e : h->ev) HIPCHK(hipEventCreate(&e));
+  HIPCHK(hipEventCreateWithFlags(&h->mx_fork, hipEventDisableTiming));
+  for (int k = 0; k < 2; ++k) {
+    HIPCHK(hipEventCreateWithFlags(&h->mx_join[k], hipEventDisableTiming));
+    HIPCHK(hipStreamCreateWithFlags(&h->mx_side[k], hipStreamNonBlocking));
+  }
   *out = h;
   return SFBHIP_OK;
 }
@@ -757,6 +799,11 @@ sfbhip_status sfbhip_destroy(sfbhip_handle *h) {
   if (h->h_err) (void)hipHostFree(h->h_err);
   for (auto &e : h->ev)
     if (e) (void)hipEventDestroy(e);
+  if (h->mx_fork) (void)hipEventDestroy(h->mx_fork);
+  for (int k = 0; k < 2; ++k) {
+    if (h->mx_join[k]) (void)hipEventDestroy(h->mx_join[k]);
+    if (h->mx_side[k]) (void)hipStreamDestroy(h->mx_side[k]);
+  }
   delete h;
   return SFBHIP_OK;
 }
@@ -883,7 +930,7 @@ sfbhip_status sfbhip_direct(sfbhip_handle *h, const double *d_points, int64_t n_
   CHK(launch_plan_f64(pl64, ditems, &off, st, pts, nullptr, dk, p->nmax, lt, use_table, tab, partial, derr));
   HIPCHK(hipEventRecord(h->ev[1], st));
   if (mixed)
-    CHK(launch_plan_mixed(plmx, h->kc.herm_nint, ditems, &off, st, pts, dk, p->nmax, lt, ht,
+    CHK(launch_plan_mixed(h, plmx, h->kc.herm_nint, ditems, &off, st, pts, dk, p->nmax, lt, ht,
                           p->mixed_operands, partial));
   HIPCHK(hipEventRecord(h->ev[2], st));
   CHK(finalize(h, p, groups, partial, norm, d_k_ln, d_c_ln, d_f_lmn, st));
@@ -1156,7 +1203,8 @@ sfbhip_status sfbhip_direct_batched(sfbhip_handle *h, const sfbhip_catalog *cats
   CHK(launch_plan_f64(pl64, ditems, &off, st, pts, nullptr, dk, 0, lt, mixed ? JL_QUINTIC : JL_EXACT, tab, partial,
                       derr));
   HIPCHK(hipEventRecord(h->ev[1], st));
-  if (mixed) CHK(launch_plan_mixed(plmx, hnint, ditems, &off, st, pts, dk, 0, lt, ht, p->mixed_operands, partial));
+  if (mixed)
+    CHK(launch_plan_mixed(h, plmx, hnint, ditems, &off, st, pts, dk, 0, lt, ht, p->mixed_operands, partial));
   HIPCHK(hipEventRecord(h->ev[2], st));
 
   // f_lmn of all catalogues in one workspace: zero, reduce, then C_l, k_ln and f_lmn per catalogue
