@@ -77,7 +77,7 @@ struct sfbhip_handle {
   std::map<std::string, DevBuf> bufs;
   int *h_err = nullptr;  // pinned: [0] error bits, [2..3] largest radius of the last call (double)
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};  // phase boundaries of the last call
-  // the mixed kernels' classes run on the call's stream and these two (default
+  // the classes of the mixed and FP64 kernels run on the call's stream and these two (default
   // priority), forked and joined by events, so each launch's tail overlaps
   // another class's work
   hipStream_t mx_side[2] = {nullptr, nullptr};
@@ -634,25 +634,67 @@ sfbhip_status launch_mixed_ops(int ops, int ksi, int nitems, const DirectItem *i
 }
 
 // Launch every class of a plan; items[] on the device holds the classes back to back from `*off`.
-sfbhip_status launch_plan_f64(const DirectPlan &pl, DirectItem *ditems, size_t *off, hipStream_t st,
-                              const PointsSoA &pts, const PixelsSoA *pix, const double *dk, int nmax,
-                              const LegendreTables &lt, int use_table, const BesselTableRef &tab, double *partial,
-                              int *derr) {
+// The FP64 classes dealt in turn to st and the handle's two side streams
+// (forked and joined by launch_plan_f64).
+sfbhip_status launch_f64_classes(sfbhip_handle *h, const DirectPlan &pl, DirectItem *ditems, size_t *off,
+                                 hipStream_t st, const PointsSoA &pts, const PixelsSoA *pix, const double *dk,
+                                 int nmax, const LegendreTables &lt, int use_table, const BesselTableRef &tab,
+                                 double *partial, int *derr, bool *used) {
+  const hipStream_t streams[3] = {st, h->mx_side[0], h->mx_side[1]};
+  int next = 0;
   for (int ci = 0; ci < 5; ++ci) {
     const int n = (int)pl.items[ci].size();
     if (!n) continue;
     const DirectItem *di = ditems + *off;
+    const int si = next;
+    next = (next + 1) % 3;
+    const hipStream_t s = streams[si];
+    if (si > 0) used[si - 1] = true;
     switch (ci) {
-      case 0: launch_f64<1>(n, di, st, pts, pix, dk, nmax, lt, use_table, tab, partial, derr); break;
-      case 1: launch_f64<2>(n, di, st, pts, pix, dk, nmax, lt, use_table, tab, partial, derr); break;
-      case 2: launch_f64<4>(n, di, st, pts, pix, dk, nmax, lt, use_table, tab, partial, derr); break;
-      case 3: launch_f64<8>(n, di, st, pts, pix, dk, nmax, lt, use_table, tab, partial, derr); break;
-      default: launch_f64<16>(n, di, st, pts, pix, dk, nmax, lt, use_table, tab, partial, derr); break;
+      case 0: launch_f64<1>(n, di, s, pts, pix, dk, nmax, lt, use_table, tab, partial, derr); break;
+      case 1: launch_f64<2>(n, di, s, pts, pix, dk, nmax, lt, use_table, tab, partial, derr); break;
+      case 2: launch_f64<4>(n, di, s, pts, pix, dk, nmax, lt, use_table, tab, partial, derr); break;
+      case 3: launch_f64<8>(n, di, s, pts, pix, dk, nmax, lt, use_table, tab, partial, derr); break;
+      default: launch_f64<16>(n, di, s, pts, pix, dk, nmax, lt, use_table, tab, partial, derr); break;
     }
     HIPCHK(hipGetLastError());
     *off += n;
   }
   return SFBHIP_OK;
+}
+
+sfbhip_status fork_sides(sfbhip_handle *h, hipStream_t st) {
+  HIPCHK(hipEventRecord(h->mx_fork, st));
+  for (int k = 0; k < 2; ++k) HIPCHK(hipStreamWaitEvent(h->mx_side[k], h->mx_fork, 0));
+  return SFBHIP_OK;
+}
+
+// st waits for the side streams that ran work; after a failure they are drained
+// instead, so no launched work outlives the call.
+sfbhip_status join_sides(sfbhip_handle *h, hipStream_t st, sfbhip_status rc, const bool *used) {
+  if (rc != SFBHIP_OK) {
+    for (int k = 0; k < 2; ++k) (void)hipStreamSynchronize(h->mx_side[k]);
+    return rc;
+  }
+  for (int k = 0; k < 2; ++k)
+    if (used[k]) {
+      HIPCHK(hipEventRecord(h->mx_join[k], h->mx_side[k]));
+      HIPCHK(hipStreamWaitEvent(st, h->mx_join[k], 0));
+    }
+  return SFBHIP_OK;
+}
+
+// All classes of an FP64 plan, forked from st and joined back to it.
+sfbhip_status launch_plan_f64(sfbhip_handle *h, const DirectPlan &pl, DirectItem *ditems, size_t *off,
+                              hipStream_t st, const PointsSoA &pts, const PixelsSoA *pix, const double *dk, int nmax,
+                              const LegendreTables &lt, int use_table, const BesselTableRef &tab, double *partial,
+                              int *derr) {
+  CHK(fork_sides(h, st));
+  bool used[2] = {false, false};
+  return join_sides(h, st,
+                    launch_f64_classes(h, pl, ditems, off, st, pts, pix, dk, nmax, lt, use_table, tab, partial, derr,
+                                       used),
+                    used);
 }
 
 // Bessel-table nodes an item needs in LDS
@@ -703,21 +745,11 @@ sfbhip_status launch_plan_mixed(sfbhip_handle *h, const DirectPlan &pl, const st
                                 DirectItem *ditems, size_t *off, hipStream_t st, const PointsSoA &pts,
                                 const double *dk, int nmax, const LegendreTables &lt, const HermiteTables &ht, int ops,
                                 double *partial) {
-  HIPCHK(hipEventRecord(h->mx_fork, st));
-  for (int k = 0; k < 2; ++k) HIPCHK(hipStreamWaitEvent(h->mx_side[k], h->mx_fork, 0));
+  CHK(fork_sides(h, st));
   bool used[2] = {false, false};
-  const sfbhip_status rc =
-      launch_mixed_classes(h, pl, nint, ditems, off, st, pts, dk, nmax, lt, ht, ops, partial, used);
-  if (rc != SFBHIP_OK) {
-    for (int k = 0; k < 2; ++k) (void)hipStreamSynchronize(h->mx_side[k]);
-    return rc;
-  }
-  for (int k = 0; k < 2; ++k)
-    if (used[k]) {
-      HIPCHK(hipEventRecord(h->mx_join[k], h->mx_side[k]));
-      HIPCHK(hipStreamWaitEvent(st, h->mx_join[k], 0));
-    }
-  return SFBHIP_OK;
+  return join_sides(h, st,
+                    launch_mixed_classes(h, pl, nint, ditems, off, st, pts, dk, nmax, lt, ht, ops, partial, used),
+                    used);
 }
 
 // The low-l FP64 items (all QMAX class 1), j_l from the quintic table.
@@ -927,7 +959,7 @@ sfbhip_status sfbhip_direct(sfbhip_handle *h, const double *d_points, int64_t n_
   HIPCHK(hipEventRecord(h->ev[0], st));
   size_t off = 0;
   CHK(launch_lowl(pllw, ditems, &off, st, pts, dk, tab, partial));
-  CHK(launch_plan_f64(pl64, ditems, &off, st, pts, nullptr, dk, p->nmax, lt, use_table, tab, partial, derr));
+  CHK(launch_plan_f64(h, pl64, ditems, &off, st, pts, nullptr, dk, p->nmax, lt, use_table, tab, partial, derr));
   HIPCHK(hipEventRecord(h->ev[1], st));
   if (mixed)
     CHK(launch_plan_mixed(h, plmx, h->kc.herm_nint, ditems, &off, st, pts, dk, p->nmax, lt, ht,
@@ -1200,7 +1232,7 @@ sfbhip_status sfbhip_direct_batched(sfbhip_handle *h, const sfbhip_catalog *cats
   HIPCHK(hipEventRecord(h->ev[0], st));
   size_t off = 0;
   CHK(launch_lowl(pllw, ditems, &off, st, pts, dk, tab, partial));
-  CHK(launch_plan_f64(pl64, ditems, &off, st, pts, nullptr, dk, 0, lt, mixed ? JL_QUINTIC : JL_EXACT, tab, partial,
+  CHK(launch_plan_f64(h, pl64, ditems, &off, st, pts, nullptr, dk, 0, lt, mixed ? JL_QUINTIC : JL_EXACT, tab, partial,
                       derr));
   HIPCHK(hipEventRecord(h->ev[1], st));
   if (mixed)
@@ -1394,7 +1426,7 @@ sfbhip_status sfbhip_healpix(sfbhip_handle *h, const int64_t *d_offsets, const d
   HIPCHK(hipEventRecord(h->ev[0], st));
   size_t off = 0;
   PointsSoA none{};
-  CHK(launch_plan_f64(pl, ditems, &off, st, none, &pix, dk, p->nmax, lt, use_table, tab, partial, derr));
+  CHK(launch_plan_f64(h, pl, ditems, &off, st, none, &pix, dk, p->nmax, lt, use_table, tab, partial, derr));
   HIPCHK(hipEventRecord(h->ev[1], st));
   HIPCHK(hipEventRecord(h->ev[2], st));
   CHK(finalize(h, p, pl.groups, partial, norm, d_k_ln, d_c_ln, d_f_lmn, st));
