@@ -4,6 +4,9 @@
                     [--workload clustered1m|lognormal2p25|uniform10k|batched64|healpix4m|shotnoise2p25|ingest2p25]
                     [--precision mixed|fp64] [--cpu-sample N | --no-cpu-baseline]
 
+The default workload is lognormal2p25: 2^25 points of a lognormal density field
+cut to a ball of rmax 1000, lmax 256, k_max 0.5 (n_l 159 -> 52), decomposed in
+mixed precision and compared with the plain FP64 GPU path on the same points.
 A step is one full sfbhip_direct call (k_ln lookup, point preparation, the
 contraction kernels, the FP64 reduction and C_l assembly) on points already
 resident in HBM.  The metric is point-mode pairs/s = N * sum_l (2l+1) n_l /
@@ -80,30 +83,61 @@ def make_points(name):
     raise ValueError(name)
 
 
-# default CPU-baseline subsample per workload (points), about 10-25 s on 16 threads
-CPU_SAMPLE = {"clustered1m": 1 << 19, "lognormal2p25": 16384, "uniform10k": 10000, "batched64": 1 << 17}
+# default CPU-baseline subsample per workload (points): each timed run is a few seconds on 16 threads
+CPU_SAMPLE = {"clustered1m": 1 << 17, "lognormal2p25": 4096, "uniform10k": 10000, "batched64": 1 << 15}
+CPU_RUNS = 3
 
 
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def cpu_baseline(points, cfg, nl, sample, threads):
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    import platform
+    return platform.processor() or "unknown"
+
+
+def cpu_baseline(points, cfg, nl, sample, threads, runs=CPU_RUNS):
     """The oracle (C/OpenMP restatement of sfb_raw.cpp:13-108 with the public
     wrapper's interpolated = N > 1000, sfb_decomposition.cpp:17) timed on a
-    seeded subsample of the same catalogue; pairs/s scale linearly in N."""
+    seeded subsample of the same catalogue, median of `runs` runs.  The
+    per-call interpolation-table build (1000 nmax j_l values per l,
+    SBesselInterpolator.cpp:10-17) does not depend on N: it is timed on its own
+    (a 1001-point call) and the full-catalogue time is estimated as
+    table + (N / n) (median - table)."""
     from oracle import oracle as O
     rng = np.random.default_rng(123)
-    n = min(sample, points.shape[0])
-    sub = points[np.sort(rng.choice(points.shape[0], n, replace=False))]
-    t0 = time.perf_counter()
-    O.sfb_raw(sub, cfg["lmax"], cfg["nmax"], cfg["rmax"], nl=nl, interpolated=n > 1000, nthreads=threads)
-    dt = time.perf_counter() - t0
+    N = points.shape[0]
+    n = min(sample, N)
+    sub = points[np.sort(rng.choice(N, n, replace=False))]
+    times = []
+    for _ in range(runs):
+        t0 = time.perf_counter()
+        O.sfb_raw(sub, cfg["lmax"], cfg["nmax"], cfg["rmax"], nl=nl, interpolated=n > 1000, nthreads=threads)
+        times.append(time.perf_counter() - t0)
+    med = float(np.median(times))
+    t_tab = 0.0
+    if n > 1000:
+        t0 = time.perf_counter()
+        O.sfb_raw(sub[:1001], cfg["lmax"], cfg["nmax"], cfg["rmax"], nl=nl, interpolated=True, nthreads=threads)
+        t_tab = min(time.perf_counter() - t0, med)
+    t_full = t_tab + (N / n) * (med - t_tab)
     l = np.arange(cfg["lmax"])
-    pairs = n * int(((2 * l + 1) * nl).sum())
-    return {"value": pairs / dt, "unit": "pairs/s", "cores": threads, "kind": "port",
-            "sample": f"{n} of {points.shape[0]} points (seeded subset), same lmax/k_max, "
-                      f"reference-default linear Bessel table, {dt:.2f} s"}
+    pairs = N * int(((2 * l + 1) * nl).sum())
+    return {"value": pairs / t_full, "unit": "pairs/s", "cores": threads, "kind": "port",
+            "cpu_model": cpu_model(), "nproc": os.cpu_count(),
+            "runs_s": [round(t, 3) for t in times], "table_build_s": round(t_tab, 3),
+            "sample": f"{n} of {N} points (seeded subset), same lmax/k_max, reference-default linear Bessel "
+                      f"table; median of {runs} runs {med:.2f} s (spread {min(times):.2f}-{max(times):.2f} s), "
+                      f"table build {t_tab:.2f} s; full catalogue estimated as table + (N/n)(median - table) "
+                      f"= {t_full:.1f} s"}
 
 
 def main():
@@ -111,7 +145,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="clustered1m", choices=sorted(WORKLOADS))
+    ap.add_argument("--workload", default="lognormal2p25", choices=sorted(WORKLOADS))
     ap.add_argument("--precision", default="mixed", choices=["mixed", "fp64"])
     ap.add_argument("--operands", default="split16", choices=["fp32", "split16", "split3"],
                     help="matrix-core operand format of the mixed path")
@@ -120,7 +154,7 @@ def main():
     ap.add_argument("--fp64-below", type=int, default=2, help="mixed path: l below this run in FP64")
     ap.add_argument("--cpu-sample", type=int, default=0,
                     help="points of the CPU baseline's seeded subsample (0: the workload's default, "
-                         "about 10-25 s of CPU work on 16 threads)")
+                         "a few seconds per run on 16 threads, three runs)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",

@@ -389,7 +389,11 @@ void launch_f64(int nitems, const DirectItem *items, hipStream_t st, const Point
                                                                  tab, partial, err);
 }
 
-// Also reads back the catalogue's largest radius into *rdata (one stream synchronisation).
+// The point table; also reads back the error bits and the catalogue's largest
+// radius into *rdata (one stream synchronisation).  A catalogue with a point
+// at r < 0 or a non-finite coordinate is refused here, before any kernel
+// evaluates a Bessel function of it (the reference's interpolator rejects
+// arguments outside its table, FunctionInterpolator.cpp:22-24).
 sfbhip_status prep_points(sfbhip_handle *h, const double *d_points, int64_t n, hipStream_t st, int *err,
                           PointsSoA *pts, double *rdata) {
   double *buf;
@@ -398,10 +402,11 @@ sfbhip_status prep_points(sfbhip_handle *h, const double *d_points, int64_t n, h
   unsigned long long *rbits = reinterpret_cast<unsigned long long *>(err + 2);
   prep_points_kernel<<<cdiv(n, 256), 256, 0, st>>>(d_points, n, r, x, s, ph, cp, sp, err, rbits);
   HIPCHK(hipGetLastError());
-  HIPCHK(hipMemcpyAsync(h->h_err + 2, rbits, sizeof(double), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipMemcpyAsync(h->h_err, err, 4 * sizeof(int), hipMemcpyDeviceToHost, st));
   HIPCHK(hipStreamSynchronize(st));
   std::memcpy(rdata, h->h_err + 2, sizeof(double));
   HIPCHK(hipGetLastError());
+  if (h->h_err[0] & ERR_BADPOINT) return fail(SFBHIP_DOMAIN_ERROR, "point with r < 0 or a non-finite coordinate");
   pts->r = r;
   pts->x = x;
   pts->s = s;
@@ -874,6 +879,7 @@ sfbhip_status sfbhip_direct(sfbhip_handle *h, const double *d_points, int64_t n_
                             double *d_k_ln, double *d_c_ln, double *d_f_lmn, void *stream) {
   g_last_error.clear();
   if (!h) return fail(SFBHIP_INVALID_ARGUMENT, "handle is NULL");
+  clear_timings(h);  // a call refused before its contraction reports zero phase times
   CHK(validate(p));
   if (n_points <= 0 || !d_points)
     return fail(SFBHIP_INVALID_ARGUMENT, "empty catalogue: the normalisation divides by the number of points");
@@ -979,6 +985,7 @@ sfbhip_status sfbhip_direct_batched(sfbhip_handle *h, const sfbhip_catalog *cats
                                     const sfbhip_params *p, void *stream) {
   g_last_error.clear();
   if (!h) return fail(SFBHIP_INVALID_ARGUMENT, "handle is NULL");
+  clear_timings(h);  // a call refused before its contraction reports zero phase times
   if (!cats || ncat < 1 || ncat > 128) return fail(SFBHIP_INVALID_ARGUMENT, "ncat must be in [1, 128]");
   if (!p) return fail(SFBHIP_INVALID_ARGUMENT, "params is NULL");
   // per-catalogue parameters, each validated as a single call would be
@@ -1043,7 +1050,10 @@ sfbhip_status sfbhip_direct_batched(sfbhip_handle *h, const sfbhip_catalog *cats
   }
   std::vector<double> rdata(ncat);
   HIPCHK(hipMemcpyAsync(rdata.data(), rbits, sizeof(double) * ncat, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipMemcpyAsync(h->h_err, derr, sizeof(int), hipMemcpyDeviceToHost, st));
   HIPCHK(hipStreamSynchronize(st));
+  // no kernel evaluates a Bessel function of a bad point: refused before the tables and contractions
+  if (h->h_err[0] & ERR_BADPOINT) return fail(SFBHIP_DOMAIN_ERROR, "point with r < 0 or a non-finite coordinate");
   PointsSoA pts{soa, soa + ntot, soa + 2 * ntot, soa + 3 * ntot, soa + 4 * ntot, soa + 5 * ntot, ntot};
   if (mixed && ntot > 1) {
     // sort by (catalogue, radius): each catalogue stays in its range, ordered by r
@@ -1332,6 +1342,7 @@ sfbhip_status sfbhip_healpix(sfbhip_handle *h, const int64_t *d_offsets, const d
                              void *stream) {
   g_last_error.clear();
   if (!h) return fail(SFBHIP_INVALID_ARGUMENT, "handle is NULL");
+  clear_timings(h);  // a call refused before its contraction reports zero phase times
   CHK(validate(p));
   if (nside < 1 || (nside & (nside - 1)) || nside > 8192)
     return fail(SFBHIP_INVALID_ARGUMENT, "argument (nside) must be a power of 2");
@@ -1341,6 +1352,8 @@ sfbhip_status sfbhip_healpix(sfbhip_handle *h, const int64_t *d_offsets, const d
   hipStream_t st = static_cast<hipStream_t>(stream);
   HIPCHK(hipSetDevice(h->device));
   const int64_t npix = 12LL * nside * nside;
+  int *derr;
+  CHK(zero_err(h, st, &derr));
   // total number of points (the normalisation's N, sfb_reverse_fft.cpp:49-50)
   int64_t npts = 0;
   HIPCHK(hipMemcpyAsync(h->h_err, d_offsets + npix, sizeof(int64_t), hipMemcpyDeviceToHost, st));
@@ -1348,12 +1361,22 @@ sfbhip_status sfbhip_healpix(sfbhip_handle *h, const int64_t *d_offsets, const d
   std::memcpy(&npts, h->h_err, sizeof(int64_t));
   if (npts <= 0 || !d_radii)
     return fail(SFBHIP_INVALID_ARGUMENT, "empty catalogue: the normalisation divides by the number of points");
+  if (npts > 0x7fffffffLL) return fail(SFBHIP_NOT_SUPPORTED, "more than 2^31 points");
+  // the CSR and its radii are checked before any kernel reads through them
+  csr_check_kernel<<<(unsigned)std::min<int64_t>(cdiv(std::max(npix, npts), 256), 8192), 256, 0, st>>>(
+      d_offsets, npix, d_radii, npts, derr);
+  HIPCHK(hipGetLastError());
+  {
+    int e0 = 0;
+    CHK(read_err(h, st, derr, &e0));
+    if (e0 & ERR_BADCSR)
+      return fail(SFBHIP_INVALID_ARGUMENT, "pixel offsets are not a CSR of the radii (off[0] = 0, non-decreasing)");
+    if (e0 & ERR_BADPOINT) return fail(SFBHIP_DOMAIN_ERROR, "point with r < 0 or a non-finite coordinate");
+  }
   if (p->verbose) {
     std::printf("SFB decomposition (reverse method): lmax=%d nmax=%d NSide=%d\n", p->lmax, p->nmax, nside);
     std::printf("Catalog consists of %lld points.\n", (long long)npts);
   }
-  int *derr;
-  CHK(zero_err(h, st, &derr));
   CHK(ensure_kln(h, p, st));
   LegendreTables lt;
   CHK(ensure_legendre(h, std::max(p->lmax, 1), st, &lt));

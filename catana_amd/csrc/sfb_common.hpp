@@ -13,7 +13,8 @@ typedef float v16f __attribute__((ext_vector_type(16)));
 // Device-side error bits (OR-ed into one int by any thread that hits them).
 enum : int {
   ERR_DOMAIN = 1,   // Bessel argument outside the reference table range
-  ERR_BADPOINT = 2  // r < 0 or a non-finite coordinate
+  ERR_BADPOINT = 2,  // r < 0 or a non-finite coordinate
+  ERR_BADCSR = 4     // pixel offsets that are not a CSR of the radii
 };
 
 // Per-point data derived once from the AoS (r, theta, phi) input; SoA, FP64.

@@ -101,16 +101,20 @@ __global__ void quintic_table_kernel(int lmax, const int32_t *__restrict__ off, 
 __device__ __forceinline__ double bessel_f64(int l, double z, int use_table, const BesselTableRef &tab, int &err) {
   if (use_table == JL_EXACT) return sbessel_jl(l, z);
   if (use_table == JL_QUINTIC) {
+    // the index is clamped on both sides, so no argument reads outside the table
+    // (arguments are checked before a launch; this guards the reads regardless)
     const double u = z * Q5_HINV;
     int i = (int)u;
     const int last = tab.q5_nint[l] - 1;
     i = i < last ? i : last;
+    i = i > 0 ? i : 0;
     const double t = u - (double)i;
     const double2 *cc = reinterpret_cast<const double2 *>(tab.q5 + 6 * ((size_t)tab.q5_off[l] + i));
     const double2 a = cc[0], b = cc[1], e = cc[2];
     return fma(t, fma(t, fma(t, fma(t, fma(t, e.y, e.x), b.y), b.x), a.y), a.x);
   }
-  if (!(z < tab.zmax[l])) {
+  // the table covers [0, z_max): anything else (including NaN) is a domain error
+  if (!(z >= 0.0 && z < tab.zmax[l])) {
     err |= ERR_DOMAIN;
     return 0.0;
   }
@@ -118,6 +122,7 @@ __device__ __forceinline__ double bessel_f64(int l, double z, int use_table, con
   const double ia = z * tab.dx_inv[l];
   int i = (int)ia;
   i = i > tab.P - 2 ? tab.P - 2 : i;
+  i = i > 0 ? i : 0;
   return y[i] + (y[i + 1] - y[i]) * (ia - i);
 }
 

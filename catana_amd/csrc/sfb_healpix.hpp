@@ -147,6 +147,23 @@ __global__ void csr_offsets_kernel(const uint32_t *__restrict__ keys, int64_t n,
   off[p] = lo;
 }
 
+// Checks a caller's CSR before the pixelised contraction reads through it:
+// off[0] = 0, off non-decreasing up to off[npix] = npts (read by the host),
+// every radius finite and >= 0.  Grid-stride over max(npix, npts).
+__global__ void csr_check_kernel(const int64_t *__restrict__ off, int64_t npix, const double *__restrict__ radii,
+                                 int64_t npts, int *__restrict__ err) {
+  const int64_t n = npix > npts ? npix : npts;
+  int bad = 0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    if (i < npix && (off[i + 1] < off[i] || (i == 0 && off[0] != 0))) bad |= ERR_BADCSR;
+    if (i < npts) {
+      const double r = radii[i];
+      if (!(r >= 0.0) || !isfinite(r)) bad |= ERR_BADPOINT;
+    }
+  }
+  if (bad) atomicOr(err, bad);
+}
+
 // Sort keys of the pixelised contraction's pixel order: the point count of
 // pixel p (a stable sort keeps ties in pixel order).  Each lane of the
 // contraction loops over its pixel's points, so a wave of pixels with about

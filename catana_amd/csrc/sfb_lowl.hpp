@@ -123,6 +123,7 @@ __global__ __launch_bounds__(256) void sfb_lowl_f64_kernel(PointsSoA pts, const 
           const double u = kr[t] * r * Q5_HINV;
           int i = (int)u;
           i = i < last ? i : last;
+          i = i > 0 ? i : 0;
           const double w = u - (double)i;
           const double2 a0 = tb[3 * i], a1 = tb[3 * i + 1], a2 = tb[3 * i + 2];
           const double jv = fma(w, fma(w, fma(w, fma(w, fma(w, a2.y, a2.x), a1.y), a1.x), a0.y), a0.x);

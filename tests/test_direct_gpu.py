@@ -170,3 +170,37 @@ def test_mixed_operand_formats_and_chunks(handle, ops, chunks):
     print("max rel c_ln", rel.max(), "median", np.median(rel))
     assert rel.max() < 1e-6
     assert _flmn_err(fm, f64) < 1e-6
+
+
+def test_device_jl_vs_mpmath_goldens(handle, golden_dir):
+    # one-point catalogues place k_{l,0} r at each golden argument x of special_functions.npz
+    # (mpmath j_l, 450 digits): c_l0 = (2/pi) V^2 k^2 j_l(x)^2 / (4 pi) gives the device's |j_l(x)|
+    # directly, not through the oracle's j_l
+    g = np.load(os.path.join(golden_dir, "special_functions.npz"))
+    ls, xs, vs = g["jl_l"], g["jl_x"], g["jl_v"]
+    print(f"case: device j_l at {ls.size} golden arguments", flush=True)
+    cats, meta = [], []
+    for l, x, v in zip(ls, xs, vs):
+        z0 = O.bessel_zeros(int(l), 1)[0]
+        rmax = 1.0
+        r = x / z0
+        cats.append(dict(points=torch.tensor([[r, 0.9, 1.3]], dtype=torch.float64, device="cuda"),
+                         lmax=int(l) + 1, nmax=1, rmax=rmax))
+        meta.append((int(l), x, v, rmax))
+    got = []
+    for i in range(0, len(cats), 128):
+        res = capi.sfb_direct_batched_torch(handle, cats[i:i + 128], precision=capi.PREC_FP64)
+        torch.cuda.synchronize()
+        got += [(k.cpu().numpy(), c.cpu().numpy()) for k, c, _ in res]
+    checked = 0
+    for (l, x, v, rmax), (k, c) in zip(meta, got):
+        V = 4.0 / 3.0 * np.pi * rmax ** 3
+        kk = k[l, 0]
+        j = np.sqrt(c[l, 0] * 4 * np.pi / (2 / np.pi * V ** 2 * kk ** 2))
+        if abs(v) > 1e-250:
+            # r = x / z0 moves x by about an ulp: |j_l'| <= 1 bounds the change by ~x 2^-52
+            np.testing.assert_allclose(j, abs(v), rtol=1e-12, atol=4e-16 * max(x, 1.0))
+            checked += 1
+        else:
+            assert j <= 1e-240
+    assert checked > 250

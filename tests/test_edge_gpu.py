@@ -145,3 +145,49 @@ def test_batched_deterministic(handle):
     for a, b in zip(r0, r1):
         for x, y in zip(a, b):
             assert torch.equal(x, y)
+
+
+@pytest.mark.parametrize("bad_r", [-1.0, -np.inf, np.inf, np.nan])
+@pytest.mark.parametrize("precision,bessel", [(capi.PREC_FP64, capi.BESSEL_AUTO), (capi.PREC_FP64, capi.BESSEL_EXACT),
+                                              (capi.PREC_MIXED, capi.BESSEL_EXACT)])
+def test_bad_point_refused_before_contraction(handle, bad_r, precision, bessel):
+    # one bad radius in a catalogue large enough for the reference table (N > 1000): the call
+    # fails with the domain error and reports zero phase times, i.e. no contraction ran
+    pts = workloads.uniform_ball(5000, 300.0, seed=41)
+    pts[1234, 0] = bad_r
+    print(f"case: bad radius {bad_r}, precision {precision}, bessel {bessel}", flush=True)
+    d = torch.from_numpy(pts).cuda()
+    with pytest.raises(ValueError, match="r < 0 or a non-finite"):
+        capi.sfb_direct_torch(handle, d, 24, 12, 300.0, precision=precision, bessel=bessel)
+    assert handle.last_phase_ms() == (0.0, 0.0, 0.0)
+    # batched: the bad catalogue among good ones
+    good = torch.from_numpy(workloads.uniform_ball(3000, 500.0, seed=42)).cuda()
+    cats = [dict(points=good, lmax=20, nmax=10, rmax=500.0), dict(points=d, lmax=24, nmax=12, rmax=300.0)]
+    with pytest.raises(ValueError, match="r < 0 or a non-finite"):
+        capi.sfb_direct_batched_torch(handle, cats, precision=precision)
+    assert handle.last_phase_ms() == (0.0, 0.0, 0.0)
+    # the handle stays usable
+    _, c, _ = capi.sfb_direct_torch(handle, good, 20, 10, 500.0, precision=precision, bessel=bessel)
+    assert torch.isfinite(c).all()
+
+
+def test_healpix_bad_csr_refused(handle):
+    # a caller-built CSR whose radii or offsets are invalid is refused before the contraction
+    nside = 4
+    npix = 12 * nside * nside
+    pts = workloads.uniform_ball(2000, 300.0, seed=43)
+    off, rad = capi.pixelize_torch(handle, torch.from_numpy(pts).cuda(), nside)
+    torch.cuda.synchronize()
+    for what, mutate, exc, msg in [
+            ("negative radius", lambda o, r: r.__setitem__(7, -5.0), ValueError, "r < 0"),
+            ("nan radius", lambda o, r: r.__setitem__(9, float("nan")), ValueError, "r < 0"),
+            ("decreasing offsets", lambda o, r: o.__setitem__(5, o[6] + 1), ValueError, "CSR"),
+            ("offset 0 not zero", lambda o, r: o.__setitem__(0, 1), ValueError, "CSR")]:
+        o, r = off.clone(), rad.clone()
+        mutate(o, r)
+        print(f"case: healpix {what}", flush=True)
+        with pytest.raises(exc, match=msg):
+            capi.sfb_healpix_torch(handle, o, r, nside, 16, 8, 300.0)
+        assert handle.last_phase_ms() == (0.0, 0.0, 0.0)
+    k, c, _ = capi.sfb_healpix_torch(handle, off, rad, nside, 16, 8, 300.0)
+    assert torch.isfinite(c).all() and off.shape[0] == npix + 1

@@ -327,3 +327,27 @@ def test_sbessel_integral_stopping_rule_keeps_the_value():
     ref = quad(lambda r: r * r * np.exp(-(r / g) ** 2) * J(l, k * r) ** 2, 0, 400, limit=400, epsabs=0,
                epsrel=1e-13)[0]
     assert abs(v - ref) <= 1e-9 * ref
+
+
+def test_interpolated_lookup_vs_numpy_interp():
+    # one point: c_ln = (2/pi) V^2 k^2 j(k r)^2 / (4 pi) whatever the angles, so the oracle's
+    # linear-table value j(k r) is read back and compared with numpy.interp over nodes
+    # x_i = i z_max / (P - 1), P = 1000 nmax, z_max = z_{l,nmax}, valued by scipy's j_l
+    # (SBesselInterpolator.cpp:10-25, FunctionInterpolator.cpp:12-27)
+    from scipy.special import spherical_jn
+    lmax, nmax, rmax = 6, 5, 300.0
+    V = 4.0 / 3.0 * np.pi * rmax ** 3
+    for r in (0.37, 12.5, 151.3, 299.9):
+        pts = np.array([[r, 1.1, 0.4]])
+        k, c, _ = O.sfb_raw(pts, lmax, nmax, rmax, interpolated=True)
+        jtab = np.sqrt(c * 4 * np.pi / (2 / np.pi * V ** 2 * k ** 2))
+        for l in range(lmax):
+            zmax = O.bessel_zeros(l, nmax + 1)[nmax]
+            x = np.linspace(0.0, zmax, 1000 * nmax)
+            x = np.arange(1000 * nmax) * (zmax / (1000 * nmax - 1))
+            y = spherical_jn(l, x)
+            expect = np.abs(np.interp(k[l] * r, x, y))
+            np.testing.assert_allclose(jtab[l], expect, rtol=1e-10, atol=1e-13)
+            # and the table is not the exact function: the interpolation error is visible
+            exact = np.abs(spherical_jn(l, k[l] * r))
+            assert np.abs(expect - exact).max() > 1e-12 or r < 1.0
