@@ -27,6 +27,7 @@ METRIC = "SFB point-mode pairs/s (N·Σ_l(2l+1)n_l); mixed vs FP64 GPU at ≤1e-
 
 # peaks (MI355X datasheet, dense)
 PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3, "fp16": 2516.6}
+PEAK_HBM_GBS = 8000.0
 
 WORKLOADS = {
     # 1,000,000 clustered points (20,000 parents x Poisson(50), sigma 5, clipped), lmax 64, k_max 0.2, rmax 1000
@@ -86,6 +87,7 @@ def make_points(name):
 # default CPU-baseline subsample per workload (points): each timed run is a few seconds on 16 threads
 CPU_SAMPLE = {"clustered1m": 1 << 17, "lognormal2p25": 4096, "uniform10k": 10000, "batched64": 1 << 15}
 CPU_RUNS = 3
+KERNELS = {"auto": 0, "256": 1, "512": 2}
 
 
 def log(msg):
@@ -152,6 +154,8 @@ def main():
     ap.add_argument("--chunks", type=int, default=4, choices=[1, 2, 4, 8],
                     help="mixed path: m chunks accumulated per work item")
     ap.add_argument("--fp64-below", type=int, default=2, help="mixed path: l below this run in FP64")
+    ap.add_argument("--kernel", default="auto", choices=["auto", "256", "512"],
+                    help="mixed path, split-FP16 operands: the contraction kernel")
     ap.add_argument("--cpu-sample", type=int, default=0,
                     help="points of the CPU baseline's seeded subsample (0: the workload's default, "
                          "a few seconds per run on 16 threads, three runs)")
@@ -203,7 +207,7 @@ def main():
     prec = capi.PREC_MIXED if args.precision == "mixed" else capi.PREC_FP64
     ops = {"fp32": capi.OPERANDS_FP32, "split16": capi.OPERANDS_SPLIT_FP16, "split3": capi.OPERANDS_SPLIT3_FP16}
     kw = dict(k_max=cfg["k_max"], precision=prec, bessel=capi.BESSEL_EXACT, mixed_fp64_below=args.fp64_below,
-              mixed_operands=ops[args.operands], mixed_chunks=args.chunks)
+              mixed_operands=ops[args.operands], mixed_chunks=args.chunks, mixed_kernel=KERNELS[args.kernel])
     p = capi.make_params(cfg["lmax"], cfg["nmax"], cfg["rmax"], **kw)
     _, nl = h.kln(p)
     l = np.arange(cfg["lmax"])
@@ -250,7 +254,7 @@ def main():
         "data": "synthetic, seeded (catana_amd/workloads.py), inputs resident in HBM",
         "config": {"workload": args.workload, "N": int(N), "lmax": cfg["lmax"], "k_max": cfg["k_max"],
                    "rmax": cfg["rmax"], "precision": args.precision, "operands": args.operands,
-                   "chunks": args.chunks,
+                   "chunks": args.chunks, "kernel": args.kernel,
                    "fp64_below": args.fp64_below, "pairs_per_step": pairs,
                    "wall_s_timed_region": wall},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
@@ -299,7 +303,12 @@ def main():
 
 def run_healpix(args, cfg):
     """The HEALPix workload: a step is one binning (sfbhip_pixelize) plus one
-    pixelised decomposition (sfbhip_healpix), FP64, points resident in HBM."""
+    pixelised decomposition (sfbhip_healpix) on points resident in HBM.  With
+    --precision mixed (default) the ring path: FP32 j_l-weighted maps, ring
+    Legendre table, split-FP16 ring contraction; HBM roofline on the algorithmic
+    bytes sum_l [4 N + 8 npix n_l] (radii read once per l, each FP32 map written
+    and read once).  With --precision fp64 the FP64 pixel contraction with the
+    reference's linear j_l table (the drop-in)."""
     import torch
     from catana_amd import capi, workloads as W
     log("generating 4,000,000 points with radial selection")
@@ -314,37 +323,78 @@ def run_healpix(args, cfg):
     pairs = N * int(((2 * l + 1) * nl).sum())
     npix = 12 * ns * ns
     flops = 4.0 * npix * int(((l + 1) * nl).sum())
+    hbm_bytes = float(sum(4.0 * N + 8.0 * npix * int(nl[x]) for x in range(L)))
+    ring = args.precision == "mixed"
+    prec = capi.PREC_MIXED if ring else capi.PREC_FP64
 
-    def step():
+    def step(**kw):
         off, rad = capi.pixelize_torch(h, d_pts, ns)
-        return capi.sfb_healpix_torch(h, off, rad, ns, L, NM, R, k_max=cfg["k_max"])
+        return capi.sfb_healpix_torch(h, off, rad, ns, L, NM, R, k_max=cfg["k_max"], **kw)
 
+    log("warmup")
     for _ in range(args.warmup):
-        step()
+        step(precision=prec)
     torch.cuda.synchronize()
     stream = torch.cuda.current_stream(dev)
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    kern = []
+    kern, phases = [], []
     e0.record(stream)
     for _ in range(args.steps):
-        k, c, _ = step()
+        k, c, _ = step(precision=prec)
         kern.append(h.last_kernel_ms())
+        phases.append(h.last_phase_ms())
     e1.record(stream)
     torch.cuda.synchronize()
     ms = e0.elapsed_time(e1) / args.steps
     kms = float(np.mean(kern))
-    achieved = flops / (kms * 1e-3) / 1e12
+    ph = np.mean(np.array(phases), axis=0)
+    log(f"timed {args.steps} steps: {ms:.2f} ms per step")
+    if ring:
+        achieved = hbm_bytes / (kms * 1e-3) / 1e9
+        roof = {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                "frac": achieved / PEAK_HBM_GBS,
+                "traffic": PROFILED_TRAFFIC_BYTES.get(("healpix4m", "mixed")),
+                "kernel": "ring_maps_kernel + healpix_ring_contract_kernel", "kernel_ms": kms,
+                "algorithmic_bytes_per_launch": hbm_bytes,
+                "phases_ms": {"maps": float(ph[0]), "ring_contraction": float(ph[1]),
+                              "reduce_and_cln": float(ph[2])},
+                "matrix_core_tflops": flops / (float(ph[1]) * 1e-3) / 1e12 if ph[1] > 0 else None}
+        dtype = "f32 maps; f16 2-piece split, 3 products (f32 accumulate, f64 tile sums)"
+        bessel = "cubic j_l table (h = 1/16) of the ring path"
+    else:
+        achieved = flops / (kms * 1e-3) / 1e12
+        roof = {"bound": "mfma", "achieved": achieved, "peak": PEAK_TFLOPS["fp64"], "unit": "TFLOP/s",
+                "frac": achieved / PEAK_TFLOPS["fp64"], "traffic": None,
+                "kernel": "sfb_direct_f64_kernel (pixel mode)", "kernel_ms": kms,
+                "algorithmic_flops_per_launch": flops}
+        dtype = "f64"
+        bessel = "reference linear table (N > 1000)"
     out = {
         "metric": METRIC, "value": pairs / (ms * 1e-3), "unit": "pairs/s (point-mode equivalent)", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "f64", "data": "synthetic, seeded (catana_amd/workloads.py), inputs resident in HBM",
+        "vs_baseline": None, "dtype": dtype, "data": "synthetic, seeded (catana_amd/workloads.py), inputs resident in HBM",
         "config": {"workload": "healpix4m", "N": int(N), "nside": ns, "lmax": L, "k_max": cfg["k_max"], "rmax": R,
-                   "pairs_per_step": pairs, "bessel": "reference linear table (N > 1000)"},
-        "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_TFLOPS["fp64"], "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_TFLOPS["fp64"], "traffic": None,
-                     "kernel": "sfb_direct_f64_kernel (pixel mode)", "kernel_ms": kms,
-                     "algorithmic_flops_per_launch": flops},
+                   "pairs_per_step": pairs, "precision": args.precision, "bessel": bessel},
+        "roofline": roof,
     }
+    if ring and not args.no_accuracy:
+        log("FP64 HEALPix path (exact j_l, then the reference table) for the accuracy figures")
+        _, cr, _ = step(precision=prec, store_flmn=True)
+        t64 = time.perf_counter()
+        _, ce, _ = step(precision=capi.PREC_FP64, bessel=capi.BESSEL_EXACT)
+        torch.cuda.synchronize()
+        t64 = time.perf_counter() - t64
+        _, ct, _ = step(precision=capi.PREC_FP64)
+        torch.cuda.synchronize()
+        a, b, t = cr.cpu().numpy(), ce.cpu().numpy(), ct.cpu().numpy()
+        m = b != 0
+        rel = np.abs(a[m] / b[m] - 1)
+        relt = np.abs(a[m] / t[m] - 1)
+        out["accuracy"] = {"max_rel_cln_vs_fp64_exact": float(rel.max()),
+                           "median_rel_cln_vs_fp64_exact": float(np.median(rel)),
+                           "max_rel_cln_vs_fp64_reference_table": float(relt.max()),
+                           "median_rel_cln_vs_fp64_reference_table": float(np.median(relt))}
+        out["fp64_gpu_exact_ms"] = t64 * 1e3
     if args.no_cpu_baseline:
         out["cpu_baseline"] = None
     else:
@@ -513,7 +563,7 @@ def run_batched(args):
     prec = capi.PREC_MIXED if args.precision == "mixed" else capi.PREC_FP64
     ops = {"fp32": capi.OPERANDS_FP32, "split16": capi.OPERANDS_SPLIT_FP16, "split3": capi.OPERANDS_SPLIT3_FP16}
     kw = dict(precision=prec, mixed_fp64_below=args.fp64_below, mixed_operands=ops[args.operands],
-              mixed_chunks=args.chunks)
+              mixed_chunks=args.chunks, mixed_kernel=KERNELS[args.kernel])
     pairs, flops_dom = 0, 0.0
     lo = min(args.fp64_below, 10 ** 9) if prec == capi.PREC_MIXED else 0
     for c in cats:

@@ -16,6 +16,7 @@ PREC_FP64, PREC_MIXED = 0, 1
 BESSEL_EXACT, BESSEL_REFERENCE, BESSEL_AUTO = 0, 1, 2
 ZEROS_REFERENCE, ZEROS_CONVERGED = 0, 1
 OPERANDS_FP32, OPERANDS_SPLIT_FP16, OPERANDS_SPLIT3_FP16 = 0, 1, 2
+MIXED_KERNEL_AUTO, MIXED_KERNEL_256, MIXED_KERNEL_512 = 0, 1, 2
 
 EXPORTED = [
     "sfbhip_default_params", "sfbhip_create", "sfbhip_destroy", "sfbhip_flmn_count", "sfbhip_kln",
@@ -34,7 +35,7 @@ class Params(ctypes.Structure):
         ("k_max", ctypes.c_double), ("window_volume", ctypes.c_double), ("precision", ctypes.c_int32),
         ("bessel", ctypes.c_int32), ("zeros", ctypes.c_int32), ("alias_compat", ctypes.c_int32),
         ("verbose", ctypes.c_int32), ("mixed_fp64_below", ctypes.c_int32), ("mixed_operands", ctypes.c_int32),
-        ("mixed_chunks", ctypes.c_int32),
+        ("mixed_chunks", ctypes.c_int32), ("mixed_kernel", ctypes.c_int32),
     ]
 
 
@@ -113,7 +114,7 @@ def check(rc, what):
 
 def make_params(lmax, nmax, rmax, k_max=0.0, window_volume=0.0, precision=PREC_FP64, bessel=BESSEL_AUTO,
                 zeros=ZEROS_REFERENCE, alias_compat=True, verbose=False, mixed_fp64_below=None,
-                mixed_operands=None, mixed_chunks=None):
+                mixed_operands=None, mixed_chunks=None, mixed_kernel=None):
     p = Params()
     lib().sfbhip_default_params(ctypes.byref(p))
     p.lmax, p.nmax, p.rmax = int(lmax), int(nmax), float(rmax)
@@ -126,6 +127,8 @@ def make_params(lmax, nmax, rmax, k_max=0.0, window_volume=0.0, precision=PREC_F
         p.mixed_operands = int(mixed_operands)
     if mixed_chunks is not None:
         p.mixed_chunks = int(mixed_chunks)
+    if mixed_kernel is not None:
+        p.mixed_kernel = int(mixed_kernel)
     return p
 
 

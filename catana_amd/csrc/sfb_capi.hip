@@ -11,6 +11,7 @@
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -21,8 +22,10 @@
 #include "sfb_common.hpp"
 #include "sfb_direct_f64.hpp"
 #include "sfb_direct_mixed.hpp"
+#include "sfb_direct_mixed2.hpp"
 #include "sfb_lowl.hpp"
 #include "sfb_healpix.hpp"
+#include "sfb_healpix_ring.hpp"
 #include "sfb_integrals.hpp"
 #include "sfb_ingest.hpp"
 #include "sfb_math.hpp"
@@ -70,6 +73,9 @@ struct KlnCache {
   int q5_lmax = 0;
   double q5_R = 0.0;
   double herm_R = 0.0;        // radius the tables cover: k_{l,n_l-1} * herm_R is the last argument
+  bool cub_valid = false;     // cubic-coefficient Bessel tables of the 512-thread mixed kernel
+  double cub_R = 0.0;
+  std::vector<int32_t> cub_nint;
 };
 
 struct sfbhip_handle {
@@ -86,7 +92,11 @@ struct sfbhip_handle {
   float last_ms = 0.f;
   KlnCache kc;
   int leg_lmax = -1;
+  int legs_lmax = -1;  // scaled Legendre constants of the 512-thread mixed kernel on the device
+  int lds_max = 65536;  // largest LDS allocation of one block (bytes)
   int pix_nside = -1;  // pixel tables on the device are for this nside
+  int ring_nside = -1;  // ring tables (cos/sin theta per ring, ring of each pixel) are for this nside
+  int rleg_nside = -1, rleg_lmax = -1;  // ring Legendre table is for this (nside, lmax)
   int ig_L = -1, ig_nz = -1;  // zero rows of the sbessel integrals on the device
 };
 
@@ -131,6 +141,8 @@ sfbhip_status validate(const sfbhip_params *p) {
   if (p->mixed_chunks != 0 && p->mixed_chunks != 1 && p->mixed_chunks != 2 && p->mixed_chunks != 4 &&
       p->mixed_chunks != 8)
     return fail(SFBHIP_INVALID_ARGUMENT, "mixed_chunks must be 0, 1, 2, 4 or 8");
+  if (p->mixed_kernel < SFBHIP_MIXED_KERNEL_AUTO || p->mixed_kernel > SFBHIP_MIXED_KERNEL_512)
+    return fail(SFBHIP_INVALID_ARGUMENT, "unknown mixed kernel");
   if (p->mixed_operands != SFBHIP_OPERANDS_FP32 && p->mixed_operands != SFBHIP_OPERANDS_SPLIT_FP16 &&
       p->mixed_operands != SFBHIP_OPERANDS_SPLIT3_FP16)
     return fail(SFBHIP_INVALID_ARGUMENT, "unknown mixed operand format");
@@ -148,6 +160,7 @@ sfbhip_status ensure_kln(sfbhip_handle *h, const sfbhip_params *p, hipStream_t s
   c.table_valid = false;
   c.herm_valid = false;
   c.q5_valid = false;
+  c.cub_valid = false;
   const int L = p->lmax, nz = p->nmax + 1;
   c.zeros.assign((size_t)std::max(L, 0) * nz, 0.0);
   c.kln.assign((size_t)std::max(L, 0) * p->nmax, 0.0);
@@ -757,6 +770,278 @@ sfbhip_status launch_plan_mixed(sfbhip_handle *h, const DirectPlan &pl, const st
                     used);
 }
 
+// ---- The 512-thread mixed kernel: its tables, plan and launches.
+
+// Scaled Legendre constants for l < lmax (LegendreScaled): with a_m, b_m of the
+// downward recurrence P^{m-1} = -(a_m cot P^m + b_m P^{m+1}), d_l = d_{l-1} = 1,
+// d_{m-1} = b_m d_{m+1}, all d divided by their minimum (so |Q| <= |P|), and
+// g_m = -a_m d_m / d_{m-1}.
+sfbhip_status ensure_legendre_scaled(sfbhip_handle *h, int lmax, hipStream_t st, LegendreScaled *ls) {
+  double2 *sect;
+  double *g, *d;
+  int32_t *off;
+  const size_t ncoef = (size_t)lmax * (lmax + 1) / 2 + 1;
+  CHK(get_buf(h, "legs_sect", (size_t)std::max(lmax, 1), &sect));
+  CHK(get_buf(h, "legs_g", ncoef, &g));
+  CHK(get_buf(h, "legs_d", ncoef, &d));
+  CHK(get_buf(h, "legs_off", (size_t)std::max(lmax, 1), &off));
+  if (h->legs_lmax < lmax) {
+    std::vector<double2> hs(std::max(lmax, 1));
+    std::vector<double> hg(ncoef, 0.0), hd(ncoef, 1.0), a, b, dd;
+    std::vector<int32_t> ho(std::max(lmax, 1));
+    double prod = 1.0;
+    int32_t o = 0;
+    for (int l = 0; l < lmax; ++l) {
+      if (l > 0) prod *= (2.0 * l - 1.0) / (2.0 * l);
+      double c = std::sqrt((2.0 * l + 1.0) / (4.0 * kPi) * prod);
+      if (l & 1) c = -c;
+      a.assign(l + 2, 0.0);
+      b.assign(l + 2, 0.0);
+      dd.assign(l + 2, 1.0);
+      for (int m = 1; m <= l; ++m) {
+        const double den = std::sqrt((double)(l + m) * (double)(l - m + 1));
+        a[m] = 2.0 * m / den;
+        b[m] = std::sqrt((double)(l - m) * (double)(l + m + 1)) / den;
+      }
+      for (int m = l - 1; m >= 1; --m) dd[m - 1] = b[m] * dd[m + 1];
+      double dmin = 1.0;
+      for (int m = 0; m <= l; ++m) dmin = std::min(dmin, dd[m]);
+      for (int m = 0; m <= l; ++m) dd[m] /= dmin;
+      ho[l] = o;
+      hg[o] = 0.0;
+      for (int m = 1; m <= l; ++m) hg[o + m] = -a[m] * dd[m] / dd[m - 1];
+      for (int m = 0; m <= l; ++m) hd[o + m] = dd[m];
+      int e;
+      const double mnt = std::frexp(c / dd[l], &e);
+      hs[l] = make_double2(mnt, (double)e);
+      o += l + 1;
+    }
+    HIPCHK(hipMemcpyAsync(sect, hs.data(), sizeof(double2) * hs.size(), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(g, hg.data(), sizeof(double) * hg.size(), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(d, hd.data(), sizeof(double) * hd.size(), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(off, ho.data(), sizeof(int32_t) * ho.size(), hipMemcpyHostToDevice, st));
+    HIPCHK(hipStreamSynchronize(st));  // host vectors go out of scope
+    h->legs_lmax = lmax;
+  }
+  *ls = LegendreScaled{sect, g, d, off};
+  return SFBHIP_OK;
+}
+
+// Cubic-coefficient Bessel tables for l < L: nint[l] = floor(xmax[l] / h) + 2
+// intervals, so that no argument k r <= xmax[l] leaves the table.  `tag`
+// names the workspace buffers.
+sfbhip_status build_cubic(sfbhip_handle *h, int L, const std::vector<double> &xmax, const std::string &tag,
+                          hipStream_t st, CubicTables *ct, std::vector<int32_t> *nint_out) {
+  std::vector<int32_t> off(std::max(L, 1), 0), ni(std::max(L, 1), 1);
+  int64_t o = 0;
+  int32_t mx = 1;
+  for (int l = 0; l < L; ++l) {
+    const int64_t n = (int64_t)std::floor(xmax[l] / MX_H) + 2;
+    if (n > MX_TAB_MAX)
+      return fail(SFBHIP_NOT_SUPPORTED, "mixed path: largest Bessel argument k r exceeds " +
+                                            std::to_string(MX_TAB_MAX * MX_H) + "; use precision FP64");
+    off[l] = (int32_t)o;
+    ni[l] = (int32_t)n;
+    o += n;
+    mx = std::max<int32_t>(mx, (int32_t)n);
+    if (o > INT32_MAX) return fail(SFBHIP_NOT_SUPPORTED, "mixed path: Bessel tables too large");
+  }
+  float4 *tab;
+  int32_t *doff, *dni;
+  CHK(get_buf(h, tag + "_tab", (size_t)std::max<int64_t>(o, 1), &tab));
+  CHK(get_buf(h, tag + "_off", off.size(), &doff));
+  CHK(get_buf(h, tag + "_nint", ni.size(), &dni));
+  HIPCHK(hipMemcpyAsync(doff, off.data(), sizeof(int32_t) * off.size(), hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(dni, ni.data(), sizeof(int32_t) * ni.size(), hipMemcpyHostToDevice, st));
+  if (L > 0) {
+    cubic_table_kernel<<<dim3((unsigned)cdiv(mx, 256), (unsigned)L), 256, 0, st>>>(L, doff, dni, tab);
+    HIPCHK(hipGetLastError());
+  }
+  HIPCHK(hipStreamSynchronize(st));  // host vectors go out of scope
+  *ct = CubicTables{tab, doff, dni};
+  if (nint_out) *nint_out = ni;
+  return SFBHIP_OK;
+}
+
+// The tables of a single call, kept while k_ln and the covered radius stay.
+sfbhip_status ensure_cubic(sfbhip_handle *h, const sfbhip_params *p, double rdata, hipStream_t st, CubicTables *ct) {
+  KlnCache &c = h->kc;
+  const int L = p->lmax;
+  const double R = std::max(p->rmax, rdata);
+  if (!c.cub_valid || c.cub_R < R) {
+    std::vector<double> xmax(std::max(L, 1), 0.0);
+    for (int l = 0; l < L; ++l)
+      if (c.nl[l] > 0) xmax[l] = c.kln[(size_t)l * p->nmax + c.nl[l] - 1] * R;
+    c.cub_valid = false;
+    CHK(build_cubic(h, L, xmax, "cub", st, ct, &c.cub_nint));
+    c.cub_R = R;
+    c.cub_valid = true;
+    return SFBHIP_OK;
+  }
+  float4 *tab;
+  int32_t *doff, *dni;
+  CHK(get_buf(h, "cub_tab", 1, &tab));
+  CHK(get_buf(h, "cub_off", 1, &doff));
+  CHK(get_buf(h, "cub_nint", 1, &dni));
+  *ct = CubicTables{tab, doff, dni};
+  return SFBHIP_OK;
+}
+
+// Row blocks of a 512-thread item of `rows` rows (<= 128).
+int m2_rb(int rows) { return rows > 64 ? 8 : rows > 32 ? 4 : rows > 16 ? 2 : 1; }
+int m2_rbi(int rb) { return rb == 8 ? 0 : rb == 4 ? 1 : rb == 2 ? 2 : 3; }
+
+// Items of the 512-thread kernel, by class ci + 3 rbi (QMAX 1, 2, 4; RB 8, 4,
+// 2, 1).  Per l the rows are cut into a segment of up to 128 rows while more
+// than 64 remain, then one of at most 64; each segment is split over point
+// ranges (multiples of 512 points) and into passes of at most qmax_cap chunks.
+void plan_mixed2(const std::vector<int32_t> &nl, int lmin, int lmax, int64_t npts, int qmax_cap,
+                 int64_t target_items, const PlanCtx &cx, DirectPlan *pl) {
+  struct Seg {
+    int l, n0, rows;
+  };
+  std::vector<Seg> segs;
+  int64_t passes_total = 0;
+  for (int l = lmax - 1; l >= lmin; --l) {
+    for (int n0 = 0; n0 < nl[l];) {
+      const int rem = nl[l] - n0;
+      const int rows = rem > 64 ? std::min(128, rem) : rem;
+      segs.push_back(Seg{l, n0, rows});
+      passes_total += cdiv(l / MX_MC + 1, qmax_cap);
+      n0 += rows;
+    }
+  }
+  if (segs.empty()) return;
+  constexpr int64_t TPM = 512;
+  int64_t S = std::max<int64_t>(1, cdiv(target_items, passes_total));
+  S = std::min<int64_t>(S, cdiv(npts, TPM));
+  const int64_t chunk = cdiv(cdiv(npts, S), TPM) * TPM;
+  S = cdiv(npts, chunk);
+  for (const Seg &sg : segs) {
+    const int l = sg.l;
+    const int Q = l / MX_MC + 1;
+    const int ncol = 2 * (l + 1);
+    const int rb = m2_rb(sg.rows);
+    ReduceGroup g;
+    g.l = l;
+    g.n0 = sg.n0;
+    g.nrows = sg.rows;
+    g.nsplit = (int32_t)S;
+    g.base = pl->partial_size;
+    g.stride = (int64_t)16 * rb * ncol;
+    g.fofs = cx.fbase + (int64_t)l * (l + 1) / 2 * cx.nmax;
+    g.kofs = cx.kbase + (int64_t)l * cx.nmax + sg.n0;
+    g.norm = cx.norm;
+    g.nmax = cx.nmax;
+    g.pad = 0;
+    pl->partial_size += S * g.stride;
+    pl->groups.push_back(g);
+    for (int64_t s = 0; s < S; ++s) {
+      for (int qlo = 0; qlo < Q; qlo += qmax_cap) {
+        DirectItem it;
+        it.l = l;
+        it.n0 = sg.n0;
+        it.nrows = sg.rows;
+        it.qlo = qlo;
+        it.qhi = std::min(Q, qlo + qmax_cap);
+        it.cat = cx.cat;
+        it.tab = (int32_t)std::min<double>(
+            std::floor(cx.kln[(size_t)l * cx.nmax + sg.n0 + sg.rows - 1] * cx.R / MX_H) + 2, 1e9);
+        it.pad = 0;
+        it.p0 = cx.pbase + s * chunk;
+        it.p1 = cx.pbase + std::min(npts, (s + 1) * chunk);
+        it.out = g.base + s * g.stride;
+        it.kofs = g.kofs;
+        const int qc = qclass(it.qhi - it.qlo);
+        const int ci = qc == 1 ? 0 : qc == 2 ? 1 : 2;
+        pl->items[ci + 3 * m2_rbi(rb)].push_back(it);
+      }
+    }
+  }
+}
+
+// Intervals of cubic coefficients a 512-thread block stages in LDS at most.
+constexpr int M2_LDS_NODES = 4608;
+
+template <int QMAX, int RB>
+sfbhip_status launch_mixed2(sfbhip_handle *h, int nitems, const DirectItem *items, hipStream_t st,
+                            const PointsSoA &pts, const double *kln, const LegendreScaled &ls, const CubicTables &ct,
+                            int nodes, double *partial) {
+  const void *fn = reinterpret_cast<const void *>(&sfb_direct_mixed2_kernel<QMAX, RB>);
+  hipFuncAttributes fa;
+  HIPCHK(hipFuncGetAttributes(&fa, fn));
+  const int cap = std::max(0, std::min(M2_LDS_NODES, (h->lds_max - (int)fa.sharedSizeBytes) / (int)sizeof(float4)));
+  nodes = std::min(nodes, cap);
+  const size_t shm = sizeof(float4) * (size_t)nodes;
+  if (shm > 0) HIPCHK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
+  sfb_direct_mixed2_kernel<QMAX, RB><<<nitems, M2_NT, shm, st>>>(pts, items, kln, ls, ct, nodes, partial);
+  HIPCHK(hipGetLastError());
+  return SFBHIP_OK;
+}
+
+template <int RB>
+sfbhip_status launch_mixed2_q(sfbhip_handle *h, int ci, int nitems, const DirectItem *items, hipStream_t st,
+                              const PointsSoA &pts, const double *kln, const LegendreScaled &ls,
+                              const CubicTables &ct, int nodes, double *partial) {
+  if (ci == 0) return launch_mixed2<1, RB>(h, nitems, items, st, pts, kln, ls, ct, nodes, partial);
+  if (ci == 1) return launch_mixed2<2, RB>(h, nitems, items, st, pts, kln, ls, ct, nodes, partial);
+  return launch_mixed2<4, RB>(h, nitems, items, st, pts, kln, ls, ct, nodes, partial);
+}
+
+// All classes of a 512-thread plan (uploaded in class order), dealt in turn to
+// the call's stream and the two side streams, forked from and joined back to st.
+sfbhip_status launch_mixed2_classes(sfbhip_handle *h, const DirectPlan &pl, const std::vector<int32_t> &nint,
+                                    DirectItem *ditems, size_t *off, hipStream_t st, const PointsSoA &pts,
+                                    const double *dk, const LegendreScaled &ls, const CubicTables &ct,
+                                    double *partial, bool *used) {
+  const hipStream_t streams[3] = {st, h->mx_side[0], h->mx_side[1]};
+  int next = 0;
+  for (int idx = 0; idx < 12; ++idx) {
+    const auto &v = pl.items[idx];
+    const int n = (int)v.size();
+    if (!n) continue;
+    const int ci = idx % 3, rbi = idx / 3;
+    int nodes = 0;
+    for (const DirectItem &it : v) nodes = std::max(nodes, std::min(lds_reach(it, nint) + 4, M2_LDS_NODES));
+    const DirectItem *di = ditems + *off;
+    const int si = next;
+    next = (next + 1) % 3;
+    const hipStream_t s = streams[si];
+    if (si > 0) used[si - 1] = true;
+    switch (rbi) {
+      case 0: CHK(launch_mixed2_q<8>(h, ci, n, di, s, pts, dk, ls, ct, nodes, partial)); break;
+      case 1: CHK(launch_mixed2_q<4>(h, ci, n, di, s, pts, dk, ls, ct, nodes, partial)); break;
+      case 2: CHK(launch_mixed2_q<2>(h, ci, n, di, s, pts, dk, ls, ct, nodes, partial)); break;
+      default: CHK(launch_mixed2_q<1>(h, ci, n, di, s, pts, dk, ls, ct, nodes, partial)); break;
+    }
+    *off += n;
+  }
+  return SFBHIP_OK;
+}
+
+sfbhip_status launch_plan_mixed2(sfbhip_handle *h, const DirectPlan &pl, const std::vector<int32_t> &nint,
+                                 DirectItem *ditems, size_t *off, hipStream_t st, const PointsSoA &pts,
+                                 const double *dk, const LegendreScaled &ls, const CubicTables &ct,
+                                 double *partial) {
+  CHK(fork_sides(h, st));
+  bool used[2] = {false, false};
+  return join_sides(h, st, launch_mixed2_classes(h, pl, nint, ditems, off, st, pts, dk, ls, ct, partial, used),
+                    used);
+}
+
+// The 512-thread kernel runs the mixed path's split-FP16 contraction when it
+// is asked for; AUTO takes the 256-thread kernel, measured faster on MI355X
+// (2^25 points, lmax 256: 4.34 s against 4.76 s; 1M clustered points: 6.70
+// against 6.96 ms): the 512-thread kernel issues 17 % fewer vector
+// instructions, but its eight waves per CU share one block's barriers and
+// wait 54 % of their cycles, where two independent 256-thread blocks per CU
+// cover each other's stalls.  QMAX <= 4 (its FP64 sums stay in registers).
+bool use_mixed2(const sfbhip_params *p) {
+  return p->precision == SFBHIP_PREC_MIXED && p->mixed_operands == SFBHIP_OPERANDS_SPLIT_FP16 &&
+         p->mixed_kernel == SFBHIP_MIXED_KERNEL_512;
+}
+int mixed2_qcap(const sfbhip_params *p) { return std::min(p->mixed_chunks ? p->mixed_chunks : 4, 4); }
+
 // The low-l FP64 items (all QMAX class 1), j_l from the quintic table.
 sfbhip_status launch_lowl(const DirectPlan &pl, DirectItem *ditems, size_t *off, hipStream_t st,
                           const PointsSoA &pts, const double *dk, const BesselTableRef &tab, double *partial) {
@@ -781,6 +1066,232 @@ sfbhip_status read_timings(sfbhip_handle *h, int dominant) {
   return SFBHIP_OK;
 }
 
+// ---- The HEALPix ring path (precision MIXED): FP32 maps, ring Legendre
+// table, split-FP16 ring contraction (sfb_healpix_ring.hpp).
+
+// Map workspace per batch of multipoles (bytes); l ranges whose maps exceed it
+// run in several batches.
+constexpr int64_t HR_MAP_CAP_BYTES = 16LL << 30;
+// Cubic-table intervals the map kernel stages in LDS at most (64 KB).
+constexpr int HR_LDS_NODES = 4096;
+
+template <int QMAX>
+void launch_ring_contract(int n, const DirectItem *di, hipStream_t s, const RingPixels &rp, const float *maps,
+                          int64_t npix, const int64_t *dmoff, double unscale, double *partial) {
+  healpix_ring_contract_kernel<QMAX><<<n, 256, 0, s>>>(rp, di, maps, npix, dmoff, unscale, partial);
+}
+
+sfbhip_status healpix_ring(sfbhip_handle *h, const int64_t *d_offsets, const double *d_radii, int32_t nside,
+                           int64_t npts, int maxcount, double rdata, const sfbhip_params *p, const PixelsSoA &pix,
+                           const LegendreTables &lt, double norm, double *dk, double *d_k_ln, double *d_c_ln,
+                           double *d_f_lmn, int *derr, hipStream_t st) {
+  const int L = p->lmax;
+  // multipoles l < lsplit run in the FP64 pixel kernel with exact j_l: their
+  // C_l have one to three degrees of freedom and sit closest to their noise
+  const int lsplit = std::min(L, std::max(0, p->mixed_fp64_below));
+  const int64_t npix = 12LL * nside * nside, nrings = 4LL * nside - 1;
+  const std::vector<int32_t> &nl = h->kc.nl;
+  // ring tables
+  double *rx, *rsn;
+  int32_t *ring_of;
+  CHK(get_buf(h, "hr_rx", (size_t)nrings, &rx));
+  CHK(get_buf(h, "hr_rs", (size_t)nrings, &rsn));
+  CHK(get_buf(h, "hr_ring_of", (size_t)npix, &ring_of));
+  if (h->ring_nside != nside) {
+    ring_setup_kernel<<<cdiv(npix, 256), 256, 0, st>>>(nside, rx, rsn, ring_of);
+    HIPCHK(hipGetLastError());
+    h->ring_nside = nside;
+  }
+  // ring Legendre table: nrings (l + 1) values per l
+  std::vector<int64_t> poff(std::max(L, 1), 0);
+  int64_t ptot = 0;
+  for (int l = 0; l < L; ++l) {
+    poff[l] = ptot;
+    ptot += nrings * (l + 1);
+  }
+  double *ptab;
+  int64_t *dpoff;
+  CHK(get_buf(h, "hr_ptab", (size_t)std::max<int64_t>(ptot, 1), &ptab));
+  CHK(get_buf(h, "hr_poff", poff.size(), &dpoff));
+  if (h->rleg_nside != nside || h->rleg_lmax < L) {
+    HIPCHK(hipMemcpyAsync(dpoff, poff.data(), sizeof(int64_t) * poff.size(), hipMemcpyHostToDevice, st));
+    if (L > 0) {
+      ring_legendre_kernel<<<dim3((unsigned)cdiv(nrings, 64), (unsigned)L), 64, 0, st>>>(L, (int)nrings, rx, rsn, lt,
+                                                                                         dpoff, ptab);
+      HIPCHK(hipGetLastError());
+    }
+    HIPCHK(hipStreamSynchronize(st));  // poff goes out of scope
+    h->rleg_nside = nside;
+    h->rleg_lmax = L;
+  }
+  // Bessel tables up to k_{l, n_l - 1} max(rmax, largest radius)
+  CubicTables ct{nullptr, nullptr, nullptr};
+  if (L > 0) CHK(ensure_cubic(h, p, rdata, st, &ct));
+  // map scale: |F| <= (largest pixel count) max |j_l| <= maxcount, and F 2^s <= 2^15
+  int sexp = 0;
+  while (sexp < 60 && std::ldexp((double)std::max(maxcount, 1), sexp + 1) <= 32768.0) ++sexp;
+  while (sexp > -60 && std::ldexp((double)std::max(maxcount, 1), sexp) > 32768.0) --sexp;
+  const double scale = std::ldexp(1.0, sexp);
+  const double unscale = std::ldexp(1.0, -(12 + sexp));
+
+  RingPixels rp;
+  rp.phi = pix.phi;
+  rp.cphi = pix.cphi;
+  rp.sphi = pix.sphi;
+  rp.phi2 = pix.phi2;
+  rp.cphi2 = pix.cphi2;
+  rp.sphi2 = pix.sphi2;
+  rp.np = pix.np;
+  rp.ring = ring_of;
+  rp.ptab = ptab;
+  rp.poff = dpoff;
+  rp.alias_compat = p->alias_compat;
+
+  // batches of multipoles whose maps fit the workspace
+  std::vector<std::pair<int, int>> batches;
+  {
+    int l0 = lsplit;
+    int64_t bytes = 0;
+    for (int l = lsplit; l < L; ++l) {
+      const int64_t b = (int64_t)nl[l] * npix * 4;
+      if (l > l0 && bytes + b > HR_MAP_CAP_BYTES) {
+        batches.push_back({l0, l});
+        l0 = l;
+        bytes = 0;
+      }
+      bytes += b;
+    }
+    if (L > lsplit) batches.push_back({l0, L});
+  }
+  int64_t maxmap = 1;
+  for (auto &bt : batches) {
+    int64_t b = 0;
+    for (int l = bt.first; l < bt.second; ++l) b += (int64_t)nl[l] * npix;
+    maxmap = std::max(maxmap, b);
+  }
+  float *maps;
+  int64_t *dmoff;
+  CHK(get_buf(h, "hr_maps", (size_t)maxmap, &maps));
+  CHK(get_buf(h, "hr_moff", (size_t)std::max(L, 1), &dmoff));
+
+  // one plan over all multipoles (the partial blocks of every batch), items launched per batch
+  PlanCtx cx;
+  cx.nmax = p->nmax;
+  cx.norm = norm;
+  DirectPlan pl64, pl;
+  if (lsplit > 0) plan_direct(nl, 0, lsplit, npix, F64_NT, F64_TP, F64_MC, 16, 4096, cx, &pl64);
+  pl.partial_size = pl64.partial_size;
+  pl.groups = pl64.groups;
+  std::vector<std::vector<DirectItem>> bitems(batches.size());
+  std::vector<std::array<int, 3>> bcounts(batches.size());
+  for (size_t b = 0; b < batches.size(); ++b) {
+    DirectPlan pb;
+    pb.partial_size = pl.partial_size;
+    plan_direct(nl, batches[b].first, batches[b].second, npix, HR_NT, HR_TP, MX_MC, 4, f64_target_items(npix, true) / 2,
+                cx, &pb);
+    pl.partial_size = pb.partial_size;
+    pl.groups.insert(pl.groups.end(), pb.groups.begin(), pb.groups.end());
+    for (int ci = 0; ci < 3; ++ci) {
+      bitems[b].insert(bitems[b].end(), pb.items[ci].begin(), pb.items[ci].end());
+      bcounts[b][ci] = (int)pb.items[ci].size();
+    }
+    if (!pb.items[3].empty() || !pb.items[4].empty()) return fail(SFBHIP_NOT_SUPPORTED, "ring plan class");
+  }
+  double *partial;
+  CHK(get_buf(h, "partial", (size_t)std::max<int64_t>(pl.partial_size, 1), &partial));
+
+  HIPCHK(hipEventRecord(h->ev[0], st));
+  if (lsplit > 0) {
+    std::vector<DirectItem> lo;
+    for (auto &v : pl64.items) lo.insert(lo.end(), v.begin(), v.end());
+    DirectItem *dlo;
+    CHK(get_buf(h, "hr_lowl_items", std::max<size_t>(lo.size(), 1), &dlo));
+    if (!lo.empty())
+      HIPCHK(hipMemcpyAsync(dlo, lo.data(), sizeof(DirectItem) * lo.size(), hipMemcpyHostToDevice, st));
+    size_t off = 0;
+    CHK(launch_plan_f64(h, pl64, dlo, &off, st, PointsSoA{}, &pix, dk, p->nmax, lt, JL_EXACT, BesselTableRef{}, partial,
+                        derr));
+    HIPCHK(hipStreamSynchronize(st));  // lo goes out of scope
+  }
+  float map_ms = 0.f;
+  for (size_t b = 0; b < batches.size(); ++b) {
+    const int lb0 = batches[b].first, lb1 = batches[b].second;
+    // map offsets within the batch's workspace, and the map items
+    std::vector<int64_t> moff(std::max(L, 1), 0);
+    std::vector<MapItem> mitems;
+    int64_t o = 0, nmaprows = 0;
+    for (int l = lb0; l < lb1; ++l) {
+      moff[l] = o;
+      o += (int64_t)nl[l] * npix;
+      nmaprows += cdiv(nl[l], HR_NB);
+    }
+    const int64_t target = 8192;
+    int64_t S = std::max<int64_t>(1, cdiv(target, std::max<int64_t>(nmaprows, 1)));
+    S = std::min<int64_t>(S, cdiv(npix, 256));
+    const int64_t chunk = cdiv(cdiv(npix, S), 256) * 256;
+    int nodes = 1;
+    for (int l = lb0; l < lb1; ++l) {
+      if (nl[l] == 0) continue;
+      nodes = std::max(nodes, h->kc.cub_nint[l]);
+      for (int n0 = 0; n0 < nl[l]; n0 += HR_NB)
+        for (int64_t p0 = 0; p0 < npix; p0 += chunk) {
+          MapItem mi;
+          mi.l = l;
+          mi.n0 = n0;
+          mi.nrows = std::min(HR_NB, nl[l] - n0);
+          mi.pad = 0;
+          mi.p0 = p0;
+          mi.p1 = std::min(npix, p0 + chunk);
+          mi.moff = moff[l];
+          mi.kofs = (int64_t)l * p->nmax + n0;
+          mitems.push_back(mi);
+        }
+    }
+    nodes = std::min(nodes, HR_LDS_NODES);
+    MapItem *dmi;
+    DirectItem *ditems;
+    CHK(get_buf(h, "hr_mitems", std::max<size_t>(mitems.size(), 1), &dmi));
+    CHK(get_buf(h, "items", std::max<size_t>(bitems[b].size(), 1), &ditems));
+    HIPCHK(hipMemcpyAsync(dmoff, moff.data(), sizeof(int64_t) * moff.size(), hipMemcpyHostToDevice, st));
+    if (!mitems.empty())
+      HIPCHK(hipMemcpyAsync(dmi, mitems.data(), sizeof(MapItem) * mitems.size(), hipMemcpyHostToDevice, st));
+    if (!bitems[b].empty())
+      HIPCHK(hipMemcpyAsync(ditems, bitems[b].data(), sizeof(DirectItem) * bitems[b].size(), hipMemcpyHostToDevice, st));
+    const size_t shm = sizeof(float4) * (size_t)nodes;
+    if (!mitems.empty()) {
+      HIPCHK(hipFuncSetAttribute(reinterpret_cast<const void *>(&ring_maps_kernel),
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
+      ring_maps_kernel<<<(unsigned)mitems.size(), 256, shm, st>>>(dmi, d_offsets, d_radii, npix, dk, ct, nodes, scale,
+                                                                 maps);
+      HIPCHK(hipGetLastError());
+    }
+    if (batches.size() == 1) HIPCHK(hipEventRecord(h->ev[1], st));
+    size_t off = 0;
+    for (int ci = 0; ci < 3; ++ci) {
+      const int n = bcounts[b][ci];
+      if (!n) continue;
+      const DirectItem *di = ditems + off;
+      if (ci == 0) launch_ring_contract<1>(n, di, st, rp, maps, npix, dmoff, unscale, partial);
+      if (ci == 1) launch_ring_contract<2>(n, di, st, rp, maps, npix, dmoff, unscale, partial);
+      if (ci == 2) launch_ring_contract<4>(n, di, st, rp, maps, npix, dmoff, unscale, partial);
+      HIPCHK(hipGetLastError());
+      off += n;
+    }
+    if (batches.size() > 1) HIPCHK(hipStreamSynchronize(st));  // the next batch rewrites the maps and items
+  }
+  if (batches.size() != 1) HIPCHK(hipEventRecord(h->ev[1], st));
+  HIPCHK(hipEventRecord(h->ev[2], st));
+  CHK(finalize(h, p, pl.groups, partial, norm, d_k_ln, d_c_ln, d_f_lmn, st));
+  HIPCHK(hipEventRecord(h->ev[3], st));
+  HIPCHK(hipStreamSynchronize(st));
+  // phases: [0] maps, [1] ring contraction, [2] reduction and C_l; the dominant
+  // kernels are the map and contraction kernels together
+  for (int i = 0; i < 3; ++i) HIPCHK(hipEventElapsedTime(&h->phase_ms[i], h->ev[i], h->ev[i + 1]));
+  (void)map_ms;
+  h->last_ms = h->phase_ms[0] + h->phase_ms[1];
+  return SFBHIP_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -801,6 +1312,7 @@ sfbhip_status sfbhip_default_params(sfbhip_params *p) {
   p->mixed_fp64_below = 2;
   p->mixed_operands = SFBHIP_OPERANDS_SPLIT_FP16;
   p->mixed_chunks = 4;
+  p->mixed_kernel = SFBHIP_MIXED_KERNEL_AUTO;
   return SFBHIP_OK;
 }
 
@@ -816,6 +1328,13 @@ sfbhip_status sfbhip_create(sfbhip_handle **out, int device) {
   if (e != hipSuccess) {
     delete h;
     return fail(SFBHIP_HIP_ERROR, std::string("hipHostMalloc: ") + hipGetErrorString(e));
+  }
+  {
+    int a = 0, b = 0;
+    if (hipDeviceGetAttribute(&a, hipDeviceAttributeMaxSharedMemoryPerBlock, device) != hipSuccess) a = 0;
+    if (hipDeviceGetAttribute(&b, hipDeviceAttributeSharedMemPerBlockOptin, device) != hipSuccess) b = 0;
+    (void)hipGetLastError();
+    h->lds_max = std::max(65536, std::max(a, b));
   }
   for (auto &e : h->ev) HIPCHK(hipEventCreate(&e));
   HIPCHK(hipEventCreateWithFlags(&h->mx_fork, hipEventDisableTiming));
@@ -924,7 +1443,17 @@ sfbhip_status sfbhip_direct(sfbhip_handle *h, const double *d_points, int64_t n_
   if (use_table == JL_REFERENCE_TABLE) CHK(ensure_table(h, p, st, &tab));
   if (use_table == JL_QUINTIC && lsplit > 0) CHK(ensure_quintic(h, p, lsplit, rdata, st, &tab));
   HermiteTables ht{nullptr, nullptr, nullptr};
-  if (mixed && lsplit < p->lmax) CHK(ensure_hermite(h, p, rdata, st, &ht));
+  CubicTables ct{nullptr, nullptr, nullptr};
+  LegendreScaled ls{};
+  const bool m2 = use_mixed2(p);
+  if (mixed && lsplit < p->lmax) {
+    if (m2) {
+      CHK(ensure_cubic(h, p, rdata, st, &ct));
+      CHK(ensure_legendre_scaled(h, std::max(p->lmax, 1), st, &ls));
+    } else {
+      CHK(ensure_hermite(h, p, rdata, st, &ht));
+    }
+  }
   // mixed path: l < min(lsplit, LW_LMAX) in the low-l FP64 kernel, the rest of
   // l < lsplit in the general FP64 kernel (both with the quintic table)
   const int llow = mixed ? std::min(lsplit, LW_LMAX) : 0;
@@ -945,8 +1474,11 @@ sfbhip_status sfbhip_direct(sfbhip_handle *h, const double *d_points, int64_t n_
   plmx.partial_size = pl64.partial_size;
   if (mixed) {
     const int64_t target = mixed_target_items(n_points);
-    plan_direct(h->kc.nl, lsplit, p->lmax, n_points, MX_NT, MX_TP, MX_MC, p->mixed_chunks ? p->mixed_chunks : 4,
-                target, cx, &plmx, mixed_ks_cap(p->mixed_operands));
+    if (m2)
+      plan_mixed2(h->kc.nl, lsplit, p->lmax, n_points, mixed2_qcap(p), target, cx, &plmx);
+    else
+      plan_direct(h->kc.nl, lsplit, p->lmax, n_points, MX_NT, MX_TP, MX_MC, p->mixed_chunks ? p->mixed_chunks : 4,
+                  target, cx, &plmx, mixed_ks_cap(p->mixed_operands));
   }
   std::vector<ReduceGroup> groups = pllw.groups;
   groups.insert(groups.end(), pl64.groups.begin(), pl64.groups.end());
@@ -967,7 +1499,9 @@ sfbhip_status sfbhip_direct(sfbhip_handle *h, const double *d_points, int64_t n_
   CHK(launch_lowl(pllw, ditems, &off, st, pts, dk, tab, partial));
   CHK(launch_plan_f64(h, pl64, ditems, &off, st, pts, nullptr, dk, p->nmax, lt, use_table, tab, partial, derr));
   HIPCHK(hipEventRecord(h->ev[1], st));
-  if (mixed)
+  if (mixed && m2)
+    CHK(launch_plan_mixed2(h, plmx, h->kc.cub_nint, ditems, &off, st, pts, dk, ls, ct, partial));
+  else if (mixed)
     CHK(launch_plan_mixed(h, plmx, h->kc.herm_nint, ditems, &off, st, pts, dk, p->nmax, lt, ht,
                           p->mixed_operands, partial));
   HIPCHK(hipEventRecord(h->ev[2], st));
@@ -1158,7 +1692,15 @@ sfbhip_status sfbhip_direct_batched(sfbhip_handle *h, const sfbhip_catalog *cats
     tab.q5_nint = dni;
     HIPCHK(hipStreamSynchronize(st));  // host vectors go out of scope
   }
-  if (mixed && lsplit < L) {
+  const bool m2 = use_mixed2(p);
+  CubicTables ct{nullptr, nullptr, nullptr};
+  LegendreScaled ls{};
+  std::vector<int32_t> cnint;
+  if (mixed && lsplit < L && m2) {
+    CHK(build_cubic(h, L, xmax, "b_cub", st, &ct, &cnint));
+    CHK(ensure_legendre_scaled(h, std::max(L, 1), st, &ls));
+  }
+  if (mixed && lsplit < L && !m2) {
     std::vector<int32_t> off(L, 0);
     int64_t o = 0;
     int32_t mx = 1;
@@ -1220,9 +1762,14 @@ sfbhip_status sfbhip_direct_batched(sfbhip_handle *h, const sfbhip_catalog *cats
                 F64_TP, F64_MC, 16, target(c), ctx(c), &pl64);
   plmx.partial_size = pl64.partial_size;
   if (mixed) {
-    for (int c = 0; c < ncat; ++c)
-      plan_direct(nl[c], std::min(lsplit, cats[c].lmax), cats[c].lmax, cats[c].n_points, MX_NT, MX_TP, MX_MC,
-                  p->mixed_chunks ? p->mixed_chunks : 4, target(c), ctx(c), &plmx, mixed_ks_cap(p->mixed_operands));
+    for (int c = 0; c < ncat; ++c) {
+      if (m2)
+        plan_mixed2(nl[c], std::min(lsplit, cats[c].lmax), cats[c].lmax, cats[c].n_points, mixed2_qcap(p), target(c),
+                    ctx(c), &plmx);
+      else
+        plan_direct(nl[c], std::min(lsplit, cats[c].lmax), cats[c].lmax, cats[c].n_points, MX_NT, MX_TP, MX_MC,
+                    p->mixed_chunks ? p->mixed_chunks : 4, target(c), ctx(c), &plmx, mixed_ks_cap(p->mixed_operands));
+    }
   }
   std::vector<ReduceGroup> groups = pllw.groups;
   groups.insert(groups.end(), pl64.groups.begin(), pl64.groups.end());
@@ -1245,7 +1792,9 @@ sfbhip_status sfbhip_direct_batched(sfbhip_handle *h, const sfbhip_catalog *cats
   CHK(launch_plan_f64(h, pl64, ditems, &off, st, pts, nullptr, dk, 0, lt, mixed ? JL_QUINTIC : JL_EXACT, tab, partial,
                       derr));
   HIPCHK(hipEventRecord(h->ev[1], st));
-  if (mixed)
+  if (mixed && m2)
+    CHK(launch_plan_mixed2(h, plmx, cnint, ditems, &off, st, pts, dk, ls, ct, partial));
+  else if (mixed)
     CHK(launch_plan_mixed(h, plmx, hnint, ditems, &off, st, pts, dk, 0, lt, ht, p->mixed_operands, partial));
   HIPCHK(hipEventRecord(h->ev[2], st));
 
@@ -1347,8 +1896,8 @@ sfbhip_status sfbhip_healpix(sfbhip_handle *h, const int64_t *d_offsets, const d
   if (nside < 1 || (nside & (nside - 1)) || nside > 8192)
     return fail(SFBHIP_INVALID_ARGUMENT, "argument (nside) must be a power of 2");
   if (!d_offsets || !d_k_ln || !d_c_ln) return fail(SFBHIP_INVALID_ARGUMENT, "NULL buffer");
-  if (p->precision != SFBHIP_PREC_FP64)
-    return fail(SFBHIP_NOT_SUPPORTED, "the pixelised path runs in FP64 (precision must be SFBHIP_PREC_FP64)");
+  if (p->precision == SFBHIP_PREC_MIXED && p->mixed_operands != SFBHIP_OPERANDS_SPLIT_FP16)
+    return fail(SFBHIP_NOT_SUPPORTED, "the ring path (precision MIXED) uses split-FP16 operands");
   hipStream_t st = static_cast<hipStream_t>(stream);
   HIPCHK(hipSetDevice(h->device));
   const int64_t npix = 12LL * nside * nside;
@@ -1366,12 +1915,19 @@ sfbhip_status sfbhip_healpix(sfbhip_handle *h, const int64_t *d_offsets, const d
   csr_check_kernel<<<(unsigned)std::min<int64_t>(cdiv(std::max(npix, npts), 256), 8192), 256, 0, st>>>(
       d_offsets, npix, d_radii, npts, derr);
   HIPCHK(hipGetLastError());
+  int maxcount = 0;
+  double rdata = 0.0;
   {
-    int e0 = 0;
-    CHK(read_err(h, st, derr, &e0));
+    HIPCHK(hipMemcpyAsync(h->h_err, derr, 4 * sizeof(int), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    const int e0 = h->h_err[0];
     if (e0 & ERR_BADCSR)
       return fail(SFBHIP_INVALID_ARGUMENT, "pixel offsets are not a CSR of the radii (off[0] = 0, non-decreasing)");
     if (e0 & ERR_BADPOINT) return fail(SFBHIP_DOMAIN_ERROR, "point with r < 0 or a non-finite coordinate");
+    maxcount = h->h_err[1];
+    std::memcpy(&rdata, h->h_err + 2, sizeof(double));
+    fill_int_kernel<<<1, 64, 0, st>>>(derr, 4, 0);
+    HIPCHK(hipGetLastError());
   }
   if (p->verbose) {
     std::printf("SFB decomposition (reverse method): lmax=%d nmax=%d NSide=%d\n", p->lmax, p->nmax, nside);
@@ -1429,6 +1985,9 @@ sfbhip_status sfbhip_healpix(sfbhip_handle *h, const int64_t *d_offsets, const d
   const double norm = norm_factor(p, (double)npts);
   double *dk;
   CHK(get_buf(h, "kln", (size_t)std::max(p->lmax, 1) * p->nmax, &dk));
+  if (p->precision == SFBHIP_PREC_MIXED)
+    return healpix_ring(h, d_offsets, d_radii, nside, npts, maxcount, rdata, p, pix, lt, norm, dk, d_k_ln, d_c_ln,
+                        d_f_lmn, derr, st);
   const bool use_table = p->bessel == SFBHIP_BESSEL_REFERENCE || (p->bessel == SFBHIP_BESSEL_AUTO && npts > 1000);
   BesselTableRef tab{nullptr, nullptr, nullptr, 0};
   if (use_table) CHK(ensure_table(h, p, st, &tab));

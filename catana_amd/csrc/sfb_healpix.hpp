@@ -149,19 +149,29 @@ __global__ void csr_offsets_kernel(const uint32_t *__restrict__ keys, int64_t n,
 
 // Checks a caller's CSR before the pixelised contraction reads through it:
 // off[0] = 0, off non-decreasing up to off[npix] = npts (read by the host),
-// every radius finite and >= 0.  Grid-stride over max(npix, npts).
+// every radius finite and >= 0.  Grid-stride over max(npix, npts).  Also
+// err[1] = the largest pixel count and err[2..3] = the bits of the largest
+// radius (of a non-negative double, ordered like the value).
 __global__ void csr_check_kernel(const int64_t *__restrict__ off, int64_t npix, const double *__restrict__ radii,
                                  int64_t npts, int *__restrict__ err) {
   const int64_t n = npix > npts ? npix : npts;
-  int bad = 0;
+  int bad = 0, cmax = 0;
+  double rmx = 0.0;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-    if (i < npix && (off[i + 1] < off[i] || (i == 0 && off[0] != 0))) bad |= ERR_BADCSR;
+    if (i < npix) {
+      const int64_t c = off[i + 1] - off[i];
+      if (c < 0 || (i == 0 && off[0] != 0)) bad |= ERR_BADCSR;
+      cmax = max(cmax, (int)min<int64_t>(c, 0x7fffffff));
+    }
     if (i < npts) {
       const double r = radii[i];
       if (!(r >= 0.0) || !isfinite(r)) bad |= ERR_BADPOINT;
+      else rmx = fmax(rmx, r);
     }
   }
   if (bad) atomicOr(err, bad);
+  atomicMax(err + 1, cmax);
+  atomicMax(reinterpret_cast<unsigned long long *>(err + 2), (unsigned long long)__double_as_longlong(rmx));
 }
 
 // Sort keys of the pixelised contraction's pixel order: the point count of

@@ -1,0 +1,381 @@
+// sfb_healpix_ring.hpp -- the HEALPix SFB path on the ring structure of the map.
+//
+// The reference (sfb_reverse_fft.cpp:115-153, :174-209) evaluates Y_lm once per
+// ring, and for every (l, n) builds the map F(p) = sum_{r in p} j_l(k_ln r),
+// Fourier-transforms each ring and sums ring by ring.  Here the same three steps
+// are three kernels:
+//   1. ring_legendre_kernel: P_l^m(cos theta) once per (l, ring) -- 4 nside - 1
+//      FP64 recurrences per l instead of one per pixel;
+//   2. ring_maps_kernel: the FP32 maps F_l(n, p), streamed from the CSR radii,
+//      j_l from the cubic-coefficient table of l staged in LDS (FP64 argument
+//      split, three FP32 FMAs per value), summed per pixel in FP64, written
+//      once as FP32, coalesced by pixel;
+//   3. healpix_ring_contract_kernel: per l the ring DFT and the ring-wise
+//      Legendre sum as one matrix-core product over the pixels,
+//        f_lmn = sum_rings P_l^m(ring) sum_{j in ring} F(n, j) e^{-i m phi_j},
+//      with the maps (A operand, read from HBM) and Y* = P_ring e^{-i m phi_j}
+//      (B operand, the ring table times the pixel azimuth, FP64 Chebyshev
+//      recurrence) split into FP16 pieces: v_mfma_f32_16x16x32_f16, three
+//      products, FP32 partial sums folded into FP64 every 64 pixels.
+// With alias_compat the m whose folded FFT bin the reference reads without
+// conjugation ((m mod n_ring) > n_ring / 2) use the mirrored azimuth
+// phi' = 2 phi_0 - phi, as the FP64 pixel kernel does (YlmDownPix).
+#pragma once
+
+#include "sfb_common.hpp"
+#include "sfb_direct_mixed.hpp"
+#include "sfb_direct_mixed2.hpp"
+#include "sfb_healpix.hpp"
+#include "sfb_ylm.hpp"
+
+namespace sfb {
+
+constexpr int HR_NB = 16;   // map rows (n) per map item
+constexpr int HR_TP = 256;  // pixels per contraction tile
+constexpr int HR_NT = 64;   // rows (n) per contraction item
+
+// z = cos(theta), 1 - z and 1 + z of HEALPix ring i (1-based), exact in the caps.
+__host__ __device__ inline void hp_ring_z(int64_t ns, int64_t ring, double &z, double &omz, double &opz) {
+  if (ring < ns) {
+    omz = (double)(ring * ring) / (3.0 * ns * ns);
+    z = 1.0 - omz;
+    opz = 2.0 - omz;
+  } else if (ring < 3 * ns) {
+    z = (2.0 * ns - ring) * 2.0 / (3.0 * ns);
+    omz = 1.0 - z;
+    opz = 1.0 + z;
+  } else {
+    const int64_t nr = 4 * ns - ring;
+    opz = (double)(nr * nr) / (3.0 * ns * ns);
+    z = opz - 1.0;
+    omz = 2.0 - opz;
+  }
+}
+
+// Ring cos/sin theta (ring rho = i - 1) and the ring index of every pixel.
+__global__ void ring_setup_kernel(int nside, double *__restrict__ rx, double *__restrict__ rs,
+                                  int32_t *__restrict__ ring_of) {
+  const int64_t ns = nside, npix = 12 * ns * ns, nrings = 4 * ns - 1;
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p < nrings) {
+    double z, omz, opz;
+    hp_ring_z(ns, p + 1, z, omz, opz);
+    rx[p] = z;
+    rs[p] = sqrt(omz * opz);
+  }
+  if (p < npix) {
+    int64_t ring, start, np;
+    bool shifted;
+    hp_ring_of(ns, p, ring, start, np, shifted);
+    ring_of[p] = (int32_t)(ring - 1);
+  }
+}
+
+// ptab[poff[l] + rho (l + 1) + m] = P_l^m(cos theta_rho) 2^12 for m = 0..l (the
+// orthonormal functions with the Condon-Shortley phase, gsl_sf_legendre_sphPlm
+// as in sfb_reverse_fft.cpp:119), by the downward recurrence of YlmDown;
+// values below the FP64 range come out as 0.
+__global__ void ring_legendre_kernel(int lmax, int nrings, const double *__restrict__ rx,
+                                     const double *__restrict__ rs, LegendreTables leg,
+                                     const int64_t *__restrict__ poff, double *__restrict__ ptab) {
+  const int l = blockIdx.y;
+  const int rho = blockIdx.x * blockDim.x + threadIdx.x;
+  if (l >= lmax || rho >= nrings) return;
+  YlmDown y;
+  y.init(l, rx[rho], rs[rho], 0.0, 1.0, 0.0, leg.sect[l]);
+  const double2 *coef = leg.coef + leg.coef_off[l];
+  double *o = ptab + poff[l] + (int64_t)rho * (l + 1);
+  for (int m = l; m >= 0; --m) {
+    o[m] = ldexp(y.v, y.E + 12);
+    if (m > 0) {
+      y.step_p(coef[m]);
+      y.rescale();
+    }
+  }
+}
+
+// One map item: rows n0 .. n0 + nrows - 1 (<= HR_NB) of multipole l over
+// pixels [p0, p1); map row n of l starts at maps + moff + n npix.
+struct MapItem {
+  int32_t l, n0, nrows, pad;
+  int64_t p0, p1;
+  int64_t moff;
+  int64_t kofs;  // k_{l, n0} in the k_ln array
+};
+
+// j_l(t h) from a cubic table (t = k r / h >= 0 and floor(t) inside the table).
+__device__ __forceinline__ float cubic_at(const float4 *C, double t) {
+  const int i = (int)t;
+  const float f = (float)__builtin_amdgcn_fract(t);
+  const float4 c = C[i];
+  return fmaf(fmaf(fmaf(c.w, f, c.z), f, c.y), f, c.x);
+}
+
+template <class F>
+__device__ __forceinline__ void map_pixels(const MapItem &it, const int64_t *__restrict__ off,
+                                           const double *__restrict__ radii, int64_t npix, const double (&khr)[HR_NB],
+                                           double scale, float *__restrict__ maps, F jl) {
+  for (int64_t p = it.p0 + threadIdx.x; p < it.p1; p += blockDim.x) {
+    // FP64 pixel sums (a pixel may hold hundreds of radii whose j_l values
+    // cancel), rounded to FP32 once
+    double acc[HR_NB];
+#pragma unroll
+    for (int j = 0; j < HR_NB; ++j) acc[j] = 0.0;
+    const int64_t q1 = off[p + 1];
+    for (int64_t q = off[p]; q < q1; ++q) {
+      const double r = radii[q];
+#pragma unroll
+      for (int j = 0; j < HR_NB; ++j) acc[j] += (double)jl(r * khr[j]);
+    }
+#pragma unroll
+    for (int j = 0; j < HR_NB; ++j)
+      if (j < it.nrows) maps[it.moff + (int64_t)(it.n0 + j) * npix + p] = (float)(acc[j] * scale);
+  }
+}
+
+// maps[moff + n npix + p] = scale sum_{r in pixel p} j_l(k_ln r).  The table
+// of l covers k_{l, n_l - 1} max(rmax, max r) (so no index is clamped); it is
+// staged in LDS when it has at most lds_nodes intervals.  Rows past nrows use
+// k = 0 (t = 0) and are not written.
+__global__ __launch_bounds__(256) void ring_maps_kernel(const MapItem *__restrict__ items,
+                                                        const int64_t *__restrict__ off,
+                                                        const double *__restrict__ radii, int64_t npix,
+                                                        const double *__restrict__ kln, CubicTables ct,
+                                                        int lds_nodes, double scale, float *__restrict__ maps) {
+  extern __shared__ float4 TM[];
+  const MapItem it = items[blockIdx.x];
+  const int l = it.l;
+  const int nint = ct.nint[l];
+  const float4 *__restrict__ T = ct.tab + ct.off[l];
+  const bool tl = nint <= lds_nodes;  // block-uniform
+  if (tl)
+    for (int j = threadIdx.x; j < nint; j += blockDim.x) TM[j] = T[j];
+  __syncthreads();
+  double khr[HR_NB];
+#pragma unroll
+  for (int j = 0; j < HR_NB; ++j) khr[j] = j < it.nrows ? kln[it.kofs + j] * (1.0 / MX_H) : 0.0;
+  if (tl)
+    map_pixels(it, off, radii, npix, khr, scale, maps, [&](double t) { return cubic_at(TM, t); });
+  else
+    map_pixels(it, off, radii, npix, khr, scale, maps, [&](double t) { return cubic_at(T, t); });
+}
+
+// Per-pixel data of the ring contraction.
+struct RingPixels {
+  const double *phi, *cphi, *sphi;     // pixel azimuth phi_j and its cos/sin
+  const double *phi2, *cphi2, *sphi2;  // mirrored azimuth 2 phi_0 - phi_j
+  const int32_t *np;                   // ring length of the pixel
+  const int32_t *ring;                 // ring index (0-based) of the pixel
+  const double *ptab;                  // ring Legendre table (ring_legendre_kernel)
+  const int64_t *poff;
+  int32_t alias_compat;
+};
+
+// Y* 2^12 of one pixel for the 16 m of a chunk (from mtop down) into the
+// chunk buffer: two FP16 planes of 32 columns (2 mm: real, 2 mm + 1:
+// imaginary) x LDP pixels.  Leaves the azimuth 16 steps lower.  Columns of
+// m < 0 and padding pixels are zero.
+template <bool QUIRK, int LDP>
+__device__ __forceinline__ void ring_emit(_Float16 *Yh, int tid, int mtop, bool pok, const double *Pr, int nr,
+                                          int &smod, bool quirk, double &cm, double &sm, double &cp, double &sp,
+                                          double c2, double &cm2, double &sm2, double cph2, double sph2) {
+  constexpr int PLANE = 2 * MX_MC * LDP;
+#pragma unroll
+  for (int mm = 0; mm < MX_MC; ++mm) {
+    const int m = mtop - mm;
+    const double pr = Pr[m > 0 ? m : 0];
+    const double pv = (pok && m >= 0) ? pr : 0.0;
+    double c = cm, sn = sm;
+    if constexpr (QUIRK) {
+      const bool flip = quirk && m > 0 && 2 * smod > nr;
+      c = flip ? cm2 : cm;
+      sn = flip ? sm2 : sm;
+    }
+    const float re = (float)(pv * c), im = (float)(-pv * sn);
+    unsigned hi, lo;
+    split2(re, im, hi, lo);
+    unsigned short *Yu = reinterpret_cast<unsigned short *>(Yh);
+    Yu[(2 * mm) * LDP + tid] = (unsigned short)hi;
+    Yu[(2 * mm + 1) * LDP + tid] = (unsigned short)(hi >> 16);
+    Yu[PLANE + (2 * mm) * LDP + tid] = (unsigned short)lo;
+    Yu[PLANE + (2 * mm + 1) * LDP + tid] = (unsigned short)(lo >> 16);
+    // m -> m - 1
+    const double cn = fma(c2, cm, -cp), snn = fma(c2, sm, -sp);
+    cp = cm;
+    sp = sm;
+    cm = cn;
+    sm = snn;
+    if constexpr (QUIRK) {
+      const double cn2 = fma(cm2, cph2, sm2 * sph2), sn2 = fma(sm2, cph2, -cm2 * sph2);
+      cm2 = cn2;
+      sm2 = sn2;
+      smod = smod == 0 ? nr - 1 : smod - 1;
+    }
+  }
+}
+
+// Work item: DirectItem (l, rows n0 .. n0 + nrows - 1 <= 64, m chunks [qlo,
+// qhi) counted from the top, pixels [p0, p1), partial block at out).  Wave w
+// owns rows 16w .. 16w+15; map rows of l start at maps + moff[l].
+template <int QMAX>
+__global__ __launch_bounds__(256, 2) void healpix_ring_contract_kernel(RingPixels rp,
+                                                                    const DirectItem *__restrict__ items,
+                                                                    const float *__restrict__ maps, int64_t npix,
+                                                                    const int64_t *__restrict__ moff, double unscale,
+                                                                    double *__restrict__ partial) {
+  constexpr int LDP = HR_TP + 8;
+  constexpr int PLANE = 2 * MX_MC * LDP;
+  constexpr int FOLD = 64;
+  constexpr int SPW = HR_TP / 32;
+  __shared__ __attribute__((aligned(16))) _Float16 Yh[2 * PLANE];
+
+  const DirectItem it = items[blockIdx.x];
+  const int l = it.l;
+  const int ncol = 2 * (l + 1);
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
+  const int g = lane >> 4;
+  const int row = 16 * wave + (lane & 15);
+  const bool wave_rows = 16 * wave < it.nrows;  // wave-uniform
+  const bool rowok = row < it.nrows;
+  const float *__restrict__ M = maps + moff[l] + (int64_t)(it.n0 + (rowok ? row : 0)) * npix;
+  const double *__restrict__ P = rp.ptab + rp.poff[l];
+  const bool alias = rp.alias_compat != 0;
+
+  v4d sum[QMAX][2];
+#pragma unroll
+  for (int q = 0; q < QMAX; ++q)
+#pragma unroll
+    for (int c = 0; c < 2; ++c) sum[q][c] = v4d{0.0, 0.0, 0.0, 0.0};
+
+  const int mtop0 = l - MX_MC * it.qlo;
+  for (int64_t base = it.p0; base < it.p1; base += HR_TP) {
+    // ---- 1. map values of this wave's rows into A fragments (zero past the
+    // item's pixels or rows; item ranges and npix are multiples of 4)
+    h8 ah[2][SPW];
+    if (wave_rows) {
+      // loads stay inside the item (clamped) and are zeroed past it, without branches
+      const int64_t last = it.p1 - 4;
+#pragma unroll
+      for (int s = 0; s < SPW; ++s) {
+        unsigned hw[4], lw[4];
+#pragma unroll
+        for (int hq = 0; hq < 2; ++hq) {
+          const int64_t p = base + 32 * s + 8 * g + 4 * hq;
+          float4 v = *reinterpret_cast<const float4 *>(M + (p < last ? p : last));
+          const bool ok = rowok && p < it.p1;
+          v = ok ? v : make_float4(0.f, 0.f, 0.f, 0.f);
+          split2(v.x, v.y, hw[2 * hq], lw[2 * hq]);
+          split2(v.z, v.w, hw[2 * hq + 1], lw[2 * hq + 1]);
+        }
+        ah[0][s] = __builtin_bit_cast(h8, (u4v){hw[0], hw[1], hw[2], hw[3]});
+        ah[1][s] = __builtin_bit_cast(h8, (u4v){lw[0], lw[1], lw[2], lw[3]});
+      }
+    }
+
+    // ---- 2. azimuth of this thread's pixel at the item's top m
+    const int64_t gp = base + tid;
+    const bool pok = gp < it.p1;
+    const double *Pr = P;
+    int nr = 1, smod = 0;
+    double cm = 0.0, sm = 0.0, cp = 0.0, sp = 0.0, c2 = 0.0;
+    double cm2 = 0.0, sm2 = 0.0, cph2 = 0.0, sph2 = 0.0;
+    bool quirk = false;
+    if (pok) {
+      Pr = P + (int64_t)rp.ring[gp] * (l + 1);
+      nr = rp.np[gp];
+      const double phi = rp.phi[gp], cph = rp.cphi[gp], sph = rp.sphi[gp];
+      c2 = cph + cph;
+      const double hi = (double)mtop0 * phi, lo = fma((double)mtop0, phi, -hi);
+      double S, C;
+      sincos(hi, &S, &C);
+      cm = C - lo * S;
+      sm = S + lo * C;
+      cp = fma(cm, cph, -sm * sph);
+      sp = fma(sm, cph, cm * sph);
+      // the mirrored azimuth only matters where some m <= l folds past n_ring / 2
+      quirk = alias && 2 * l > nr;
+      if (quirk) {
+        const double phi2 = rp.phi2[gp];
+        const double h2 = (double)mtop0 * phi2, l2 = fma((double)mtop0, phi2, -h2);
+        sincos(h2, &S, &C);
+        cm2 = C - l2 * S;
+        sm2 = S + l2 * C;
+        cph2 = rp.cphi2[gp];
+        sph2 = rp.sphi2[gp];
+      }
+      smod = mtop0 % nr;
+    }
+
+    // waves without a polar pixel whose m fold skip the mirrored azimuth
+    const bool wave_quirk = __builtin_amdgcn_ballot_w64(quirk) != 0;
+#pragma unroll
+    for (int qq = 0; qq < QMAX; ++qq) {
+      const int q = it.qlo + qq;
+      if (q < it.qhi) {  // block-uniform
+        __syncthreads();  // previous chunk's readers are done
+        const int mtop = l - MX_MC * q;
+        if (wave_quirk)
+          ring_emit<true, LDP>(Yh, tid, mtop, pok, Pr, nr, smod, quirk, cm, sm, cp, sp, c2, cm2, sm2, cph2, sph2);
+        else
+          ring_emit<false, LDP>(Yh, tid, mtop, pok, Pr, nr, smod, quirk, cm, sm, cp, sp, c2, cm2, sm2, cph2, sph2);
+        __syncthreads();
+        if (wave_rows) {
+          const int col = lane & 15;
+          v4f a0 = v4f{0.f, 0.f, 0.f, 0.f}, a1 = a0, c0 = a0, c1 = a0;
+#pragma unroll
+          for (int s = 0; s < SPW; ++s) {
+            const int o = col * LDP + 32 * s + 8 * g;
+            const h8 b00 = *reinterpret_cast<const h8 *>(Yh + o);
+            const h8 b01 = *reinterpret_cast<const h8 *>(Yh + o + 16 * LDP);
+            const h8 b10 = *reinterpret_cast<const h8 *>(Yh + PLANE + o);
+            const h8 b11 = *reinterpret_cast<const h8 *>(Yh + PLANE + o + 16 * LDP);
+            c0 = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[0][s], b10, c0, 0, 0, 0);
+            c0 = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[1][s], b00, c0, 0, 0, 0);
+            c1 = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[0][s], b11, c1, 0, 0, 0);
+            c1 = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[1][s], b01, c1, 0, 0, 0);
+            a0 = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[0][s], b00, a0, 0, 0, 0);
+            a1 = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[0][s], b01, a1, 0, 0, 0);
+            if ((32 * (s + 1)) % FOLD == 0 && s + 1 < SPW) {
+#pragma unroll
+              for (int j = 0; j < 4; ++j) {
+                sum[qq][0][j] += (double)a0[j];
+                sum[qq][1][j] += (double)a1[j];
+              }
+              a0 = v4f{0.f, 0.f, 0.f, 0.f};
+              a1 = a0;
+            }
+          }
+#pragma unroll
+          for (int j = 0; j < 4; ++j) {
+            sum[qq][0][j] += (double)a0[j] + (double)c0[j];
+            sum[qq][1][j] += (double)a1[j] + (double)c1[j];
+          }
+        }
+      }
+    }
+  }
+
+  // ---- the item's partial block (64 rows x its columns), FP64
+  double *out = partial + it.out;
+#pragma unroll
+  for (int qq = 0; qq < QMAX; ++qq) {
+    const int q = it.qlo + qq;
+    if (q < it.qhi && wave_rows) {
+#pragma unroll
+      for (int c = 0; c < 2; ++c) {
+        const int lc = 16 * c + (lane & 15);
+        const int m = l - MX_MC * q - (lc >> 1);
+        if (m >= 0) {
+          const int cc = 2 * m + (lc & 1);
+          // v_mfma_f32_16x16x32_f16 result: element j of lane i is row 4(i/16) + j, column i%16
+#pragma unroll
+          for (int j = 0; j < 4; ++j) out[(size_t)(16 * wave + 4 * g + j) * ncol + cc] = sum[qq][c][j] * unscale;
+        }
+      }
+    }
+  }
+}
+
+}  // namespace sfb

@@ -79,12 +79,22 @@ typedef struct sfbhip_params {
   int32_t mixed_chunks;     /* mixed path: largest number of 16-wide m chunks one work item
                                accumulates (1, 2, 4 or 8; 0 = 4) -- more chunks reuse each Bessel
                                value more often but hold more accumulators in registers */
+  int32_t mixed_kernel;     /* mixed path, split-FP16 operands: SFBHIP_MIXED_KERNEL_* */
 } sfbhip_params;
+
+/* which contraction kernel the mixed path runs (split-FP16 operands; the other operand formats
+ * always run the 256-thread kernel) */
+enum {
+  SFBHIP_MIXED_KERNEL_AUTO = 0,  /* the 256-thread kernel (measured faster on MI355X) */
+  SFBHIP_MIXED_KERNEL_256 = 1,   /* 256-thread blocks, 64-row items, Bessel nodes in LDS */
+  SFBHIP_MIXED_KERNEL_512 = 2    /* 512-thread blocks, items of up to 128 rows, cubic Bessel
+                                    coefficients in LDS, double-buffered Y* halves */
+};
 
 typedef struct sfbhip_handle sfbhip_handle;
 
 /* Fills defaults: precision FP64, bessel AUTO, zeros REFERENCE, alias_compat 1,
- * mixed_fp64_below 2, mixed_operands SPLIT_FP16, mixed_chunks 4. */
+ * mixed_fp64_below 2, mixed_operands SPLIT_FP16, mixed_chunks 4, mixed_kernel AUTO. */
 sfbhip_status sfbhip_default_params(sfbhip_params *p);
 
 /* Library state bound to one device: workspace and cached k_ln tables.

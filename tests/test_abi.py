@@ -42,7 +42,9 @@ def test_default_params():
     assert p.mixed_fp64_below == 2
     assert p.mixed_operands == capi.OPERANDS_SPLIT_FP16
     assert p.mixed_chunks == 4
-    assert ctypes.sizeof(capi.Params) == 4 * 2 + 8 * 3 + 4 * 8
+    assert p.mixed_kernel == capi.MIXED_KERNEL_AUTO
+    # two ints, three doubles, nine ints, padded to the struct's 8-byte alignment
+    assert ctypes.sizeof(capi.Params) == 72
 
 
 def test_flmn_count():

@@ -49,11 +49,12 @@ def test_batched_fp64_vs_oracle(handle):
         assert _flmn_err(capi.split_flmn(f.cpu().numpy(), c["lmax"], c["nmax"]), fo) < 1e-10
 
 
-def test_batched_mixed_vs_fp64_and_single(handle):
+@pytest.mark.parametrize("kern", [capi.MIXED_KERNEL_256, capi.MIXED_KERNEL_512])
+def test_batched_mixed_vs_fp64_and_single(handle, kern):
     cats = _catalogs()
-    print("case: batched mixed vs batched FP64 and single calls", flush=True)
+    print(f"case: batched mixed vs batched FP64 and single calls, kernel {kern}", flush=True)
     r64 = capi.sfb_direct_batched_torch(handle, cats, store_flmn=True, precision=capi.PREC_FP64)
-    rmx = capi.sfb_direct_batched_torch(handle, cats, store_flmn=True, precision=capi.PREC_MIXED)
+    rmx = capi.sfb_direct_batched_torch(handle, cats, store_flmn=True, precision=capi.PREC_MIXED, mixed_kernel=kern)
     torch.cuda.synchronize()
     for c, (k64, c64, f64), (km, cm, fm) in zip(cats, r64, rmx):
         a, b = cm.cpu().numpy(), c64.cpu().numpy()
@@ -65,7 +66,7 @@ def test_batched_mixed_vs_fp64_and_single(handle):
         # the same catalogue alone through sfbhip_direct
         ks, cs, _ = capi.sfb_direct_torch(handle, c["points"], L, N, c["rmax"], k_max=c["k_max"],
                                           window_volume=c["window_volume"], precision=capi.PREC_MIXED,
-                                          bessel=capi.BESSEL_EXACT)
+                                          bessel=capi.BESSEL_EXACT, mixed_kernel=kern)
         s = cs.cpu().numpy()
         assert np.max(np.abs(a[m] / s[m] - 1)) < 1e-6
 
@@ -80,7 +81,8 @@ def test_batched_errors(handle):
         capi.sfb_direct_batched_torch(handle, [cats[0], bad], precision=capi.PREC_FP64)
 
 
-def test_batched_mixed_first_radius_not_smallest(handle):
+@pytest.mark.parametrize("kern", [capi.MIXED_KERNEL_256, capi.MIXED_KERNEL_512])
+def test_batched_mixed_first_radius_not_smallest(handle, kern):
     # the batched sort orders by r with its lowest 7 bits dropped, so within a
     # key group points keep their input order.  Here the first key group holds
     # radii on both sides of a Bessel node of the l = 4, n = 0 row (t = k r / h
@@ -113,7 +115,7 @@ def test_batched_mixed_first_radius_not_smallest(handle):
             dict(points=torch.from_numpy(ball).cuda(), lmax=lmax, nmax=nmax, rmax=rmax, k_max=0.0,
                  window_volume=0.0)]
     r64 = capi.sfb_direct_batched_torch(handle, cats, store_flmn=True, precision=capi.PREC_FP64)
-    rmx = capi.sfb_direct_batched_torch(handle, cats, store_flmn=True, precision=capi.PREC_MIXED)
+    rmx = capi.sfb_direct_batched_torch(handle, cats, store_flmn=True, precision=capi.PREC_MIXED, mixed_kernel=kern)
     torch.cuda.synchronize()
     for (_, _, f64), (_, _, fm) in zip(r64, rmx):
         a = capi.split_flmn(fm.cpu().numpy(), lmax, nmax)

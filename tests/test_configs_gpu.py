@@ -167,3 +167,32 @@ def test_healpix4m(handle):
         np.testing.assert_allclose(fa[l][:, :nl[l]], fo[l][:, :nl[l]], rtol=0,
                                    atol=TOL_FP64 * np.abs(fo[l]).max())
     print(f"healpix4m: FP64 HEALPix path vs oracle at l = {sample}", flush=True)
+
+
+@pytest.mark.timeout(600)
+def test_healpix4m_ring_path(handle):
+    # the ring path (FP32 maps, ring Legendre table, split-FP16 ring contraction)
+    # at full size against the FP64 HEALPix path with exact j_l, every (l, n)
+    cfg = bench.WORKLOADS["healpix4m"]
+    L, NM, R, ns = cfg["lmax"], cfg["nmax"], cfg["rmax"], cfg["nside"]
+    print("case: healpix4m ring path vs FP64 exact, nside 256, lmax 128", flush=True)
+    pts = workloads.radial_selection(4_000_000, R, r0=400.0, seed=3)
+    off, rad = capi.pixelize_torch(handle, torch.from_numpy(pts).cuda(), ns)
+    kw = dict(k_max=cfg["k_max"], store_flmn=True)
+    _, cr, fr = capi.sfb_healpix_torch(handle, off, rad, ns, L, NM, R, precision=capi.PREC_MIXED, **kw)
+    _, ce, fe = capi.sfb_healpix_torch(handle, off, rad, ns, L, NM, R, precision=capi.PREC_FP64,
+                                       bessel=capi.BESSEL_EXACT, **kw)
+    torch.cuda.synchronize()
+    # the ring path's stated tolerance: C_l <= 1e-6 for l >= 8, <= 5e-6 for 2 <= l < 8
+    # (l < 2 run in FP64), per-l f_lmn <= 1e-6
+    a, b = cr.cpu().numpy(), ce.cpu().numpy()
+    assert np.all((a == 0) == (b == 0))
+    m = b != 0
+    full = np.where(m, np.abs(a / np.where(m, b, 1) - 1), 0)
+    print(f"healpix4m ring vs FP64 exact: max rel C_l l < 8 {full[:8].max():.3e}, l >= 8 {full[8:].max():.3e}, "
+          f"median {np.median(full[m]):.3e}", flush=True)
+    assert full[8:].max() <= TOL_MIXED and full[:8].max() <= 5 * TOL_MIXED
+    fa, fb = capi.split_flmn(fr.cpu().numpy(), L, NM), capi.split_flmn(fe.cpu().numpy(), L, NM)
+    ferr = [np.linalg.norm(x - y) / np.linalg.norm(y) for x, y in zip(fa, fb) if np.linalg.norm(y) > 0]
+    print(f"healpix4m ring: max per-l rel f_lmn norm {max(ferr):.3e}", flush=True)
+    assert max(ferr) <= TOL_MIXED

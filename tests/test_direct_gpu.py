@@ -73,12 +73,13 @@ def _rel(a, b):
     return np.abs(a[m] / b[m] - 1)
 
 
-def test_mixed_vs_fp64_uniform(handle):
+@pytest.mark.parametrize("kern", [capi.MIXED_KERNEL_256, capi.MIXED_KERNEL_512])
+def test_mixed_vs_fp64_uniform(handle, kern):
     pts = workloads.uniform_ball(20000, 500.0, seed=21)
-    print("case: mixed vs fp64, uniform N=20000 lmax=24 k_max=0.15 rmax=500", flush=True)
+    print(f"kernel {kern} case: mixed vs fp64, uniform N=20000 lmax=24 k_max=0.15 rmax=500", flush=True)
     kw = dict(k_max=0.15, bessel=capi.BESSEL_EXACT)
     k64, c64, f64 = _run(handle, pts, 24, 30, 500.0, precision=capi.PREC_FP64, **kw)
-    km, cm, fm = _run(handle, pts, 24, 30, 500.0, precision=capi.PREC_MIXED, **kw)
+    km, cm, fm = _run(handle, pts, 24, 30, 500.0, precision=capi.PREC_MIXED, mixed_kernel=kern, **kw)
     np.testing.assert_array_equal(km, k64)
     rel = _rel(cm, c64)
     print("max rel c_ln", rel.max(), "median", np.median(rel))
@@ -87,49 +88,53 @@ def test_mixed_vs_fp64_uniform(handle):
     assert np.all((cm == 0) == (c64 == 0))
 
 
-def test_mixed_all_l_in_mixed_kernel(handle):
+@pytest.mark.parametrize("kern", [capi.MIXED_KERNEL_256, capi.MIXED_KERNEL_512])
+def test_mixed_all_l_in_mixed_kernel(handle, kern):
     # mixed_fp64_below = 0: every l through the FP32-operand kernel; f_lmn stays
     # within 1e-6 of FP64 relative to each l block's largest coefficient
     pts = workloads.uniform_ball(20000, 500.0, seed=24)
-    print("case: mixed kernel for all l, uniform N=20000 lmax=24", flush=True)
+    print(f"kernel {kern} case: mixed kernel for all l, uniform N=20000 lmax=24", flush=True)
     kw = dict(k_max=0.15, bessel=capi.BESSEL_EXACT)
     _, c64, f64 = _run(handle, pts, 24, 30, 500.0, precision=capi.PREC_FP64, **kw)
-    _, cm, fm = _run(handle, pts, 24, 30, 500.0, precision=capi.PREC_MIXED, mixed_fp64_below=0, **kw)
+    _, cm, fm = _run(handle, pts, 24, 30, 500.0, precision=capi.PREC_MIXED, mixed_kernel=kern, mixed_fp64_below=0, **kw)
     assert _flmn_err(fm, f64) < 1e-6
     assert np.median(_rel(cm, c64)) < 2e-7
 
 
-def test_mixed_vs_oracle_points_beyond_rmax(handle):
+@pytest.mark.parametrize("kern", [capi.MIXED_KERNEL_256, capi.MIXED_KERNEL_512])
+def test_mixed_vs_oracle_points_beyond_rmax(handle, kern):
     # r up to 1.3 rmax: Bessel arguments past the table fall back to FP64 j_l
     pts = workloads.uniform_ball(4000, 650.0, seed=22)
-    print("case: mixed, points beyond rmax, N=4000 lmax=10 nmax=8 rmax=500", flush=True)
-    k, c, f = _run(handle, pts, 10, 8, 500.0, precision=capi.PREC_MIXED, bessel=capi.BESSEL_EXACT)
+    print(f"kernel {kern} case: mixed, points beyond rmax, N=4000 lmax=10 nmax=8 rmax=500", flush=True)
+    k, c, f = _run(handle, pts, 10, 8, 500.0, precision=capi.PREC_MIXED, mixed_kernel=kern, bessel=capi.BESSEL_EXACT)
     ko, co, fo = O.sfb_raw(pts, 10, 8, 500.0, store_flmn=True)
     assert _rel(c, co).max() < 2e-6
     assert _flmn_err(f, fo) < 2e-6
 
 
-def test_mixed_vs_fp64_clustered_high_l(handle):
+@pytest.mark.parametrize("kern", [capi.MIXED_KERNEL_256, capi.MIXED_KERNEL_512])
+def test_mixed_vs_fp64_clustered_high_l(handle, kern):
     pts = workloads.clustered_ball(1500, 1000.0, mean_children=40, sigma=5.0, seed=23)
-    print(f"case: mixed vs fp64, clustered N={pts.shape[0]} lmax=140 k_max=0.2 rmax=1000", flush=True)
+    print(f"kernel {kern} case: mixed vs fp64, clustered N={pts.shape[0]} lmax=140 k_max=0.2 rmax=1000", flush=True)
     kw = dict(k_max=0.2, bessel=capi.BESSEL_EXACT)
     _, c64, f64 = _run(handle, pts, 140, 64, 1000.0, precision=capi.PREC_FP64, **kw)
-    _, cm, fm = _run(handle, pts, 140, 64, 1000.0, precision=capi.PREC_MIXED, **kw)
+    _, cm, fm = _run(handle, pts, 140, 64, 1000.0, precision=capi.PREC_MIXED, mixed_kernel=kern, **kw)
     rel = _rel(cm, c64)
     print("max rel c_ln", rel.max(), "median", np.median(rel))
     assert rel.max() < 1e-6
     assert _flmn_err(fm, f64) < 1e-6
 
 
-def test_mixed_vs_fp64_two_row_groups(handle):
+@pytest.mark.parametrize("kern", [capi.MIXED_KERNEL_256, capi.MIXED_KERNEL_512])
+def test_mixed_vs_fp64_two_row_groups(handle, kern):
     # n_l up to 76 > 64: multipoles with a second 64-row group (its 12 rows run
     # with the tile's points split four ways) and, for l >= 64, items whose m
     # chunks start below m = l
     pts = workloads.uniform_ball(3000, 600.0, seed=26)
-    print("case: mixed vs fp64, uniform N=3000 lmax=80 k_max=0.4 rmax=600 (n_l up to 76)", flush=True)
+    print(f"kernel {kern} case: mixed vs fp64, uniform N=3000 lmax=80 k_max=0.4 rmax=600 (n_l up to 76)", flush=True)
     kw = dict(k_max=0.4, bessel=capi.BESSEL_EXACT)
     k64, c64, f64 = _run(handle, pts, 80, 80, 600.0, precision=capi.PREC_FP64, **kw)
-    km, cm, fm = _run(handle, pts, 80, 80, 600.0, precision=capi.PREC_MIXED, **kw)
+    km, cm, fm = _run(handle, pts, 80, 80, 600.0, precision=capi.PREC_MIXED, mixed_kernel=kern, **kw)
     assert np.count_nonzero(c64[:, 64]) > 0  # some l has a 65th mode
     np.testing.assert_array_equal(km, k64)
     rel = _rel(cm, c64)
@@ -155,17 +160,20 @@ def test_mixed_fp64_multipoles_tables(handle):
     assert rhi < 1e-7 and hi < 1e-8
 
 
-@pytest.mark.parametrize("ops,chunks", [(capi.OPERANDS_FP32, 4), (capi.OPERANDS_SPLIT_FP16, 1),
-                                        (capi.OPERANDS_SPLIT_FP16, 8), (capi.OPERANDS_SPLIT3_FP16, 2)])
-def test_mixed_operand_formats_and_chunks(handle, ops, chunks):
+@pytest.mark.parametrize("ops,chunks,kern", [(capi.OPERANDS_FP32, 4, 0), (capi.OPERANDS_SPLIT_FP16, 1, 1),
+                                             (capi.OPERANDS_SPLIT_FP16, 8, 1), (capi.OPERANDS_SPLIT3_FP16, 2, 0),
+                                             (capi.OPERANDS_SPLIT_FP16, 1, 2), (capi.OPERANDS_SPLIT_FP16, 2, 2),
+                                             (capi.OPERANDS_SPLIT_FP16, 4, 2)])
+def test_mixed_operand_formats_and_chunks(handle, ops, chunks, kern):
     # every operand format and work-item chunk count against FP64; l up to 47
     # spans 3 chunks, so chunks=1 and 2 split the m range over several passes
     pts = workloads.clustered_ball(600, 800.0, mean_children=40, sigma=6.0, seed=26)
-    print(f"case: operands={ops} chunks={chunks} N={pts.shape[0]} lmax=48 k_max=0.2 rmax=800", flush=True)
+    print(f"case: operands={ops} chunks={chunks} kernel={kern} N={pts.shape[0]} lmax=48 k_max=0.2 rmax=800",
+          flush=True)
     kw = dict(k_max=0.2, bessel=capi.BESSEL_EXACT)
     _, c64, f64 = _run(handle, pts, 48, 60, 800.0, precision=capi.PREC_FP64, **kw)
     _, cm, fm = _run(handle, pts, 48, 60, 800.0, precision=capi.PREC_MIXED, mixed_operands=ops,
-                     mixed_chunks=chunks, **kw)
+                     mixed_chunks=chunks, mixed_kernel=kern, **kw)
     rel = _rel(cm, c64)
     print("max rel c_ln", rel.max(), "median", np.median(rel))
     assert rel.max() < 1e-6
@@ -204,3 +212,24 @@ def test_device_jl_vs_mpmath_goldens(handle, golden_dir):
         else:
             assert j <= 1e-240
     assert checked > 250
+
+
+@pytest.mark.parametrize("n,kern", [(400, capi.MIXED_KERNEL_512), (400, capi.MIXED_KERNEL_256),
+                                    (6000, capi.MIXED_KERNEL_512)])
+def test_mixed_lmax256_third_row_group_and_global_table(handle, n, kern):
+    # lmax 256, k_max 0.5, rmax 1000: n_l from 159 down to 52, so l >= 141 and rows
+    # n >= 128 (a third 64-row group, or the rows after a 128-row item) are
+    # exercised; with N = 400 one work item spans radii 0 .. rmax, its Bessel
+    # arguments cover more intervals than LDS holds and it reads the global table
+    pts = workloads.uniform_ball(n, 1000.0, seed=27)
+    print(f"kernel {kern} case: mixed vs fp64, uniform N={n} lmax=256 k_max=0.5 rmax=1000", flush=True)
+    kw = dict(k_max=0.5, bessel=capi.BESSEL_EXACT)
+    _, c64, f64 = _run(handle, pts, 256, 160, 1000.0, precision=capi.PREC_FP64, **kw)
+    _, cm, fm = _run(handle, pts, 256, 160, 1000.0, precision=capi.PREC_MIXED, mixed_kernel=kern, **kw)
+    assert np.count_nonzero(c64[:, 128:]) > 0 and np.count_nonzero(c64[141:]) > 0
+    rel = _rel(cm, c64)
+    print("max rel c_ln", rel.max(), "median", np.median(rel))
+    assert rel.max() < 1e-6
+    assert _rel(cm[:, 128:], c64[:, 128:]).max() < 1e-6
+    assert _flmn_err(fm, f64) < 1e-6
+    assert np.all((cm == 0) == (c64 == 0))
