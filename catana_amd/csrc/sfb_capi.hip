@@ -1208,6 +1208,8 @@ sfbhip_status healpix_ring(sfbhip_handle *h, const int64_t *d_offsets, const dou
     CHK(get_buf(h, "hr_lowl_items", std::max<size_t>(lo.size(), 1), &dlo));
     if (!lo.empty())
       HIPCHK(hipMemcpyAsync(dlo, lo.data(), sizeof(DirectItem) * lo.size(), hipMemcpyHostToDevice, st));
+    // exact j_l: some of these C_l sit at cancellation level, where even the
+    // quintic table's 1e-10 of the amplitude shows (a C_0n at nside 64 moved by 8e-5)
     size_t off = 0;
     CHK(launch_plan_f64(h, pl64, dlo, &off, st, PointsSoA{}, &pix, dk, p->nmax, lt, JL_EXACT, BesselTableRef{}, partial,
                         derr));

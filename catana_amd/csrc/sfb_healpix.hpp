@@ -152,7 +152,8 @@ __global__ void csr_offsets_kernel(const uint32_t *__restrict__ keys, int64_t n,
 // every radius finite and >= 0.  Grid-stride over max(npix, npts).  Also
 // err[1] = the largest pixel count and err[2..3] = the bits of the largest
 // radius (of a non-negative double, ordered like the value).
-__global__ void csr_check_kernel(const int64_t *__restrict__ off, int64_t npix, const double *__restrict__ radii,
+__global__ __launch_bounds__(256) void csr_check_kernel(const int64_t *__restrict__ off, int64_t npix,
+                                                        const double *__restrict__ radii,
                                  int64_t npts, int *__restrict__ err) {
   const int64_t n = npix > npts ? npix : npts;
   int bad = 0, cmax = 0;
@@ -169,9 +170,26 @@ __global__ void csr_check_kernel(const int64_t *__restrict__ off, int64_t npix, 
       else rmx = fmax(rmx, r);
     }
   }
-  if (bad) atomicOr(err, bad);
-  atomicMax(err + 1, cmax);
-  atomicMax(reinterpret_cast<unsigned long long *>(err + 2), (unsigned long long)__double_as_longlong(rmx));
+  // block reduction, then one atomic of each kind per block
+  __shared__ int sb[256], sc[256];
+  __shared__ double sr[256];
+  sb[threadIdx.x] = bad;
+  sc[threadIdx.x] = cmax;
+  sr[threadIdx.x] = rmx;
+  __syncthreads();
+  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+      sb[threadIdx.x] |= sb[threadIdx.x + w];
+      sc[threadIdx.x] = max(sc[threadIdx.x], sc[threadIdx.x + w]);
+      sr[threadIdx.x] = fmax(sr[threadIdx.x], sr[threadIdx.x + w]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    if (sb[0]) atomicOr(err, sb[0]);
+    atomicMax(err + 1, sc[0]);
+    atomicMax(reinterpret_cast<unsigned long long *>(err + 2), (unsigned long long)__double_as_longlong(sr[0]));
+  }
 }
 
 // Sort keys of the pixelised contraction's pixel order: the point count of

@@ -33,6 +33,7 @@ namespace sfb {
 constexpr int HR_NB = 16;   // map rows (n) per map item
 constexpr int HR_TP = 256;  // pixels per contraction tile
 constexpr int HR_NT = 64;   // rows (n) per contraction item
+constexpr int HR_RINGS = 16;  // rings of one contraction tile whose Legendre values are staged in LDS
 
 // z = cos(theta), 1 - z and 1 + z of HEALPix ring i (1-based), exact in the caps.
 __host__ __device__ inline void hp_ring_z(int64_t ns, int64_t ring, double &z, double &omz, double &opz) {
@@ -113,8 +114,12 @@ __device__ __forceinline__ float cubic_at(const float4 *C, double t) {
 
 template <class F>
 __device__ __forceinline__ void map_pixels(const MapItem &it, const int64_t *__restrict__ off,
-                                           const double *__restrict__ radii, int64_t npix, const double (&khr)[HR_NB],
-                                           double scale, float *__restrict__ maps, F jl) {
+                                           const double *__restrict__ radii, int64_t npix,
+                                           const double (&khr)[HR_NB], double scale, float *__restrict__ maps,
+                                           F jl) {
+  // one pixel per lane, in pixel order: a wave's radius reads and map writes
+  // stay contiguous (ordering each tile's pixels by point count, so that a
+  // wave's lanes loop over equally many radii, measured slower)
   for (int64_t p = it.p0 + threadIdx.x; p < it.p1; p += blockDim.x) {
     // FP64 pixel sums (a pixel may hold hundreds of radii whose j_l values
     // cancel), rounded to FP32 once
@@ -175,15 +180,19 @@ struct RingPixels {
 // chunk buffer: two FP16 planes of 32 columns (2 mm: real, 2 mm + 1:
 // imaginary) x LDP pixels.  Leaves the azimuth 16 steps lower.  Columns of
 // m < 0 and padding pixels are zero.
+// Pq[sgn * mm] is P_l^m at m = mtop - mm (sgn = 1: the LDS copy, already zero
+// for m < 0; sgn = -1: the global table, read at max(m, 0)).
 template <bool QUIRK, int LDP>
-__device__ __forceinline__ void ring_emit(_Float16 *Yh, int tid, int mtop, bool pok, const double *Pr, int nr,
-                                          int &smod, bool quirk, double &cm, double &sm, double &cp, double &sp,
-                                          double c2, double &cm2, double &sm2, double cph2, double sph2) {
+__device__ __forceinline__ void ring_emit(_Float16 *Yh, int tid, int mtop, bool pok, const double *Pq, int sgn,
+                                          int nr, int &smod, bool quirk, double &cm, double &sm, double &cp,
+                                          double &sp, double c2, double &cm2, double &sm2, double cph2, double sph2) {
   constexpr int PLANE = 2 * MX_MC * LDP;
 #pragma unroll
   for (int mm = 0; mm < MX_MC; ++mm) {
     const int m = mtop - mm;
-    const double pr = Pr[m > 0 ? m : 0];
+    // staged: index mm (the LDS copy holds zeros for m < 0, all inside it);
+    // global: index -(mm) = P at m, or P at m = 0 for m < 0 (masked below)
+    const double pr = sgn > 0 ? Pq[mm] : Pq[-(m >= 0 ? mm : mtop)];
     const double pv = (pok && m >= 0) ? pr : 0.0;
     double c = cm, sn = sm;
     if constexpr (QUIRK) {
@@ -228,6 +237,9 @@ __global__ __launch_bounds__(256, 2) void healpix_ring_contract_kernel(RingPixel
   constexpr int FOLD = 64;
   constexpr int SPW = HR_TP / 32;
   __shared__ __attribute__((aligned(16))) _Float16 Yh[2 * PLANE];
+  // ring Legendre values of the tile's rings for the item's m range:
+  // Ps[(rho - rho_lo) 16 QMAX + (mtop0 - m)]
+  __shared__ double Ps[HR_RINGS * MX_MC * QMAX];
 
   const DirectItem it = items[blockIdx.x];
   const int l = it.l;
@@ -274,16 +286,33 @@ __global__ __launch_bounds__(256, 2) void healpix_ring_contract_kernel(RingPixel
       }
     }
 
-    // ---- 2. azimuth of this thread's pixel at the item's top m
+    // ---- 2. the tile's ring Legendre values into LDS (visible after the
+    // first chunk's barrier; the previous tile's readers are past their last
+    // chunk barrier), unless the tile spans more than HR_RINGS rings
     const int64_t gp = base + tid;
     const bool pok = gp < it.p1;
+    const int64_t plast = min(base + HR_TP, it.p1) - 1;
+    const int rho_lo = rp.ring[base], nrt = rp.ring[plast] - rho_lo + 1;
+    const bool staged = nrt <= HR_RINGS;  // block-uniform
+    const int mspan = MX_MC * (it.qhi - it.qlo);
+    if (staged) {
+      for (int e = tid; e < nrt * MX_MC * QMAX; e += 256) {
+        const int rr = e / (MX_MC * QMAX), k = e - rr * (MX_MC * QMAX);
+        const int m = mtop0 - k;
+        Ps[e] = (k < mspan && m >= 0) ? P[(int64_t)(rho_lo + rr) * (l + 1) + m] : 0.0;
+      }
+    }
+    // ---- 3. azimuth of this thread's pixel at the item's top m
     const double *Pr = P;
+    const double *Pl = Ps;
     int nr = 1, smod = 0;
     double cm = 0.0, sm = 0.0, cp = 0.0, sp = 0.0, c2 = 0.0;
     double cm2 = 0.0, sm2 = 0.0, cph2 = 0.0, sph2 = 0.0;
     bool quirk = false;
     if (pok) {
-      Pr = P + (int64_t)rp.ring[gp] * (l + 1);
+      const int rho = rp.ring[gp];
+      Pr = P + (int64_t)rho * (l + 1);
+      if (staged) Pl = Ps + (rho - rho_lo) * MX_MC * QMAX;
       nr = rp.np[gp];
       const double phi = rp.phi[gp], cph = rp.cphi[gp], sph = rp.sphi[gp];
       c2 = cph + cph;
@@ -316,10 +345,13 @@ __global__ __launch_bounds__(256, 2) void healpix_ring_contract_kernel(RingPixel
       if (q < it.qhi) {  // block-uniform
         __syncthreads();  // previous chunk's readers are done
         const int mtop = l - MX_MC * q;
+        // Legendre values from LDS (staged) at index (mtop0 - m), else from the global table at m
+        const double *Pq = staged ? Pl + MX_MC * qq : Pr + mtop;
+        const int sgn = staged ? 1 : -1;
         if (wave_quirk)
-          ring_emit<true, LDP>(Yh, tid, mtop, pok, Pr, nr, smod, quirk, cm, sm, cp, sp, c2, cm2, sm2, cph2, sph2);
+          ring_emit<true, LDP>(Yh, tid, mtop, pok, Pq, sgn, nr, smod, quirk, cm, sm, cp, sp, c2, cm2, sm2, cph2, sph2);
         else
-          ring_emit<false, LDP>(Yh, tid, mtop, pok, Pr, nr, smod, quirk, cm, sm, cp, sp, c2, cm2, sm2, cph2, sph2);
+          ring_emit<false, LDP>(Yh, tid, mtop, pok, Pq, sgn, nr, smod, quirk, cm, sm, cp, sp, c2, cm2, sm2, cph2, sph2);
         __syncthreads();
         if (wave_rows) {
           const int col = lane & 15;
