@@ -23,7 +23,11 @@ EXPORTED = [
     "sfbhip_direct", "sfbhip_direct_batched", "sfbhip_pixelize", "sfbhip_ang2pix_ring", "sfbhip_healpix", "sfbhip_last_kernel_ms",
     "sfbhip_last_phase_ms", "sfbhip_last_error", "sfbhip_bessel_zeros", "sfbhip_sbessel_nodes",
     "sfbhip_sbessel_integrals", "sfbhip_filter_points", "sfbhip_random_subset",
+    "sfbhip_set_tile_table", "sfbhip_get_tile_table", "sfbhip_batch_plan_create", "sfbhip_batch_plan_destroy",
+    "sfbhip_batch_plan_enqueue", "sfbhip_batch_plan_capture", "sfbhip_batch_plan_launch", "sfbhip_batch_plan_sync",
+    "sfbhip_batch_autotune",
 ]
+TILE_BANDS = 6  # bands of l: [0,16) [16,32) [32,64) [64,128) [128,256) [256,512)
 RECORD_POINT, RECORD_CARTESIAN_F32, RECORD_CARTESIAN_F64, RECORD_SPHERICAL_F32, RECORD_SPHERICAL_F64 = range(5)
 FILTER_TOPHAT, FILTER_GAUSSIAN, FILTER_WINDOW_TABLE, FILTER_WINDOW_VALUES, FILTER_ANGULAR_MASK = range(5)
 WEIGHT_NODES, WEIGHT_TABLE = 0, 1
@@ -46,6 +50,26 @@ class Catalog(ctypes.Structure):
         ("window_volume", ctypes.c_double), ("d_k_ln", ctypes.c_void_p), ("d_c_ln", ctypes.c_void_p),
         ("d_f_lmn", ctypes.c_void_p),
     ]
+
+
+class TileTable(ctypes.Structure):
+    _fields_ = [
+        ("chunks", ctypes.c_int32 * TILE_BANDS), ("point_split", ctypes.c_int32 * TILE_BANDS),
+        ("items_per_cu", ctypes.c_int32),
+    ]
+
+    def as_dict(self):
+        return {"chunks": list(self.chunks), "point_split": list(self.point_split),
+                "items_per_cu": int(self.items_per_cu)}
+
+    @classmethod
+    def from_dict(cls, d):
+        t = cls()
+        for b in range(TILE_BANDS):
+            t.chunks[b] = int(d["chunks"][b])
+            t.point_split[b] = int(d["point_split"][b])
+        t.items_per_cu = int(d["items_per_cu"])
+        return t
 
 
 class Weight(ctypes.Structure):
@@ -97,6 +121,17 @@ def lib():
         L.sfbhip_filter_points.argtypes = [P, P, i64, i32, ctypes.c_double, ctypes.POINTER(Filter), i32,
                                            ctypes.c_uint64, ctypes.c_uint64, P, ctypes.POINTER(i64), P]
         L.sfbhip_random_subset.argtypes = [P, P, i64, i64, ctypes.c_uint64, P, P]
+        L.sfbhip_set_tile_table.argtypes = [P, ctypes.POINTER(TileTable)]
+        L.sfbhip_get_tile_table.argtypes = [P, ctypes.POINTER(TileTable)]
+        L.sfbhip_batch_plan_create.argtypes = [P, ctypes.POINTER(Catalog), i32, ctypes.POINTER(Params), P,
+                                               ctypes.POINTER(P)]
+        L.sfbhip_batch_plan_destroy.argtypes = [P]
+        L.sfbhip_batch_plan_enqueue.argtypes = [P, P]
+        L.sfbhip_batch_plan_capture.argtypes = [P, P]
+        L.sfbhip_batch_plan_launch.argtypes = [P, P]
+        L.sfbhip_batch_plan_sync.argtypes = [P, P, P]
+        L.sfbhip_batch_autotune.argtypes = [P, ctypes.POINTER(Catalog), i32, ctypes.POINTER(Params), P, i32,
+                                            ctypes.POINTER(TileTable)]
         L.sfbhip_last_error.restype = ctypes.c_char_p
         L.sfbhip_last_error.argtypes = []
         _lib = L
@@ -241,6 +276,105 @@ def sfb_direct_batched_torch(handle, catalogs, store_flmn=False, stream=None, **
                                      ctypes.c_void_p(stream) if stream else None)
     check(rc, "sfbhip_direct_batched")
     return outs
+
+
+def _catalog_array(catalogs, store_flmn):
+    """Catalog structs for the batched entry points, with freshly allocated
+    outputs: (array, [(k_ln, c_ln, f_lmn|None)], points kept alive)."""
+    torch = _torch()
+    arr = (Catalog * len(catalogs))()
+    outs, keep = [], []
+    for i, c in enumerate(catalogs):
+        pts = c["points"]
+        assert pts.dtype == torch.float64 and pts.dim() == 2 and pts.shape[1] == 3 and pts.is_contiguous()
+        lmax, nmax = int(c["lmax"]), int(c["nmax"])
+        k = torch.empty((lmax, nmax), dtype=torch.float64, device=pts.device)
+        cl = torch.empty((lmax, nmax), dtype=torch.float64, device=pts.device)
+        f = torch.empty(2 * flmn_count(lmax, nmax), dtype=torch.float64, device=pts.device) if store_flmn else None
+        arr[i] = Catalog(pts.data_ptr(), pts.shape[0], lmax, nmax, float(c["rmax"]), float(c.get("k_max", 0.0)),
+                         float(c.get("window_volume", 0.0)), k.data_ptr(), cl.data_ptr(),
+                         f.data_ptr() if f is not None else None)
+        keep.append(pts)
+        outs.append((k, cl, f))
+    return arr, outs, keep
+
+
+def _bounds(r_bound, n):
+    if r_bound is None:
+        return None, None
+    rb = np.ascontiguousarray(np.broadcast_to(np.asarray(r_bound, dtype=np.float64), (n,)))
+    return rb, ctypes.c_void_p(rb.ctypes.data)
+
+
+class BatchPlan:
+    """A plan of the batched mixed decomposition (sfbhip_batch_plan_*): binds
+    the catalogues' point tensors (contiguous float64 (N, 3) on the GPU, kept
+    alive here) and fresh output tensors.  Writing new points into the same
+    tensors and enqueueing (or replaying the captured graph) recomputes
+    `outputs`, a list of (k_ln, c_ln, f_lmn|None) per catalogue."""
+
+    def __init__(self, handle, catalogs, store_flmn=False, r_bound=None, **kw):
+        self._h = handle
+        p = make_params(1, 1, 1.0, precision=PREC_MIXED, **kw)
+        arr, self.outputs, self._keep = _catalog_array(catalogs, store_flmn)
+        self._rb, rbp = _bounds(r_bound, len(catalogs))
+        out = ctypes.c_void_p()
+        check(lib().sfbhip_batch_plan_create(handle.ptr, arr, len(catalogs), ctypes.byref(p), rbp,
+                                             ctypes.byref(out)), "sfbhip_batch_plan_create")
+        self._p = out
+
+    @staticmethod
+    def _s(stream):
+        return ctypes.c_void_p(stream) if stream else None
+
+    def enqueue(self, stream=None):
+        check(lib().sfbhip_batch_plan_enqueue(self._p, self._s(stream)), "sfbhip_batch_plan_enqueue")
+
+    def capture(self, stream=None):
+        check(lib().sfbhip_batch_plan_capture(self._p, self._s(stream)), "sfbhip_batch_plan_capture")
+
+    def launch(self, stream=None):
+        check(lib().sfbhip_batch_plan_launch(self._p, self._s(stream)), "sfbhip_batch_plan_launch")
+
+    def sync(self, stream=None):
+        """Waits, raises ValueError on the device's error flags, and returns the
+        timings (FP64 low l, grouped contraction, reduction, whole run) in ms."""
+        ms = (ctypes.c_float * 4)()
+        check(lib().sfbhip_batch_plan_sync(self._p, self._s(stream), ms), "sfbhip_batch_plan_sync")
+        return tuple(ms)
+
+    def close(self):
+        if getattr(self, "_p", None):
+            lib().sfbhip_batch_plan_destroy(self._p)
+            self._p = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def get_tile_table(handle):
+    t = TileTable()
+    check(lib().sfbhip_get_tile_table(handle.ptr, ctypes.byref(t)), "sfbhip_get_tile_table")
+    return t.as_dict()
+
+
+def set_tile_table(handle, table=None):
+    t = TileTable.from_dict(table) if table is not None else None
+    check(lib().sfbhip_set_tile_table(handle.ptr, ctypes.byref(t) if t is not None else None), "sfbhip_set_tile_table")
+
+
+def batch_autotune(handle, catalogs, reps=2, r_bound=None, **kw):
+    """Runs the autotuner on these catalogues; installs and returns the tile table (a dict)."""
+    p = make_params(1, 1, 1.0, precision=PREC_MIXED, **kw)
+    arr, _outs, _keep = _catalog_array(catalogs, False)
+    _rb, rbp = _bounds(r_bound, len(catalogs))
+    t = TileTable()
+    check(lib().sfbhip_batch_autotune(handle.ptr, arr, len(catalogs), ctypes.byref(p), rbp, int(reps),
+                                      ctypes.byref(t)), "sfbhip_batch_autotune")
+    return t.as_dict()
 
 
 def pixelize_torch(handle, points, nside, stream=None):

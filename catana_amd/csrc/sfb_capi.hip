@@ -52,6 +52,21 @@ constexpr double kPi = 3.14159265358979323846;
 
 int64_t cdiv(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
+void default_tiles(sfbhip_tile_table *t) {
+  for (int b = 0; b < SFBHIP_TILE_BANDS; ++b) {
+    t->chunks[b] = 4;
+    t->point_split[b] = 4;
+  }
+  t->items_per_cu = 64;
+}
+
+// band of the tile table: [0,16) [16,32) [32,64) [64,128) [128,256) [256,512)
+int tile_band(int l) {
+  int b = 0;
+  while (b < SFBHIP_TILE_BANDS - 1 && l >= (16 << b)) ++b;
+  return b;
+}
+
 }  // namespace
 
 struct DevBuf {
@@ -98,6 +113,8 @@ struct sfbhip_handle {
   int ring_nside = -1;  // ring tables (cos/sin theta per ring, ring of each pixel) are for this nside
   int rleg_nside = -1, rleg_lmax = -1;  // ring Legendre table is for this (nside, lmax)
   int ig_L = -1, ig_nz = -1;  // zero rows of the sbessel integrals on the device
+  int ncu = 256;  // compute units of the device
+  sfbhip_tile_table tiles;  // tile table of batch plans
 };
 
 namespace {
@@ -226,6 +243,31 @@ sfbhip_status ensure_table(sfbhip_handle *h, const sfbhip_params *p, hipStream_t
   return SFBHIP_OK;
 }
 
+// Host arrays of the Legendre constants for l < lmax (see LegendreTables):
+// hs, ho of max(lmax, 1) entries, hc of lmax (lmax + 1) / 2 + 1.
+void legendre_host(int lmax, std::vector<double2> &hs, std::vector<double2> &hc, std::vector<int32_t> &ho) {
+  hs.assign(std::max(lmax, 1), make_double2(0.0, 0.0));
+  hc.assign((size_t)lmax * (lmax + 1) / 2 + 1, make_double2(0.0, 0.0));
+  ho.assign(std::max(lmax, 1), 0);
+  double prod = 1.0;
+  int32_t o = 0;
+  for (int l = 0; l < lmax; ++l) {
+    if (l > 0) prod *= (2.0 * l - 1.0) / (2.0 * l);
+    double c = std::sqrt((2.0 * l + 1.0) / (4.0 * kPi) * prod);
+    if (l & 1) c = -c;
+    int e;
+    const double mnt = std::frexp(c, &e);
+    hs[l] = make_double2(mnt, (double)e);
+    ho[l] = o;
+    hc[o] = make_double2(0.0, 0.0);
+    for (int m = 1; m <= l; ++m) {
+      const double den = std::sqrt((double)(l + m) * (double)(l - m + 1));
+      hc[o + m] = make_double2(2.0 * m / den, std::sqrt((double)(l - m) * (double)(l + m + 1)) / den);
+    }
+    o += l + 1;
+  }
+}
+
 // Legendre constants for l < lmax (see LegendreTables).
 sfbhip_status ensure_legendre(sfbhip_handle *h, int lmax, hipStream_t st, LegendreTables *lt) {
   double2 *sect, *coef;
@@ -235,25 +277,9 @@ sfbhip_status ensure_legendre(sfbhip_handle *h, int lmax, hipStream_t st, Legend
   CHK(get_buf(h, "leg_coef", ncoef, &coef));
   CHK(get_buf(h, "leg_off", (size_t)std::max(lmax, 1), &off));
   if (h->leg_lmax < lmax) {
-    std::vector<double2> hs(std::max(lmax, 1)), hc(ncoef);
-    std::vector<int32_t> ho(std::max(lmax, 1));
-    double prod = 1.0;
-    int32_t o = 0;
-    for (int l = 0; l < lmax; ++l) {
-      if (l > 0) prod *= (2.0 * l - 1.0) / (2.0 * l);
-      double c = std::sqrt((2.0 * l + 1.0) / (4.0 * kPi) * prod);
-      if (l & 1) c = -c;
-      int e;
-      const double mnt = std::frexp(c, &e);
-      hs[l] = make_double2(mnt, (double)e);
-      ho[l] = o;
-      hc[o] = make_double2(0.0, 0.0);
-      for (int m = 1; m <= l; ++m) {
-        const double den = std::sqrt((double)(l + m) * (double)(l - m + 1));
-        hc[o + m] = make_double2(2.0 * m / den, std::sqrt((double)(l - m) * (double)(l + m + 1)) / den);
-      }
-      o += l + 1;
-    }
+    std::vector<double2> hs, hc;
+    std::vector<int32_t> ho;
+    legendre_host(lmax, hs, hc, ho);
     HIPCHK(hipMemcpyAsync(sect, hs.data(), sizeof(double2) * hs.size(), hipMemcpyHostToDevice, st));
     HIPCHK(hipMemcpyAsync(coef, hc.data(), sizeof(double2) * hc.size(), hipMemcpyHostToDevice, st));
     HIPCHK(hipMemcpyAsync(off, ho.data(), sizeof(int32_t) * ho.size(), hipMemcpyHostToDevice, st));
@@ -370,7 +396,7 @@ void plan_direct(const std::vector<int32_t> &nl, int lmin, int lmax, int64_t npt
           it.tab = cx.kln ? (int32_t)std::min<double>(
                                 std::floor(cx.kln[(size_t)l * cx.nmax + n0 + g.nrows - 1] * cx.R / MX_H) + 2, 1e9)
                           : 0;
-          it.pad = 0;
+          it.flags = 0;
           it.p0 = cx.pbase + s * chunk;
           it.p1 = cx.pbase + std::min(npts, (s + 1) * chunk);
           it.out = g.base + s * g.stride;
@@ -947,7 +973,7 @@ void plan_mixed2(const std::vector<int32_t> &nl, int lmin, int lmax, int64_t npt
         it.cat = cx.cat;
         it.tab = (int32_t)std::min<double>(
             std::floor(cx.kln[(size_t)l * cx.nmax + sg.n0 + sg.rows - 1] * cx.R / MX_H) + 2, 1e9);
-        it.pad = 0;
+        it.flags = 0;
         it.p0 = cx.pbase + s * chunk;
         it.p1 = cx.pbase + std::min(npts, (s + 1) * chunk);
         it.out = g.base + s * g.stride;
@@ -1337,7 +1363,12 @@ sfbhip_status sfbhip_create(sfbhip_handle **out, int device) {
     if (hipDeviceGetAttribute(&b, hipDeviceAttributeSharedMemPerBlockOptin, device) != hipSuccess) b = 0;
     (void)hipGetLastError();
     h->lds_max = std::max(65536, std::max(a, b));
+    int cu = 0;
+    if (hipDeviceGetAttribute(&cu, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess) cu = 0;
+    (void)hipGetLastError();
+    h->ncu = cu > 0 ? cu : 256;
   }
+  default_tiles(&h->tiles);
   for (auto &e : h->ev) HIPCHK(hipEventCreate(&e));
   HIPCHK(hipEventCreateWithFlags(&h->mx_fork, hipEventDisableTiming));
   for (int k = 0; k < 2; ++k) {
@@ -2363,6 +2394,674 @@ sfbhip_status sfbhip_random_subset(sfbhip_handle *h, const double *d_points, int
   gather_aos_kernel<<<(unsigned)cdiv(k, 256), 256, 0, st>>>(d_points, i0, k, d_out);
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(st));
+  return SFBHIP_OK;
+}
+
+}  // extern "C"
+
+// ---- Batch plans (catana_hip.h): every host step of a batched call done
+// once at creation; an enqueue launches only the library's kernels on one
+// stream, so it can be captured in a hipGraph.
+
+struct sfbhip_batch_plan {
+  int device = 0;
+  int ncat = 0;
+  std::vector<void *> mem;         // device allocations owned by the plan
+  hipStream_t own = nullptr;       // NULL stream argument: enqueue / launch here (synchronises with stream 0)
+  hipStream_t cap = nullptr;       // NULL stream argument: capture here
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+  int *h_err = nullptr;            // pinned copy of the error flags
+  int timed = 0;                   // last run: 0 none, 1 eager enqueue (phases), 2 graph launch (total)
+  // device data
+  int *derr = nullptr;             // [0] error bits, [4] work counter of the grouped launch
+  double *soa = nullptr;
+  int64_t ntot = 0;
+  CatPrep *dprep = nullptr;
+  int prep_bx = 1;
+  double *dk = nullptr;
+  int32_t *dnl = nullptr;
+  int64_t *dfoff = nullptr;
+  LegendreTables lt{};
+  HermiteTables ht{};
+  BesselTableRef q5{};
+  int lds_nodes = 0;
+  DirectItem *ditems = nullptr;
+  int nlow = 0;
+  int n64[5] = {0, 0, 0, 0, 0};
+  int nmx = 0;
+  int grid_mx = 1;
+  ReduceGroup *dgroups = nullptr;
+  int ngroups = 0, red_chunks = 1;
+  double *partial = nullptr;
+  double *f = nullptr;
+  int64_t ftot = 0;
+  CatOut *dout = nullptr;
+  int out_bx = 1;
+};
+
+namespace {
+
+template <class T>
+sfbhip_status plan_alloc(sfbhip_batch_plan *pl, size_t count, T **out) {
+  void *ptr = nullptr;
+  const size_t bytes = std::max<size_t>(count * sizeof(T), 256);
+  const hipError_t e = hipMalloc(&ptr, bytes);
+  if (e != hipSuccess)
+    return fail(SFBHIP_OUT_OF_MEMORY, "hipMalloc(" + std::to_string(bytes) + " bytes) for a batch plan: " +
+                                          hipGetErrorString(e));
+  pl->mem.push_back(ptr);
+  *out = static_cast<T *>(ptr);
+  return SFBHIP_OK;
+}
+
+template <class T>
+sfbhip_status plan_upload(sfbhip_batch_plan *pl, const std::vector<T> &v, T **out) {
+  CHK(plan_alloc(pl, v.size(), out));
+  if (!v.empty()) HIPCHK(hipMemcpy(*out, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice));
+  return SFBHIP_OK;
+}
+
+// Items of the grouped split-FP16 kernel for one catalogue: per (l, 64-row
+// tile) group, S point ranges of one length, each cut into passes of the
+// band's chunk count; an item of <= 16 (32) rows splits its tiles' points over
+// 4 (2) waves where the band allows.  Points are in input order, so every
+// item's Bessel window starts at x = 0 (ITEM_FULL_WINDOW).
+void plan_grouped(const std::vector<int32_t> &nl, int lmin, int lmax, int64_t npts, const sfbhip_tile_table &tt,
+                  int64_t target_items, const PlanCtx &cx, std::vector<DirectItem> *items,
+                  std::vector<ReduceGroup> *groups, int64_t *partial_size) {
+  int64_t passes_total = 0;
+  for (int l = lmin; l < lmax; ++l)
+    passes_total += cdiv(nl[l], MX_NT) * cdiv(l / MX_MC + 1, tt.chunks[tile_band(l)]);
+  if (passes_total == 0) return;
+  int64_t S = std::max<int64_t>(1, cdiv(target_items, passes_total));
+  S = std::min<int64_t>(S, cdiv(npts, MX_TP));
+  const int64_t chunk = cdiv(cdiv(npts, S), MX_TP) * MX_TP;
+  S = cdiv(npts, chunk);
+  for (int l = lmax - 1; l >= lmin; --l) {
+    const int b = tile_band(l);
+    const int qcap = tt.chunks[b], kscap = tt.point_split[b];
+    const int Q = l / MX_MC + 1;
+    const int ncol = 2 * (l + 1);
+    for (int n0 = 0; n0 < nl[l]; n0 += MX_NT) {
+      ReduceGroup g;
+      g.l = l;
+      g.n0 = n0;
+      g.nrows = std::min(MX_NT, nl[l] - n0);
+      g.nsplit = (int32_t)S;
+      g.base = *partial_size;
+      g.stride = (int64_t)MX_NT * ncol;
+      g.fofs = cx.fbase + (int64_t)l * (l + 1) / 2 * cx.nmax;
+      g.kofs = cx.kbase + (int64_t)l * cx.nmax + n0;
+      g.norm = cx.norm;
+      g.nmax = cx.nmax;
+      g.pad = 0;
+      *partial_size += S * g.stride;
+      groups->push_back(g);
+      int32_t ks = 0;
+      if (kscap >= 4 && g.nrows <= 16)
+        ks = ITEM_KS4;
+      else if (kscap >= 2 && g.nrows <= 32)
+        ks = ITEM_KS2;
+      const int32_t tab = (int32_t)std::min<double>(
+          std::floor(cx.kln[(size_t)l * cx.nmax + n0 + g.nrows - 1] * cx.R / MX_H) + 2, 1e9);
+      for (int64_t s = 0; s < S; ++s) {
+        for (int qlo = 0; qlo < Q; qlo += qcap) {
+          DirectItem it;
+          it.l = l;
+          it.n0 = n0;
+          it.nrows = g.nrows;
+          it.qlo = qlo;
+          it.qhi = std::min(Q, qlo + qcap);
+          it.cat = cx.cat;
+          it.tab = tab;
+          it.flags = ITEM_FULL_WINDOW | ks;
+          it.p0 = cx.pbase + s * chunk;
+          it.p1 = cx.pbase + std::min(npts, (s + 1) * chunk);
+          it.out = g.base + s * g.stride;
+          it.kofs = g.kofs;
+          items->push_back(it);
+        }
+      }
+    }
+  }
+}
+
+// Relative cost of a grouped item, for ordering the list longest first: the
+// Y* rows of its chunks and the Legendre steps down to its first chunk, per point.
+double item_cost(const DirectItem &it) {
+  return (double)(it.p1 - it.p0) * ((double)(it.qhi - it.qlo) + 0.1 * it.qlo + 0.3);
+}
+
+sfbhip_status plan_enqueue(sfbhip_batch_plan *pl, hipStream_t st, bool timed) {
+  if (timed) HIPCHK(hipEventRecord(pl->ev[0], st));
+  fill_int_kernel<<<1, 64, 0, st>>>(pl->derr, 8, 0);
+  HIPCHK(hipGetLastError());
+  prep_points_batch_kernel<<<dim3((unsigned)pl->prep_bx, (unsigned)pl->ncat), 256, 0, st>>>(pl->dprep, pl->soa, pl->ntot,
+                                                                                           pl->derr);
+  HIPCHK(hipGetLastError());
+  const int64_t n = pl->ntot;
+  const PointsSoA pts{pl->soa, pl->soa + n, pl->soa + 2 * n, pl->soa + 3 * n, pl->soa + 4 * n, pl->soa + 5 * n, n};
+  if (timed) HIPCHK(hipEventRecord(pl->ev[1], st));
+  size_t off = 0;
+  if (pl->nlow) {
+    sfb_lowl_f64_kernel<<<pl->nlow, 256, 0, st>>>(pts, pl->ditems, pl->dk, pl->q5, pl->partial);
+    HIPCHK(hipGetLastError());
+    off += pl->nlow;
+  }
+  for (int ci = 0; ci < 5; ++ci) {
+    const int k = pl->n64[ci];
+    if (!k) continue;
+    const DirectItem *di = pl->ditems + off;
+    switch (ci) {
+      case 0: launch_f64<1>(k, di, st, pts, nullptr, pl->dk, 0, pl->lt, JL_QUINTIC, pl->q5, pl->partial, pl->derr); break;
+      case 1: launch_f64<2>(k, di, st, pts, nullptr, pl->dk, 0, pl->lt, JL_QUINTIC, pl->q5, pl->partial, pl->derr); break;
+      case 2: launch_f64<4>(k, di, st, pts, nullptr, pl->dk, 0, pl->lt, JL_QUINTIC, pl->q5, pl->partial, pl->derr); break;
+      case 3: launch_f64<8>(k, di, st, pts, nullptr, pl->dk, 0, pl->lt, JL_QUINTIC, pl->q5, pl->partial, pl->derr); break;
+      default: launch_f64<16>(k, di, st, pts, nullptr, pl->dk, 0, pl->lt, JL_QUINTIC, pl->q5, pl->partial, pl->derr); break;
+    }
+    HIPCHK(hipGetLastError());
+    off += k;
+  }
+  if (timed) HIPCHK(hipEventRecord(pl->ev[2], st));
+  if (pl->nmx) {
+    sfb_direct_mixed_grouped_kernel<<<pl->grid_mx, 256, sizeof(float2) * (size_t)pl->lds_nodes, st>>>(
+        pts, pl->ditems + off, pl->nmx, pl->derr + 4, pl->dk, pl->lt, pl->ht, pl->lds_nodes, pl->partial);
+    HIPCHK(hipGetLastError());
+  }
+  if (timed) HIPCHK(hipEventRecord(pl->ev[3], st));
+  if (pl->ftot) {
+    fill_kernel<<<(unsigned)cdiv(2 * pl->ftot, 256), 256, 0, st>>>(pl->f, 2 * pl->ftot, 0.0);
+    HIPCHK(hipGetLastError());
+  }
+  if (pl->ngroups) {
+    reduce_partials_kernel<<<dim3((unsigned)pl->ngroups, (unsigned)pl->red_chunks), 256, 0, st>>>(pl->dgroups, pl->partial,
+                                                                                                pl->dk, pl->f);
+    HIPCHK(hipGetLastError());
+  }
+  batch_outputs_kernel<<<dim3((unsigned)pl->out_bx, (unsigned)pl->ncat), 256, 0, st>>>(pl->dout, pl->f, pl->dk, pl->dnl,
+                                                                                     pl->dfoff);
+  HIPCHK(hipGetLastError());
+  if (timed) HIPCHK(hipEventRecord(pl->ev[4], st));
+  return SFBHIP_OK;
+}
+
+sfbhip_status plan_create(sfbhip_handle *h, const sfbhip_catalog *cats, int32_t ncat, const sfbhip_params *p,
+                          const double *r_bound, sfbhip_batch_plan *pl) {
+  if (!cats || ncat < 1 || ncat > 128) return fail(SFBHIP_INVALID_ARGUMENT, "ncat must be in [1, 128]");
+  if (!p) return fail(SFBHIP_INVALID_ARGUMENT, "params is NULL");
+  if (p->precision != SFBHIP_PREC_MIXED || p->mixed_operands != SFBHIP_OPERANDS_SPLIT_FP16)
+    return fail(SFBHIP_NOT_SUPPORTED,
+                "batch plans run precision MIXED with SPLIT_FP16 operands; use sfbhip_direct_batched otherwise");
+  if (p->mixed_kernel == SFBHIP_MIXED_KERNEL_512)
+    return fail(SFBHIP_NOT_SUPPORTED, "batch plans run the 256-thread mixed kernel");
+  std::vector<sfbhip_params> cp(ncat, *p);
+  std::vector<double> R(ncat);
+  int64_t ntot = 0;
+  int L = 0, NZ = 1;
+  for (int c = 0; c < ncat; ++c) {
+    cp[c].lmax = cats[c].lmax;
+    cp[c].nmax = cats[c].nmax;
+    cp[c].rmax = cats[c].rmax;
+    cp[c].k_max = cats[c].k_max;
+    cp[c].window_volume = cats[c].window_volume;
+    CHK(validate(&cp[c]));
+    if (cats[c].n_points <= 0 || !cats[c].d_points)
+      return fail(SFBHIP_INVALID_ARGUMENT, "catalogue " + std::to_string(c) +
+                                               ": empty catalogue: the normalisation divides by the number of points");
+    if (!cats[c].d_k_ln || !cats[c].d_c_ln)
+      return fail(SFBHIP_INVALID_ARGUMENT, "catalogue " + std::to_string(c) + ": k_ln / c_ln output is NULL");
+    double rb = cats[c].rmax;
+    if (r_bound) {
+      if (!(r_bound[c] >= 0.0) || !std::isfinite(r_bound[c]))
+        return fail(SFBHIP_INVALID_ARGUMENT, "catalogue " + std::to_string(c) + ": r_bound must be finite and >= 0");
+      rb = std::max(rb, r_bound[c]);
+    }
+    R[c] = rb;
+    ntot += cats[c].n_points;
+    L = std::max(L, cats[c].lmax);
+    NZ = std::max(NZ, cats[c].nmax + 1);
+  }
+  if (ntot > INT32_MAX) return fail(SFBHIP_NOT_SUPPORTED, "more than 2^31 points in one batch");
+  if (L > MX_LMAX)
+    return fail(SFBHIP_NOT_SUPPORTED, "mixed path supports lmax <= " + std::to_string(MX_LMAX) + "; use precision FP64");
+  pl->ncat = ncat;
+  pl->ntot = ntot;
+
+  // zeros of j_l, l < L (the first nmax_c + 1 of them are those of a single call with nmax_c)
+  std::vector<double> zeros((size_t)std::max(L, 1) * NZ, 0.0);
+  if (L > 0) {
+    double *dz;
+    CHK(plan_alloc(pl, (size_t)L * NZ, &dz));
+    bessel_zeros_kernel<<<cdiv(L, 64), 64, 0, nullptr>>>(L, NZ, p->zeros, dz);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpy(zeros.data(), dz, sizeof(double) * L * NZ, hipMemcpyDeviceToHost));
+  }
+  // k_ln, n_l, f_lmn offsets per catalogue
+  std::vector<double> kcat;
+  std::vector<int32_t> nlcat;
+  std::vector<int64_t> foffcat, kbase(ncat), nlbase(ncat), fbase(ncat), foffbase(ncat), pbase(ncat);
+  std::vector<std::vector<int32_t>> nl(ncat);
+  int64_t ftot = 0, o = 0;
+  for (int c = 0; c < ncat; ++c) {
+    const int Lc = cats[c].lmax, Nc = cats[c].nmax;
+    const double kmax = cats[c].k_max > 0 ? cats[c].k_max : 0.0;
+    pbase[c] = o;
+    o += cats[c].n_points;
+    kbase[c] = (int64_t)kcat.size();
+    nlbase[c] = (int64_t)nlcat.size();
+    foffbase[c] = (int64_t)foffcat.size();
+    fbase[c] = ftot;
+    nl[c].assign(std::max(Lc, 1), 0);
+    for (int l = 0; l < Lc; ++l) {
+      int cnt = 0;
+      for (int n = 0; n < Nc; ++n) {
+        const double z = zeros[(size_t)l * NZ + n];
+        kcat.push_back(z / cats[c].rmax);
+        if (kmax <= 0.0 || z < kmax * cats[c].rmax) cnt = n + 1;
+      }
+      nl[c][l] = cnt;
+    }
+    nlcat.insert(nlcat.end(), nl[c].begin(), nl[c].end());
+    const std::vector<int64_t> fo = flmn_offsets(Lc, Nc);
+    foffcat.insert(foffcat.end(), fo.begin(), fo.end());
+    ftot += (int64_t)sfbhip_flmn_count(Lc, Nc);
+  }
+  CHK(plan_upload(pl, kcat, &pl->dk));
+  CHK(plan_upload(pl, nlcat, &pl->dnl));
+  CHK(plan_upload(pl, foffcat, &pl->dfoff));
+  pl->ftot = ftot;
+
+  // tables shared by all catalogues
+  {
+    std::vector<double2> hs, hc;
+    std::vector<int32_t> ho;
+    legendre_host(std::max(L, 1), hs, hc, ho);
+    double2 *ds, *dc;
+    int32_t *dof;
+    CHK(plan_upload(pl, hs, &ds));
+    CHK(plan_upload(pl, hc, &dc));
+    CHK(plan_upload(pl, ho, &dof));
+    pl->lt = LegendreTables{ds, dc, dof};
+  }
+  const int lsplit = std::min(L, std::max(0, p->mixed_fp64_below));
+  std::vector<double> xmax(std::max(L, 1), 0.0);
+  for (int c = 0; c < ncat; ++c)
+    for (int l = 0; l < cats[c].lmax; ++l)
+      if (nl[c][l] > 0)
+        xmax[l] = std::max(xmax[l], kcat[kbase[c] + (int64_t)l * cats[c].nmax + nl[c][l] - 1] * R[c]);
+  if (lsplit > 0) {
+    std::vector<int32_t> qoff(lsplit), qni(lsplit);
+    int64_t qo = 0;
+    int32_t mx = 1;
+    for (int l = 0; l < lsplit; ++l) {
+      qni[l] = (int32_t)std::floor(xmax[l] * Q5_HINV) + 2;
+      qoff[l] = (int32_t)qo;
+      qo += qni[l];
+      mx = std::max(mx, qni[l]);
+    }
+    double *q5;
+    int32_t *doff, *dni;
+    CHK(plan_alloc(pl, (size_t)6 * qo, &q5));
+    CHK(plan_upload(pl, qoff, &doff));
+    CHK(plan_upload(pl, qni, &dni));
+    quintic_table_kernel<<<dim3((unsigned)cdiv(mx, 256), (unsigned)lsplit), 256, 0, nullptr>>>(lsplit, doff, dni, q5);
+    HIPCHK(hipGetLastError());
+    pl->q5.q5 = q5;
+    pl->q5.q5_off = doff;
+    pl->q5.q5_nint = dni;
+  }
+  if (lsplit < L) {
+    std::vector<int32_t> hoff(L, 0), hni(L, 1);
+    int64_t ho = 0;
+    int32_t mx = 1;
+    for (int l = 0; l < L; ++l) {
+      const int64_t k = (int64_t)std::floor(xmax[l] / MX_H) + 2;
+      if (k > MX_TAB_MAX)
+        return fail(SFBHIP_NOT_SUPPORTED, "mixed path: largest Bessel argument k r exceeds " +
+                                              std::to_string(MX_TAB_MAX * MX_H) + "; use precision FP64");
+      hni[l] = (int32_t)k;
+      hoff[l] = (int32_t)ho;
+      ho += k;
+      mx = std::max<int32_t>(mx, (int32_t)k);
+      if (ho > INT32_MAX) return fail(SFBHIP_NOT_SUPPORTED, "mixed path: Bessel tables too large");
+    }
+    float4 *tb;
+    int32_t *doff, *dni;
+    CHK(plan_alloc(pl, (size_t)ho, &tb));
+    CHK(plan_upload(pl, hoff, &doff));
+    CHK(plan_upload(pl, hni, &dni));
+    hermite_table_kernel<<<dim3((unsigned)cdiv(mx, 256), (unsigned)L), 256, 0, nullptr>>>(L, doff, dni, tb);
+    HIPCHK(hipGetLastError());
+    pl->ht = HermiteTables{tb, doff, dni};
+  }
+
+  // work items: FP64 low l, FP64 l < lsplit beyond the low-l kernel's range, grouped mixed
+  double wtot = 0.0;
+  std::vector<double> work(ncat, 0.0);
+  for (int c = 0; c < ncat; ++c) {
+    for (int l = 0; l < cats[c].lmax; ++l) work[c] += (double)(l + 1) * nl[c][l];
+    work[c] *= (double)cats[c].n_points;
+    wtot += work[c];
+  }
+  auto ctx = [&](int c) {
+    PlanCtx cx;
+    cx.nmax = cats[c].nmax;
+    cx.norm = norm_factor(&cp[c], (double)cats[c].n_points);
+    cx.kbase = kbase[c];
+    cx.fbase = fbase[c];
+    cx.pbase = pbase[c];
+    cx.cat = c;
+    cx.kln = kcat.data() + kbase[c];
+    cx.R = R[c];
+    return cx;
+  };
+  const sfbhip_tile_table &tt = h->tiles;
+  const double ttot_f64 = (double)mixed_target_items(ntot);
+  const double ttot_mx = (double)tt.items_per_cu * h->ncu;
+  auto share = [&](int c, double t) { return std::max<int64_t>(16, (int64_t)(t * work[c] / std::max(wtot, 1.0))); };
+  const int llow = std::min(lsplit, LW_LMAX);
+  DirectPlan pllw, pl64;
+  for (int c = 0; c < ncat; ++c)
+    if (llow > 0)
+      plan_direct(nl[c], 0, std::min(llow, cats[c].lmax), cats[c].n_points, LW_ROWS, LW_TILE, F64_MC, 1,
+                  share(c, ttot_f64), ctx(c), &pllw);
+  pl64.partial_size = pllw.partial_size;
+  for (int c = 0; c < ncat; ++c)
+    plan_direct(nl[c], std::min(llow, cats[c].lmax), std::min(lsplit, cats[c].lmax), cats[c].n_points, F64_NT, F64_TP,
+                F64_MC, 16, share(c, ttot_f64), ctx(c), &pl64);
+  std::vector<DirectItem> mx;
+  std::vector<ReduceGroup> groups = pllw.groups;
+  groups.insert(groups.end(), pl64.groups.begin(), pl64.groups.end());
+  int64_t psize = pl64.partial_size;
+  for (int c = 0; c < ncat; ++c)
+    plan_grouped(nl[c], std::min(lsplit, cats[c].lmax), cats[c].lmax, cats[c].n_points, tt, share(c, ttot_mx), ctx(c),
+                 &mx, &groups, &psize);
+  std::stable_sort(mx.begin(), mx.end(),
+                   [](const DirectItem &a, const DirectItem &b) { return item_cost(a) > item_cost(b); });
+  std::vector<DirectItem> all;
+  for (auto &v : pllw.items) all.insert(all.end(), v.begin(), v.end());
+  pl->nlow = (int)all.size();
+  for (int ci = 0; ci < 5; ++ci) {
+    all.insert(all.end(), pl64.items[ci].begin(), pl64.items[ci].end());
+    pl->n64[ci] = (int)pl64.items[ci].size();
+  }
+  if (!pl64.items[5].empty()) return fail(SFBHIP_NOT_SUPPORTED, "FP64 plan class beyond 16 chunks");
+  all.insert(all.end(), mx.begin(), mx.end());
+  if (all.size() > (size_t)INT32_MAX) return fail(SFBHIP_NOT_SUPPORTED, "too many work items");
+  pl->nmx = (int)mx.size();
+  CHK(plan_upload(pl, all, &pl->ditems));
+  CHK(plan_upload(pl, groups, &pl->dgroups));
+  pl->ngroups = (int)groups.size();
+  {
+    int chunks = 1;
+    for (const ReduceGroup &g : groups)
+      chunks = std::max<int>(chunks, (int)cdiv((int64_t)g.nrows * 2 * (g.l + 1), 256));
+    pl->red_chunks = std::min(chunks, 64);
+  }
+  CHK(plan_alloc(pl, (size_t)std::max<int64_t>(psize, 1), &pl->partial));
+  CHK(plan_alloc(pl, (size_t)std::max<int64_t>(2 * ftot, 1), &pl->f));
+  CHK(plan_alloc(pl, (size_t)6 * ntot, &pl->soa));
+  CHK(plan_alloc(pl, 8, &pl->derr));
+  int nodes = 0;
+  for (const DirectItem &it : mx) nodes = std::max(nodes, std::min(it.tab + 2, MX_LDS_NODES));
+  pl->lds_nodes = nodes;
+  pl->grid_mx = (int)std::max<int64_t>(1, std::min<int64_t>(pl->nmx, 2LL * h->ncu));
+  // the grouped kernel's dynamic LDS limit is set to the largest any plan uses
+  HIPCHK(hipFuncSetAttribute(reinterpret_cast<const void *>(&sfb_direct_mixed_grouped_kernel),
+                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)(sizeof(float2) * MX_LDS_NODES)));
+  // catalogue descriptors
+  std::vector<CatPrep> prep(ncat);
+  std::vector<CatOut> outs(ncat);
+  int64_t nmaxp = 1, omax = 1;
+  for (int c = 0; c < ncat; ++c) {
+    prep[c] = CatPrep{cats[c].d_points, cats[c].n_points, pbase[c], R[c]};
+    outs[c] = CatOut{cats[c].d_k_ln, cats[c].d_c_ln, cats[c].d_f_lmn, cats[c].lmax, cats[c].nmax,
+                     kbase[c], nlbase[c], foffbase[c], fbase[c]};
+    nmaxp = std::max<int64_t>(nmaxp, cats[c].n_points);
+    omax = std::max<int64_t>(omax, (int64_t)cats[c].lmax * cats[c].nmax);
+    if (cats[c].d_f_lmn) omax = std::max<int64_t>(omax, 2 * (int64_t)sfbhip_flmn_count(cats[c].lmax, cats[c].nmax));
+  }
+  CHK(plan_upload(pl, prep, &pl->dprep));
+  CHK(plan_upload(pl, outs, &pl->dout));
+  pl->prep_bx = (int)std::min<int64_t>(2048, cdiv(nmaxp, 1024));
+  pl->out_bx = (int)std::min<int64_t>(256, cdiv(omax, 1024));
+  HIPCHK(hipDeviceSynchronize());  // tables built, host vectors go out of scope
+  return SFBHIP_OK;
+}
+
+void plan_release(sfbhip_batch_plan *pl) {
+  if (pl->exec) (void)hipGraphExecDestroy(pl->exec);
+  if (pl->graph) (void)hipGraphDestroy(pl->graph);
+  for (void *ptr : pl->mem) (void)hipFree(ptr);
+  if (pl->h_err) (void)hipHostFree(pl->h_err);
+  for (auto &e : pl->ev)
+    if (e) (void)hipEventDestroy(e);
+  if (pl->own) (void)hipStreamDestroy(pl->own);
+  if (pl->cap) (void)hipStreamDestroy(pl->cap);
+  delete pl;
+}
+
+sfbhip_status check_tiles(const sfbhip_tile_table *t) {
+  for (int b = 0; b < SFBHIP_TILE_BANDS; ++b) {
+    if (t->chunks[b] != 1 && t->chunks[b] != 2 && t->chunks[b] != 4)
+      return fail(SFBHIP_INVALID_ARGUMENT, "tile table: chunks must be 1, 2 or 4");
+    if (t->point_split[b] != 1 && t->point_split[b] != 2 && t->point_split[b] != 4)
+      return fail(SFBHIP_INVALID_ARGUMENT, "tile table: point_split must be 1, 2 or 4");
+  }
+  if (t->items_per_cu < 1 || t->items_per_cu > 4096)
+    return fail(SFBHIP_INVALID_ARGUMENT, "tile table: items_per_cu must be in [1, 4096]");
+  return SFBHIP_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+sfbhip_status sfbhip_set_tile_table(sfbhip_handle *h, const sfbhip_tile_table *t) {
+  g_last_error.clear();
+  if (!h) return fail(SFBHIP_INVALID_ARGUMENT, "handle is NULL");
+  if (!t) {
+    default_tiles(&h->tiles);
+    return SFBHIP_OK;
+  }
+  CHK(check_tiles(t));
+  h->tiles = *t;
+  return SFBHIP_OK;
+}
+
+sfbhip_status sfbhip_get_tile_table(sfbhip_handle *h, sfbhip_tile_table *t) {
+  g_last_error.clear();
+  if (!h || !t) return fail(SFBHIP_INVALID_ARGUMENT, "NULL argument");
+  *t = h->tiles;
+  return SFBHIP_OK;
+}
+
+sfbhip_status sfbhip_batch_plan_create(sfbhip_handle *h, const sfbhip_catalog *cats, int32_t ncat,
+                                       const sfbhip_params *p, const double *r_bound, sfbhip_batch_plan **out) {
+  g_last_error.clear();
+  if (!h || !out) return fail(SFBHIP_INVALID_ARGUMENT, "handle or out is NULL");
+  *out = nullptr;
+  HIPCHK(hipSetDevice(h->device));
+  sfbhip_batch_plan *pl = new sfbhip_batch_plan();
+  pl->device = h->device;
+  sfbhip_status rc = SFBHIP_OK;
+  auto step = [&](hipError_t e, const char *what) {
+    if (rc == SFBHIP_OK && e != hipSuccess) rc = fail(SFBHIP_HIP_ERROR, std::string(what) + ": " + hipGetErrorString(e));
+  };
+  step(hipHostMalloc((void **)&pl->h_err, sizeof(int) * 8, hipHostMallocDefault), "hipHostMalloc");
+  for (auto &e : pl->ev) step(hipEventCreate(&e), "hipEventCreate");
+  step(hipStreamCreateWithFlags(&pl->own, hipStreamDefault), "hipStreamCreate");
+  step(hipStreamCreateWithFlags(&pl->cap, hipStreamNonBlocking), "hipStreamCreate");
+  if (rc == SFBHIP_OK) rc = plan_create(h, cats, ncat, p, r_bound, pl);
+  if (rc != SFBHIP_OK) {
+    (void)hipDeviceSynchronize();
+    plan_release(pl);
+    return rc;
+  }
+  *out = pl;
+  return SFBHIP_OK;
+}
+
+sfbhip_status sfbhip_batch_plan_destroy(sfbhip_batch_plan *plan) {
+  if (!plan) return SFBHIP_OK;
+  (void)hipSetDevice(plan->device);
+  (void)hipDeviceSynchronize();
+  plan_release(plan);
+  return SFBHIP_OK;
+}
+
+sfbhip_status sfbhip_batch_plan_enqueue(sfbhip_batch_plan *plan, void *stream) {
+  g_last_error.clear();
+  if (!plan) return fail(SFBHIP_INVALID_ARGUMENT, "plan is NULL");
+  HIPCHK(hipSetDevice(plan->device));
+  hipStream_t st = stream ? static_cast<hipStream_t>(stream) : plan->own;
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  HIPCHK(hipStreamIsCapturing(st, &cs));
+  const bool timed = cs == hipStreamCaptureStatusNone;
+  CHK(plan_enqueue(plan, st, timed));
+  if (timed) plan->timed = 1;
+  return SFBHIP_OK;
+}
+
+sfbhip_status sfbhip_batch_plan_capture(sfbhip_batch_plan *plan, void *stream) {
+  g_last_error.clear();
+  if (!plan) return fail(SFBHIP_INVALID_ARGUMENT, "plan is NULL");
+  HIPCHK(hipSetDevice(plan->device));
+  hipStream_t st = stream ? static_cast<hipStream_t>(stream) : plan->cap;
+  if (plan->exec) {
+    HIPCHK(hipGraphExecDestroy(plan->exec));
+    plan->exec = nullptr;
+  }
+  if (plan->graph) {
+    HIPCHK(hipGraphDestroy(plan->graph));
+    plan->graph = nullptr;
+  }
+  HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+  const sfbhip_status rc = plan_enqueue(plan, st, false);
+  hipGraph_t g = nullptr;
+  const hipError_t e = hipStreamEndCapture(st, &g);
+  if (rc != SFBHIP_OK) {
+    if (e == hipSuccess && g) (void)hipGraphDestroy(g);
+    return rc;
+  }
+  HIPCHK(e);
+  plan->graph = g;
+  HIPCHK(hipGraphInstantiate(&plan->exec, g, nullptr, nullptr, 0));
+  return SFBHIP_OK;
+}
+
+sfbhip_status sfbhip_batch_plan_launch(sfbhip_batch_plan *plan, void *stream) {
+  g_last_error.clear();
+  if (!plan) return fail(SFBHIP_INVALID_ARGUMENT, "plan is NULL");
+  if (!plan->exec) return fail(SFBHIP_INVALID_ARGUMENT, "plan has no captured graph (sfbhip_batch_plan_capture)");
+  HIPCHK(hipSetDevice(plan->device));
+  hipStream_t st = stream ? static_cast<hipStream_t>(stream) : plan->own;
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  HIPCHK(hipStreamIsCapturing(st, &cs));
+  const bool timed = cs == hipStreamCaptureStatusNone;
+  if (timed) HIPCHK(hipEventRecord(plan->ev[0], st));
+  HIPCHK(hipGraphLaunch(plan->exec, st));
+  if (timed) {
+    HIPCHK(hipEventRecord(plan->ev[4], st));
+    plan->timed = 2;
+  }
+  return SFBHIP_OK;
+}
+
+sfbhip_status sfbhip_batch_plan_sync(sfbhip_batch_plan *plan, void *stream, float *ms4) {
+  g_last_error.clear();
+  if (!plan) return fail(SFBHIP_INVALID_ARGUMENT, "plan is NULL");
+  HIPCHK(hipSetDevice(plan->device));
+  hipStream_t st = stream ? static_cast<hipStream_t>(stream) : plan->own;
+  HIPCHK(hipMemcpyAsync(plan->h_err, plan->derr, sizeof(int) * 4, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  if (ms4) {
+    for (int i = 0; i < 4; ++i) ms4[i] = 0.f;
+    if (plan->timed == 1)
+      for (int i = 0; i < 3; ++i) HIPCHK(hipEventElapsedTime(&ms4[i], plan->ev[i + 1], plan->ev[i + 2]));
+    if (plan->timed) HIPCHK(hipEventElapsedTime(&ms4[3], plan->ev[0], plan->ev[4]));
+  }
+  const int e = plan->h_err[0];
+  if (e & ERR_BADPOINT) return fail(SFBHIP_DOMAIN_ERROR, "point with r < 0 or a non-finite coordinate");
+  if (e & ERR_DOMAIN) return fail(SFBHIP_DOMAIN_ERROR, "point beyond its catalogue's radius bound");
+  return SFBHIP_OK;
+}
+
+sfbhip_status sfbhip_batch_autotune(sfbhip_handle *h, const sfbhip_catalog *cats, int32_t ncat,
+                                    const sfbhip_params *p, const double *r_bound, int32_t reps,
+                                    sfbhip_tile_table *out) {
+  g_last_error.clear();
+  if (!h) return fail(SFBHIP_INVALID_ARGUMENT, "handle is NULL");
+  if (!cats || ncat < 1 || ncat > 128) return fail(SFBHIP_INVALID_ARGUMENT, "ncat must be in [1, 128]");
+  if (!p) return fail(SFBHIP_INVALID_ARGUMENT, "params is NULL");
+  reps = std::max(1, std::min(reps, 100));
+  const sfbhip_tile_table start = h->tiles;
+  // time of a table: the fastest of `reps` eager runs of the contraction and reduction
+  auto measure = [&](const sfbhip_tile_table &t, float *ms) -> sfbhip_status {
+    h->tiles = t;
+    sfbhip_batch_plan *pl = nullptr;
+    sfbhip_status rc = sfbhip_batch_plan_create(h, cats, ncat, p, r_bound, &pl);
+    if (rc != SFBHIP_OK) return rc;
+    float best = 3.0e38f;
+    for (int r = 0; r <= reps && rc == SFBHIP_OK; ++r) {
+      float m4[4];
+      rc = sfbhip_batch_plan_enqueue(pl, nullptr);
+      if (rc == SFBHIP_OK) rc = sfbhip_batch_plan_sync(pl, nullptr, m4);
+      if (rc == SFBHIP_OK && r > 0) best = std::min(best, m4[1] + m4[2]);  // run 0 warms up
+    }
+    sfbhip_batch_plan_destroy(pl);
+    *ms = best;
+    return rc;
+  };
+  int L = 0;
+  for (int c = 0; c < ncat; ++c) L = std::max(L, cats[c].lmax);
+  const int lsplit = std::min(L, std::max(0, p->mixed_fp64_below));
+  sfbhip_tile_table best = start;
+  float best_ms = 0.f;
+  sfbhip_status rc = measure(best, &best_ms);
+  // bands from the top (the largest l carry most of the work)
+  for (int b = SFBHIP_TILE_BANDS - 1; b >= 0 && rc == SFBHIP_OK; --b) {
+    const int blo = b == 0 ? 0 : (16 << (b - 1)), bhi = 16 << b;
+    if (blo >= L || bhi <= lsplit) continue;
+    const sfbhip_tile_table base = best;
+    for (int ch : {1, 2, 4})
+      for (int ks : {1, 2, 4}) {
+        if (ch == base.chunks[b] && ks == base.point_split[b]) continue;
+        sfbhip_tile_table t = base;
+        t.chunks[b] = ch;
+        t.point_split[b] = ks;
+        float ms = 0.f;
+        rc = measure(t, &ms);
+        if (rc != SFBHIP_OK) break;
+        if (ms < best_ms) {
+          best = t;
+          best_ms = ms;
+        }
+      }
+  }
+  if (rc == SFBHIP_OK) {
+    const sfbhip_tile_table base = best;
+    for (int ipc : {16, 32, 64, 128, 256}) {
+      if (ipc == base.items_per_cu) continue;
+      sfbhip_tile_table t = base;
+      t.items_per_cu = ipc;
+      float ms = 0.f;
+      rc = measure(t, &ms);
+      if (rc != SFBHIP_OK) break;
+      if (ms < best_ms) {
+        best = t;
+        best_ms = ms;
+      }
+    }
+  }
+  if (rc != SFBHIP_OK) {
+    h->tiles = start;
+    return rc;
+  }
+  h->tiles = best;
+  if (out) *out = best;
   return SFBHIP_OK;
 }
 

@@ -46,10 +46,16 @@ struct DirectItem {
   int32_t qlo, qhi;  // m chunks [qlo, qhi) handled by this item
   int32_t cat;       // catalogue of a batched call (0 otherwise)
   int32_t tab;       // mixed path: Bessel-table intervals the item's arguments reach
-  int32_t pad;
+  int32_t flags;     // mixed path: ITEM_FULL_WINDOW -- points not sorted by radius, window from x = 0
   int64_t p0, p1;    // point (or pixel) range
   int64_t out;       // offset (in doubles) of this item's partial block
   int64_t kofs;      // offset of k_{l, n0} in the k_ln array of the call
+};
+
+// DirectItem::flags
+enum : int32_t {
+  ITEM_FULL_WINDOW = 1  // the item's points are in any order (radii in [0, R]): its Bessel
+                        // window covers nodes 0 .. tab + 1, not [k_first r_first, k_last r_last]
 };
 
 // One (catalogue, l, row-tile) group: its items are `nsplit` consecutive point

@@ -178,36 +178,54 @@ __device__ __forceinline__ void emit_y(const YlmDown &y, float *Yf, _Float16 *Yh
   }
 }
 
-// The Bessel intervals an item's arguments reach (it.tab) are copied into LDS
-// (dynamic shared memory of lds_nodes nodes, sized for two blocks per CU) when
-// they fit; otherwise the item reads the table from global memory (L1/L2).
-// The choice is per block, so one launch serves both kinds of items.
-template <int QMAX, int NP, int KS = 1>
-__global__ __launch_bounds__(256) void sfb_direct_mixed_kernel(
-    PointsSoA pts, const DirectItem *__restrict__ items, const double *__restrict__ kln, int nmax,
-    LegendreTables leg, HermiteTables ht, int lds_nodes, double *__restrict__ partial) {
+// Row pitch of the Y* chunk (elements): a column starts 4 banks after the
+// previous one, so the 16-byte B-fragment reads of 8 consecutive columns
+// (FP16) and the 4-byte reads of 16 columns x 4 points (FP32) hit distinct banks.
+template <int NP>
+struct MixedLds {
+  static constexpr int LDP = NP ? MX_TP + 8 : MX_TP + 4;
+  static constexpr int PLANE = 2 * MX_MC * LDP;  // elements of one piece of the chunk
+  static constexpr int YBYTES = NP ? NP * PLANE * 2 : PLANE * 4;
+};
+
+// ItemShared: the block's LDS areas -- the Y* chunk (ybuf, YBYTES), the tile's
+// radii (rs, MX_TP), the recurrence coefficients (cfs, MX_MC + MX_LMAX) and
+// the Bessel node window (TN, lds_nodes, dynamic).
+struct ItemShared {
+  char *ybuf;
+  double *rs;
+  double2 *cfs;
+  float2 *TN;
+};
+
+// One work item on a 256-thread block.  The Bessel intervals the item's
+// arguments reach (it.tab) are copied into LDS (the dynamic window of
+// lds_nodes nodes, sized for two blocks per CU) when they fit; otherwise the
+// item reads the table from global memory (L1/L2).  The choice is per item,
+// so one launch serves both kinds of items.  The caller separates two items
+// run by one block with a barrier.
+template <int QMAX, int NP, int KS>
+__device__ __forceinline__ void mixed_item(const DirectItem &it, const PointsSoA &pts, const double *__restrict__ kln,
+                                           const LegendreTables &leg, const HermiteTables &ht, int lds_nodes,
+                                           double *__restrict__ partial, const ItemShared &sh) {
   static_assert(NP == 0 || NP == 2 || NP == 3, "operand pieces");
   static_assert(KS == 1 || ((KS == 2 || KS == 4) && NP != 0), "point split: FP16 operands only");
   constexpr int RBN = 4 / KS;                // row blocks of 16 per item
   constexpr int SPW = MX_TP / 32 / KS;       // 32-point blocks of a tile per wave (FP16 operands)
-  // row pitch of the Y* chunk (elements): a column starts 4 banks after the
-  // previous one, so the 16-byte B-fragment reads of 8 consecutive columns
-  // (FP16) and the 4-byte reads of 16 columns x 4 points (FP32) hit distinct banks
-  constexpr int LDP = NP ? MX_TP + 8 : MX_TP + 4;
-  constexpr int PLANE = 2 * MX_MC * LDP;            // elements of one piece of the chunk
-  constexpr int YBYTES = NP ? NP * PLANE * 2 : PLANE * 4;
+  constexpr int LDP = MixedLds<NP>::LDP;
+  constexpr int PLANE = MixedLds<NP>::PLANE;
+  constexpr int YBYTES = MixedLds<NP>::YBYTES;
   constexpr int FOLD = 64;                          // points per FP32 partial sum
   constexpr int NA = NP ? NP : 1;
-  __shared__ __attribute__((aligned(16))) char ybuf[YBYTES];
-  __shared__ double rs[MX_TP];
+  char *ybuf = sh.ybuf;
+  double *rs = sh.rs;
   // recurrence coefficients (a_m, b_m) of this l at cf[MX_MC + m]; zero for m <= 0, so that
   // stepping past m = 0 leaves P = 0 and the lowest chunk needs no per-step guards
-  __shared__ double2 cfs[MX_MC + MX_LMAX];
-  double2 *cf = cfs + MX_MC;
+  double2 *cf = sh.cfs + MX_MC;
+  float2 *TN = sh.TN;  // window of the nodes this item's arguments reach
   float *Yf = reinterpret_cast<float *>(ybuf);
   _Float16 *Yh = reinterpret_cast<_Float16 *>(ybuf);
 
-  const DirectItem it = items[blockIdx.x];
   const int l = it.l;
   const int ncol = 2 * (l + 1);
   const int tid = threadIdx.x;
@@ -221,7 +239,6 @@ __global__ __launch_bounds__(256) void sfb_direct_mixed_kernel(
 
   const int nint = ht.nint[l];
   const float4 *__restrict__ T = ht.tab + ht.off[l];
-  extern __shared__ float2 TN[];  // window of the nodes this item's arguments reach
   // The item's points are sorted by radius and its rows by k, so every
   // argument k r / h lies in [k_first r_first, k_last r_last] / h: nodes lo ..
   // hi (two past the largest argument's interval) are staged when they fit.
@@ -229,10 +246,13 @@ __global__ __launch_bounds__(256) void sfb_direct_mixed_kernel(
   const double khr1 = kln[it.kofs + it.nrows - 1] * (1.0 / MX_H);
   const double rlo = pts.r[it.p0];
   // one node of margin below: a batched catalogue is sorted by a key that drops
-  // the lowest 7 bits of r, so the first radius may exceed the smallest by an ulp
-  const int lo = max((int)(rlo * khr0) - 1, 0);
+  // the lowest 7 bits of r, so the first radius may exceed the smallest by an ulp.
+  // Points unsorted (ITEM_FULL_WINDOW): every radius lies in [0, R] and the
+  // window covers nodes 0 .. it.tab + 1 (it.tab = floor(k_last R / h) + 2)
+  const bool full = (it.flags & ITEM_FULL_WINDOW) != 0;
+  const int lo = full ? 0 : max((int)(rlo * khr0) - 1, 0);
   const double lod = (double)lo;
-  const int nwin = (int)(pts.r[it.p1 - 1] * khr1) + 3 - lo;
+  const int nwin = full ? it.tab + 2 : (int)(pts.r[it.p1 - 1] * khr1) + 3 - lo;
   const bool tl = nwin <= lds_nodes;  // block-uniform
   if (tl) {
     // FP16 operands carry v * 2^12: the staged nodes carry the (exact) scale;
@@ -495,6 +515,64 @@ __global__ __launch_bounds__(256) void sfb_direct_mixed_kernel(
           if (col < ncol) out[(size_t)orow * ncol + col] = sum[qq][c][j] * scale;
         }
       }
+    }
+  }
+}
+
+// One item per block (blockIdx.x), all of one class (QMAX, NP, KS).
+template <int QMAX, int NP, int KS = 1>
+__global__ __launch_bounds__(256) void sfb_direct_mixed_kernel(
+    PointsSoA pts, const DirectItem *__restrict__ items, const double *__restrict__ kln, int nmax,
+    LegendreTables leg, HermiteTables ht, int lds_nodes, double *__restrict__ partial) {
+  __shared__ __attribute__((aligned(16))) char ybuf[MixedLds<NP>::YBYTES];
+  __shared__ double rs[MX_TP];
+  __shared__ double2 cfs[MX_MC + MX_LMAX];
+  extern __shared__ float2 TN[];
+  const DirectItem it = items[blockIdx.x];
+  mixed_item<QMAX, NP, KS>(it, pts, kln, leg, ht, lds_nodes, partial, ItemShared{ybuf, rs, cfs, TN});
+}
+
+// DirectItem::flags of the grouped kernel's items: the point split KS
+enum : int32_t { ITEM_KS2 = 2, ITEM_KS4 = 4 };
+
+// Persistent grouped launch of the split-FP16 contraction: every block pulls
+// items from *counter (zeroed before the launch) until the list is exhausted,
+// and runs each with the class its m-chunk count (1, 2 or up to 4) and point
+// split (flags ITEM_KS2 / ITEM_KS4) select, so that items of every class and
+// catalogue share one launch with no tail per class.  Items are listed by
+// decreasing cost.  Every block leaves the loop once the counter passes
+// nitems.
+__global__ __launch_bounds__(256, 2) void sfb_direct_mixed_grouped_kernel(
+    PointsSoA pts, const DirectItem *__restrict__ items, int nitems, int *__restrict__ counter,
+    const double *__restrict__ kln, LegendreTables leg, HermiteTables ht, int lds_nodes,
+    double *__restrict__ partial) {
+  __shared__ __attribute__((aligned(16))) char ybuf[MixedLds<2>::YBYTES];
+  __shared__ double rs[MX_TP];
+  __shared__ double2 cfs[MX_MC + MX_LMAX];
+  __shared__ int next;
+  extern __shared__ float2 TN[];
+  const ItemShared sh{ybuf, rs, cfs, TN};
+  for (;;) {
+    if (threadIdx.x == 0) next = atomicAdd(counter, 1);
+    __syncthreads();  // next written; the previous item's LDS readers are done
+    const int i = __builtin_amdgcn_readfirstlane(next);
+    __syncthreads();  // every thread has read next before thread 0 overwrites it
+    if (i >= nitems) break;
+    const DirectItem it = items[i];
+    const int nq = it.qhi - it.qlo;
+    const int ks = it.flags & (ITEM_KS2 | ITEM_KS4);
+    if (ks == ITEM_KS4) {
+      if (nq == 1) mixed_item<1, 2, 4>(it, pts, kln, leg, ht, lds_nodes, partial, sh);
+      else if (nq == 2) mixed_item<2, 2, 4>(it, pts, kln, leg, ht, lds_nodes, partial, sh);
+      else mixed_item<4, 2, 4>(it, pts, kln, leg, ht, lds_nodes, partial, sh);
+    } else if (ks == ITEM_KS2) {
+      if (nq == 1) mixed_item<1, 2, 2>(it, pts, kln, leg, ht, lds_nodes, partial, sh);
+      else if (nq == 2) mixed_item<2, 2, 2>(it, pts, kln, leg, ht, lds_nodes, partial, sh);
+      else mixed_item<4, 2, 2>(it, pts, kln, leg, ht, lds_nodes, partial, sh);
+    } else {
+      if (nq == 1) mixed_item<1, 2, 1>(it, pts, kln, leg, ht, lds_nodes, partial, sh);
+      else if (nq == 2) mixed_item<2, 2, 1>(it, pts, kln, leg, ht, lds_nodes, partial, sh);
+      else mixed_item<4, 2, 1>(it, pts, kln, leg, ht, lds_nodes, partial, sh);
     }
   }
 }

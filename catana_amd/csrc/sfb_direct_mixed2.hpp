@@ -284,9 +284,12 @@ __global__ __launch_bounds__(M2_NT) void sfb_direct_mixed2_kernel(
   const double khr0 = kln[it.kofs] * (1.0 / MX_H);
   const double khr1 = kln[it.kofs + it.nrows - 1] * (1.0 / MX_H);
   const double rlo = pts.r[it.p0];
-  const int lo = max((int)(rlo * khr0) - 1, 0);
+  // points unsorted (ITEM_FULL_WINDOW): every radius lies in [0, R] and the
+  // window covers nodes 0 .. it.tab + 1 (it.tab = floor(k_last R / h) + 2)
+  const bool full = (it.flags & ITEM_FULL_WINDOW) != 0;
+  const int lo = full ? 0 : max((int)(rlo * khr0) - 1, 0);
   const double lod = (double)lo;
-  const int nwin = (int)(pts.r[it.p1 - 1] * khr1) + 3 - lo;
+  const int nwin = full ? it.tab + 2 : (int)(pts.r[it.p1 - 1] * khr1) + 3 - lo;
   const bool tl = nwin <= lds_nodes;  // block-uniform
   if (tl) {
     for (int j = tid; j < nwin; j += M2_NT) {

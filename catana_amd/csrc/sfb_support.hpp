@@ -147,6 +147,83 @@ __global__ void cln_kernel(int lmax, int nmax, const int32_t *__restrict__ nl, c
   cln[idx] = c;
 }
 
+// One catalogue of a batch plan: its AoS points, their count, where its rows
+// start in the concatenated point table, and the radius its Bessel tables cover.
+struct CatPrep {
+  const double *aos;
+  int64_t n, base;
+  double R;
+};
+
+// Point table of a batch plan (blockIdx.y = catalogue), in input order.  No
+// point leaves the tables the plan sized: r > R is clamped to R and flagged
+// ERR_DOMAIN; r < 0 or a non-finite coordinate becomes (0, 0, 0), flagged
+// ERR_BADPOINT.  The caller reads the flags after the launch.
+__global__ void prep_points_batch_kernel(const CatPrep *__restrict__ cats, double *__restrict__ soa, int64_t ntot,
+                                         int *__restrict__ err) {
+  const CatPrep c = cats[blockIdx.y];
+  int bits = 0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < c.n; i += (int64_t)gridDim.x * blockDim.x) {
+    double rr = c.aos[3 * i], th = c.aos[3 * i + 1], ph = c.aos[3 * i + 2];
+    if (!(rr >= 0.0) || !isfinite(rr) || !isfinite(th) || !isfinite(ph)) {
+      bits |= ERR_BADPOINT;
+      rr = th = ph = 0.0;
+    } else if (rr > c.R) {
+      bits |= ERR_DOMAIN;
+      rr = c.R;
+    }
+    double st, ctt, sp, cp;
+    sincos(th, &st, &ctt);
+    sincos(ph, &sp, &cp);
+    const int64_t j = c.base + i;
+    soa[j] = rr;
+    soa[ntot + j] = ctt;
+    soa[2 * ntot + j] = fabs(st);
+    soa[3 * ntot + j] = ph;
+    soa[4 * ntot + j] = cp;
+    soa[5 * ntot + j] = sp;
+  }
+  if (bits) atomicOr(err, bits);
+}
+
+// Outputs of one catalogue of a batch plan: its k_ln, c_ln and (optional)
+// f_lmn buffers, its shape and where its rows sit in the plan's arrays.
+struct CatOut {
+  double *k_ln, *c_ln, *f_lmn;
+  int32_t lmax, nmax;
+  int64_t kbase, nlbase, foffbase, fbase;
+};
+
+// blockIdx.y = catalogue: c_ln as cln_kernel computes it, k_ln, and a copy of
+// f_lmn out of the plan's f workspace.
+__global__ void batch_outputs_kernel(const CatOut *__restrict__ outs, const double *__restrict__ f,
+                                     const double *__restrict__ kln, const int32_t *__restrict__ nl,
+                                     const int64_t *__restrict__ foff) {
+  const CatOut o = outs[blockIdx.y];
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  const int64_t t0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const double *fc = f + 2 * o.fbase;
+  for (int64_t idx = t0; idx < (int64_t)o.lmax * o.nmax; idx += stride) {
+    const int l = (int)(idx / o.nmax), n = (int)(idx - (int64_t)l * o.nmax);
+    double c = 0.0;
+    if (n < nl[o.nlbase + l]) {
+      const double *fl = fc + foff[o.foffbase + l] * 2;
+      for (int m = -l; m <= l; ++m) {
+        const int am = m < 0 ? -m : m;
+        const double re = fl[((size_t)am * o.nmax + n) * 2], im = fl[((size_t)am * o.nmax + n) * 2 + 1];
+        c += re * re + im * im;
+      }
+      c /= (2 * l + 1);
+    }
+    o.c_ln[idx] = c;
+    o.k_ln[idx] = kln[o.kbase + idx];
+  }
+  if (o.f_lmn) {
+    const int64_t nf = (int64_t)o.lmax * (o.lmax + 1) / 2 * o.nmax * 2;
+    for (int64_t idx = t0; idx < nf; idx += stride) o.f_lmn[idx] = fc[idx];
+  }
+}
+
 __global__ void fill_kernel(double *__restrict__ p, int64_t n, double v) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = v;

@@ -139,6 +139,75 @@ typedef struct sfbhip_catalog {
 sfbhip_status sfbhip_direct_batched(sfbhip_handle *h, const sfbhip_catalog *cats, int32_t ncat,
                                     const sfbhip_params *p, void *stream);
 
+/* ---- batch plans: sfbhip_direct_batched as a plan made once and replayed ----
+ * A plan binds the catalogues' device buffers (points, k_ln, c_ln, f_lmn) and
+ * shapes, and does all host work once: Bessel zeros, k_ln, tables, the work
+ * items of every catalogue.  Enqueueing it launches only the library's own
+ * kernels on one stream -- no host synchronisation, no allocation, no
+ * library call -- so the enqueue can be captured in a hipGraph and the graph
+ * replayed whenever the points in the bound buffers change.  The contraction
+ * is one persistent grouped launch: every block pulls work items of any
+ * catalogue, multipole and shape from a device counter, in order of
+ * decreasing cost.  Points stay in input order (no radius sort); each
+ * catalogue's Bessel tables cover r <= r_bound (default: its rmax).
+ * Precision MIXED with SPLIT_FP16 operands only (the FP64 and other operand
+ * formats run through sfbhip_direct_batched).  No counterpart in the
+ * reference, where a caller loops over catana::sfb_decomposition
+ * (sfb_decomposition.cpp:15-19); each catalogue's result equals that of
+ * sfbhip_direct on it alone, up to summation order. */
+typedef struct sfbhip_batch_plan sfbhip_batch_plan;
+
+/* Bands of l the tile table distinguishes: [0,16) [16,32) [32,64) [64,128) [128,256) [256,512). */
+#define SFBHIP_TILE_BANDS 6
+typedef struct sfbhip_tile_table {
+  int32_t chunks[SFBHIP_TILE_BANDS];       /* 16-wide m chunks one work item accumulates: 1, 2 or 4 */
+  int32_t point_split[SFBHIP_TILE_BANDS];  /* largest point split: 1, 2 or 4 (an item of <= 16 rows splits
+                                              each point tile over 4 waves, of <= 32 rows over 2) */
+  int32_t items_per_cu;                    /* work items planned per compute unit (1 .. 4096) */
+} sfbhip_tile_table;
+
+/* The tile table later plans of this handle use.  NULL restores the defaults
+ * (4 chunks, point split 4 in every band, 64 items per compute unit). */
+sfbhip_status sfbhip_set_tile_table(sfbhip_handle *h, const sfbhip_tile_table *t);
+sfbhip_status sfbhip_get_tile_table(sfbhip_handle *h, sfbhip_tile_table *t);
+
+/* Plans the batch (cats as for sfbhip_direct_batched; up to 128 catalogues).
+ * r_bound: NULL, or a host array of ncat radii that bound each catalogue's
+ * points (values below rmax count as rmax).  A point beyond its bound is
+ * clamped to it and reported as a domain error by sfbhip_batch_plan_sync;
+ * a point with r < 0 or a non-finite coordinate likewise.  Synchronous. */
+sfbhip_status sfbhip_batch_plan_create(sfbhip_handle *h, const sfbhip_catalog *cats, int32_t ncat,
+                                       const sfbhip_params *p, const double *r_bound, sfbhip_batch_plan **out);
+sfbhip_status sfbhip_batch_plan_destroy(sfbhip_batch_plan *plan);
+
+/* Enqueues the plan on stream (NULL: the plan's own stream) and returns
+ * without waiting.  Capturable: called inside hipStreamBeginCapture /
+ * hipStreamEndCapture it records the whole batched decomposition. */
+sfbhip_status sfbhip_batch_plan_enqueue(sfbhip_batch_plan *plan, void *stream);
+
+/* Captures one enqueue into a hipGraph owned by the plan (replacing an
+ * earlier one) and instantiates it; sfbhip_batch_plan_launch replays it. */
+sfbhip_status sfbhip_batch_plan_capture(sfbhip_batch_plan *plan, void *stream);
+sfbhip_status sfbhip_batch_plan_launch(sfbhip_batch_plan *plan, void *stream);
+
+/* Waits for the plan's work on stream and reports the device's error flags of
+ * the last enqueue or launch: SFBHIP_DOMAIN_ERROR for a point beyond its
+ * bound, r < 0 or a non-finite coordinate; SFBHIP_OK otherwise.  Also times
+ * the last run: ms4[0] FP64 low-l kernels, [1] the grouped mixed contraction,
+ * [2] reduction and outputs (these three after an eager enqueue only), [3] the
+ * whole enqueue or graph launch (HIP events on the stream; ms4 may be NULL). */
+sfbhip_status sfbhip_batch_plan_sync(sfbhip_batch_plan *plan, void *stream, float *ms4);
+
+/* Autotuner: for the bands of l this batch reaches, times candidate
+ * (chunks, point split) pairs band by band, then items per compute unit,
+ * each on the whole batch (plans of this handle, `reps` timed runs of the
+ * grouped contraction per candidate, the fastest run kept), installs the best
+ * table in the handle and copies it to *out (may be NULL).  Writes the
+ * catalogues' outputs as a side effect. */
+sfbhip_status sfbhip_batch_autotune(sfbhip_handle *h, const sfbhip_catalog *cats, int32_t ncat,
+                                    const sfbhip_params *p, const double *r_bound, int32_t reps,
+                                    sfbhip_tile_table *out);
+
 /* HEALPix RING binning of device points into CSR order by pixel: d_offsets
  * (12 nside^2 + 1 int64), d_radii (n_points doubles), radii of a pixel kept in
  * input order.  Replaces PixelizedPointContainer(nside, pc)
