@@ -156,6 +156,10 @@ def main():
     ap.add_argument("--fp64-below", type=int, default=2, help="mixed path: l below this run in FP64")
     ap.add_argument("--kernel", default="auto", choices=["auto", "256", "512"],
                     help="mixed path, split-FP16 operands: the contraction kernel")
+    ap.add_argument("--batch-mode", default="graph", choices=["graph", "plan", "call"],
+                    help="batched64: replay a captured batch plan (graph), enqueue the plan (plan), or one "
+                         "sfbhip_direct_batched call per step (call)")
+    ap.add_argument("--autotune", action="store_true", help="batched64: run the tile-table autotuner first")
     ap.add_argument("--cpu-sample", type=int, default=0,
                     help="points of the CPU baseline's seeded subsample (0: the workload's default, "
                          "a few seconds per run on 16 threads, three runs)")
@@ -551,7 +555,9 @@ def run_ingest(args, cfg):
 
 
 def run_batched(args):
-    """The 64-catalogue workload: one sfbhip_direct_batched call per step."""
+    """The 64-catalogue workload.  A step is one replay of the captured batch
+    plan (--batch-mode graph, the default), one eager enqueue of the plan
+    (plan), or one sfbhip_direct_batched call (call)."""
     import torch
     from catana_amd import capi
     log("generating 64 catalogues")
@@ -573,22 +579,74 @@ def run_batched(args):
         flops_dom += 4.0 * c["np"].shape[0] * int(((l[lo:] + 1) * nl[lo:]).sum())
     ntot = sum(c["np"].shape[0] for c in cats)
     log(f"{ntot} points in {len(cats)} catalogues")
-    for _ in range(args.warmup):
-        capi.sfb_direct_batched_torch(h, cats, **kw)
-    torch.cuda.synchronize()
-    stream = torch.cuda.current_stream(dev)
+    mode = args.batch_mode
+    if mode != "call" and (prec != capi.PREC_MIXED or args.operands != "split16" or args.kernel == "512"):
+        log("batch plans run the mixed split-FP16 256-thread path: falling back to --batch-mode call")
+        mode = "call"
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    kern, phases = [], []
-    e0.record(stream)
-    for _ in range(args.steps):
-        res = capi.sfb_direct_batched_torch(h, cats, **kw)
-        kern.append(h.last_kernel_ms())
-        phases.append(h.last_phase_ms())
-    e1.record(stream)
-    torch.cuda.synchronize()
-    ms = e0.elapsed_time(e1) / args.steps
-    kms = float(np.mean(kern))
-    ph = np.mean(np.array(phases), axis=0)
+    extra = {}
+    if mode == "call":
+        for _ in range(args.warmup):
+            capi.sfb_direct_batched_torch(h, cats, **kw)
+        torch.cuda.synchronize()
+        stream = torch.cuda.current_stream(dev)
+        kern, phases = [], []
+        e0.record(stream)
+        for _ in range(args.steps):
+            res = capi.sfb_direct_batched_torch(h, cats, **kw)
+            kern.append(h.last_kernel_ms())
+            phases.append(h.last_phase_ms())
+        e1.record(stream)
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / args.steps
+        kms = float(np.mean(kern))
+        ph = np.mean(np.array(phases), axis=0)
+        kname = "sfb_direct_%s_kernel" % ("f64" if prec == capi.PREC_FP64 else "mixed")
+    else:
+        pkw = dict(mixed_fp64_below=args.fp64_below, zeros=kw.get("zeros", capi.ZEROS_REFERENCE))
+        if args.autotune:
+            log("autotuning the tile table")
+            t0 = time.perf_counter()
+            extra["tile_table"] = capi.batch_autotune(h, cats, reps=2, **pkw)
+            extra["autotune_s"] = time.perf_counter() - t0
+        else:
+            extra["tile_table"] = capi.get_tile_table(h)
+        s = torch.cuda.Stream(device=dev)
+        sp = s.cuda_stream
+        t0 = time.perf_counter()
+        plan = capi.BatchPlan(h, cats, **pkw)
+        extra["plan_create_ms"] = (time.perf_counter() - t0) * 1e3
+        run = plan.enqueue
+        if mode == "graph":
+            plan.capture(sp)
+            run = plan.launch
+        torch.cuda.synchronize()
+        for _ in range(args.warmup):
+            run(sp)
+        plan.sync(sp)
+        e0.record(s)
+        for _ in range(args.steps):
+            run(sp)
+        e1.record(s)
+        plan.sync(sp)
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / args.steps
+        # the grouped kernel's duration from HIP events of eager enqueues of the same plan
+        phs = []
+        for _ in range(max(3, min(args.steps, 10))):
+            plan.enqueue(sp)
+            phs.append(plan.sync(sp))
+        ph = np.mean(np.array(phs), axis=0)
+        kms = float(ph[1])
+        res = plan.outputs
+        kname = "sfb_direct_mixed_grouped_kernel"
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        capi.sfb_direct_batched_torch(h, cats, **kw)
+        torch.cuda.synchronize()
+        extra["direct_batched_call_ms"] = (time.perf_counter() - t0) * 1e3
+        extra["phases_ms_eager_plan"] = {"fp64_low_l": float(ph[0]), "grouped_contraction": float(ph[1]),
+                                         "reduce_and_outputs": float(ph[2]), "whole_enqueue": float(ph[3])}
     # the same catalogues one call each (what a caller looping over sfb_decomposition does)
     t0 = time.perf_counter()
     for c in cats:
@@ -611,14 +669,14 @@ def run_batched(args):
         "dtype": dtype, "data": "synthetic, seeded (bench.make_batch), inputs resident in HBM",
         "config": {"workload": "batched64", "catalogues": len(cats), "N_total": int(ntot),
                    "lmax_range": [min(c["lmax"] for c in cats), max(c["lmax"] for c in cats)], "k_max": 0.15,
-                   "precision": args.precision, "operands": args.operands, "chunks": args.chunks,
-                   "pairs_per_step": pairs, "sequential_single_calls_ms": seq_ms},
+                   "precision": args.precision, "operands": args.operands, "batch_mode": mode,
+                   "pairs_per_step": pairs, "sequential_single_calls_ms": seq_ms, **extra},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s", "frac": achieved / peak,
-                     "traffic": None, "kernel": "sfb_direct_%s_kernel" % ("f64" if prec == capi.PREC_FP64 else "mixed"),
-                     "kernel_ms": kms, "algorithmic_flops_per_launch": flops_dom,
-                     "phases_ms": {"fp64_contraction": float(ph[0]), "mixed_contraction": float(ph[1]),
-                                   "reduce_and_cln": float(ph[2])}},
+                     "traffic": None, "kernel": kname, "kernel_ms": kms, "algorithmic_flops_per_launch": flops_dom},
     }
+    if mode == "call":
+        out["roofline"]["phases_ms"] = {"fp64_contraction": float(ph[0]), "mixed_contraction": float(ph[1]),
+                                        "reduce_and_cln": float(ph[2])}
     if prec != capi.PREC_FP64 and not args.no_accuracy:
         log("FP64 batched run for the accuracy figures")
         torch.cuda.synchronize()
@@ -644,6 +702,8 @@ def run_batched(args):
         out["cpu_baseline"] = cpu_baseline(c["np"], dict(lmax=c["lmax"], nmax=c["nmax"], rmax=c["rmax"]), nl,
                                            args.cpu_sample or CPU_SAMPLE["batched64"], threads)
         out["cpu_baseline"]["sample"] += " (largest catalogue of the batch)"
+    if mode != "call":
+        plan.close()
     h.close()
     print(json.dumps(out))
 

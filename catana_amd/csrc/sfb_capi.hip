@@ -29,6 +29,7 @@
 #include "sfb_integrals.hpp"
 #include "sfb_ingest.hpp"
 #include "sfb_math.hpp"
+#include "sfb_sort.hpp"
 #include "sfb_support.hpp"
 
 using namespace sfb;
@@ -2416,8 +2417,14 @@ struct sfbhip_batch_plan {
   int timed = 0;                   // last run: 0 none, 1 eager enqueue (phases), 2 graph launch (total)
   // device data
   int *derr = nullptr;             // [0] error bits, [4] work counter of the grouped launch
-  double *soa = nullptr;
+  double *soa = nullptr;           // point table in radius order (what the kernels read)
+  double *soa_in = nullptr;        // point table in input order
   int64_t ntot = 0;
+  uint16_t *keys[2] = {nullptr, nullptr};  // radius buckets, input order / sort ping-pong
+  int32_t *idx[2] = {nullptr, nullptr};
+  uint32_t *hist = nullptr;
+  SortSeg *dseg = nullptr;
+  int sort_nb = 1;                 // blocks of the largest catalogue's sort
   CatPrep *dprep = nullptr;
   int prep_bx = 1;
   double *dk = nullptr;
@@ -2466,8 +2473,11 @@ sfbhip_status plan_upload(sfbhip_batch_plan *pl, const std::vector<T> &v, T **ou
 // Items of the grouped split-FP16 kernel for one catalogue: per (l, 64-row
 // tile) group, S point ranges of one length, each cut into passes of the
 // band's chunk count; an item of <= 16 (32) rows splits its tiles' points over
-// 4 (2) waves where the band allows.  Points are in input order, so every
-// item's Bessel window starts at x = 0 (ITEM_FULL_WINDOW).
+// 4 (2) waves where the band allows.  The points are ordered by 16-bit radius
+// bucket (R / 2^16 wide), so an item's radii lie within one bucket of
+// [r_first, r_last]: its Bessel window, which has a node of margin on either
+// side, covers them while a bucket spans less than a node (table reach below
+// 2^15 intervals); past that the window starts at x = 0 (ITEM_FULL_WINDOW).
 void plan_grouped(const std::vector<int32_t> &nl, int lmin, int lmax, int64_t npts, const sfbhip_tile_table &tt,
                   int64_t target_items, const PlanCtx &cx, std::vector<DirectItem> *items,
                   std::vector<ReduceGroup> *groups, int64_t *partial_size) {
@@ -2516,7 +2526,7 @@ void plan_grouped(const std::vector<int32_t> &nl, int lmin, int lmax, int64_t np
           it.qhi = std::min(Q, qlo + qcap);
           it.cat = cx.cat;
           it.tab = tab;
-          it.flags = ITEM_FULL_WINDOW | ks;
+          it.flags = (tab < (1 << 15) ? 0 : ITEM_FULL_WINDOW) | ks;
           it.p0 = cx.pbase + s * chunk;
           it.p1 = cx.pbase + std::min(npts, (s + 1) * chunk);
           it.out = g.base + s * g.stride;
@@ -2529,7 +2539,9 @@ void plan_grouped(const std::vector<int32_t> &nl, int lmin, int lmax, int64_t np
 }
 
 // Relative cost of a grouped item, for ordering the list longest first: the
-// Y* rows of its chunks and the Legendre steps down to its first chunk, per point.
+// Y* rows of its chunks and the Legendre steps down to its first chunk, per
+// point.  (Ordering by kernel variant first, so that the blocks of a compute
+// unit share one variant's code, measured 1 % slower on 64 catalogues.)
 double item_cost(const DirectItem &it) {
   return (double)(it.p1 - it.p0) * ((double)(it.qhi - it.qlo) + 0.1 * it.qlo + 0.3);
 }
@@ -2538,8 +2550,21 @@ sfbhip_status plan_enqueue(sfbhip_batch_plan *pl, hipStream_t st, bool timed) {
   if (timed) HIPCHK(hipEventRecord(pl->ev[0], st));
   fill_int_kernel<<<1, 64, 0, st>>>(pl->derr, 8, 0);
   HIPCHK(hipGetLastError());
-  prep_points_batch_kernel<<<dim3((unsigned)pl->prep_bx, (unsigned)pl->ncat), 256, 0, st>>>(pl->dprep, pl->soa, pl->ntot,
-                                                                                           pl->derr);
+  const dim3 gcat((unsigned)pl->prep_bx, (unsigned)pl->ncat);
+  prep_points_batch_kernel<<<gcat, 256, 0, st>>>(pl->dprep, pl->soa_in, pl->ntot, pl->keys[0], pl->derr);
+  HIPCHK(hipGetLastError());
+  // radius order within each catalogue: two 8-bit passes (keys 0 -> 1 -> 0), then the gather
+  const dim3 gsort((unsigned)pl->sort_nb, (unsigned)pl->ncat);
+  for (int pass = 0; pass < 2; ++pass) {
+    sort_hist_kernel<<<gsort, 256, 0, st>>>(pl->dseg, pl->keys[pass], 8 * pass, pl->hist);
+    HIPCHK(hipGetLastError());
+    sort_scan_kernel<<<(unsigned)pl->ncat, 1024, 0, st>>>(pl->dseg, pl->hist);
+    HIPCHK(hipGetLastError());
+    sort_scatter_kernel<<<gsort, 256, 0, st>>>(pl->dseg, pl->keys[pass], pass ? pl->idx[0] : nullptr, 8 * pass,
+                                               pl->hist, pl->keys[1 - pass], pl->idx[pass]);
+    HIPCHK(hipGetLastError());
+  }
+  sort_gather_kernel<<<gcat, 256, 0, st>>>(pl->dseg, pl->idx[1], pl->soa_in, pl->soa, pl->ntot);
   HIPCHK(hipGetLastError());
   const int64_t n = pl->ntot;
   const PointsSoA pts{pl->soa, pl->soa + n, pl->soa + 2 * n, pl->soa + 3 * n, pl->soa + 4 * n, pl->soa + 5 * n, n};
@@ -2803,7 +2828,26 @@ sfbhip_status plan_create(sfbhip_handle *h, const sfbhip_catalog *cats, int32_t 
   CHK(plan_alloc(pl, (size_t)std::max<int64_t>(psize, 1), &pl->partial));
   CHK(plan_alloc(pl, (size_t)std::max<int64_t>(2 * ftot, 1), &pl->f));
   CHK(plan_alloc(pl, (size_t)6 * ntot, &pl->soa));
+  CHK(plan_alloc(pl, (size_t)6 * ntot, &pl->soa_in));
+  for (int k = 0; k < 2; ++k) {
+    CHK(plan_alloc(pl, (size_t)ntot, &pl->keys[k]));
+    CHK(plan_alloc(pl, (size_t)ntot, &pl->idx[k]));
+  }
   CHK(plan_alloc(pl, 8, &pl->derr));
+  {
+    std::vector<SortSeg> seg(ncat);
+    int64_t hb = 0;
+    int nbmax = 1;
+    for (int c = 0; c < ncat; ++c) {
+      const int nb = (int)cdiv(cats[c].n_points, SORT_CHUNK);
+      seg[c] = SortSeg{pbase[c], cats[c].n_points, hb, nb, 0};
+      hb += 256LL * nb;
+      nbmax = std::max(nbmax, nb);
+    }
+    CHK(plan_upload(pl, seg, &pl->dseg));
+    CHK(plan_alloc(pl, (size_t)hb, &pl->hist));
+    pl->sort_nb = nbmax;
+  }
   int nodes = 0;
   for (const DirectItem &it : mx) nodes = std::max(nodes, std::min(it.tab + 2, MX_LDS_NODES));
   pl->lds_nodes = nodes;

@@ -539,9 +539,8 @@ enum : int32_t { ITEM_KS2 = 2, ITEM_KS4 = 4 };
 // items from *counter (zeroed before the launch) until the list is exhausted,
 // and runs each with the class its m-chunk count (1, 2 or up to 4) and point
 // split (flags ITEM_KS2 / ITEM_KS4) select, so that items of every class and
-// catalogue share one launch with no tail per class.  Items are listed by
-// decreasing cost.  Every block leaves the loop once the counter passes
-// nitems.
+// catalogue share one launch with no tail per class.  Every block leaves the
+// loop once the counter passes nitems.
 __global__ __launch_bounds__(256, 2) void sfb_direct_mixed_grouped_kernel(
     PointsSoA pts, const DirectItem *__restrict__ items, int nitems, int *__restrict__ counter,
     const double *__restrict__ kln, LegendreTables leg, HermiteTables ht, int lds_nodes,

@@ -155,12 +155,13 @@ struct CatPrep {
   double R;
 };
 
-// Point table of a batch plan (blockIdx.y = catalogue), in input order.  No
-// point leaves the tables the plan sized: r > R is clamped to R and flagged
-// ERR_DOMAIN; r < 0 or a non-finite coordinate becomes (0, 0, 0), flagged
-// ERR_BADPOINT.  The caller reads the flags after the launch.
+// Point table of a batch plan (blockIdx.y = catalogue), in input order, and
+// each point's radius bucket floor(r 2^16 / R) (keys).  No point leaves the
+// tables the plan sized: r > R is clamped to R and flagged ERR_DOMAIN; r < 0
+// or a non-finite coordinate becomes (0, 0, 0), flagged ERR_BADPOINT.  The
+// caller reads the flags after the launch.
 __global__ void prep_points_batch_kernel(const CatPrep *__restrict__ cats, double *__restrict__ soa, int64_t ntot,
-                                         int *__restrict__ err) {
+                                         uint16_t *__restrict__ keys, int *__restrict__ err) {
   const CatPrep c = cats[blockIdx.y];
   int bits = 0;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < c.n; i += (int64_t)gridDim.x * blockDim.x) {
@@ -176,6 +177,7 @@ __global__ void prep_points_batch_kernel(const CatPrep *__restrict__ cats, doubl
     sincos(th, &st, &ctt);
     sincos(ph, &sp, &cp);
     const int64_t j = c.base + i;
+    keys[j] = (uint16_t)min((int)(rr * (65536.0 / c.R)), 65535);
     soa[j] = rr;
     soa[ntot + j] = ctt;
     soa[2 * ntot + j] = fabs(st);
