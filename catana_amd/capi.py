@@ -25,7 +25,7 @@ EXPORTED = [
     "sfbhip_sbessel_integrals", "sfbhip_filter_points", "sfbhip_random_subset",
     "sfbhip_set_tile_table", "sfbhip_get_tile_table", "sfbhip_batch_plan_create", "sfbhip_batch_plan_destroy",
     "sfbhip_batch_plan_enqueue", "sfbhip_batch_plan_capture", "sfbhip_batch_plan_launch", "sfbhip_batch_plan_sync",
-    "sfbhip_batch_autotune",
+    "sfbhip_batch_autotune", "sfbhip_direct_noalloc",
 ]
 TILE_BANDS = 6  # bands of l: [0,16) [16,32) [32,64) [64,128) [128,256) [256,512)
 RECORD_POINT, RECORD_CARTESIAN_F32, RECORD_CARTESIAN_F64, RECORD_SPHERICAL_F32, RECORD_SPHERICAL_F64 = range(5)
@@ -110,6 +110,7 @@ def lib():
         L.sfbhip_kln.argtypes = [P, ctypes.POINTER(Params), P, P]
         L.sfbhip_direct.argtypes = [P, P, i64, ctypes.POINTER(Params), P, P, P, P]
         L.sfbhip_direct_batched.argtypes = [P, ctypes.POINTER(Catalog), i32, ctypes.POINTER(Params), P]
+        L.sfbhip_direct_noalloc.argtypes = [P, i64, ctypes.POINTER(Params), P, P, P, P]
         L.sfbhip_pixelize.argtypes = [P, P, i64, i32, P, P, P]
         L.sfbhip_ang2pix_ring.argtypes = [i32, ctypes.c_double, ctypes.c_double, ctypes.POINTER(i64)]
         L.sfbhip_healpix.argtypes = [P, P, P, i32, ctypes.POINTER(Params), P, P, P, P]

@@ -16,6 +16,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -3106,6 +3107,201 @@ sfbhip_status sfbhip_batch_autotune(sfbhip_handle *h, const sfbhip_catalog *cats
   }
   h->tiles = best;
   if (out) *out = best;
+  return SFBHIP_OK;
+}
+
+}  // extern "C"
+
+// ---- sfbhip_direct_noalloc: sfbhip_direct (FP64, exact j_l) for callers
+// that cannot allocate device memory.  Every device buffer beyond the
+// caller's points and outputs -- error flags, Bessel zeros, n_l, Legendre
+// constants, the point table, work items, reduction groups, partial blocks
+// and, without a caller f_lmn, the f_lmn workspace -- is carved from one
+// static device array per GPU.  Calls that overlap on one GPU race for it.
+
+constexpr size_t NOALLOC_SCRATCH_BYTES = size_t(128) << 20;
+static __device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH_BYTES];
+
+namespace {
+
+constexpr int NOALLOC_MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+unsigned char *g_scratch_ptr[NOALLOC_MAX_DEVICES] = {};  // device address of `scratch` on each GPU
+
+// The device address of this GPU's copy of `scratch`, looked up once per
+// device with the device current (host code never uses the array's name).
+sfbhip_status scratch_of_device(int dev, unsigned char **out) {
+  if (dev < 0 || dev >= NOALLOC_MAX_DEVICES)
+    return fail(SFBHIP_NOT_SUPPORTED, "no-allocation entry point: device index beyond " +
+                                          std::to_string(NOALLOC_MAX_DEVICES - 1));
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void *ptr = nullptr;
+    HIPCHK(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    if (!ptr) return fail(SFBHIP_HIP_ERROR, "hipGetSymbolAddress(scratch) returned NULL");
+    g_scratch_ptr[dev] = static_cast<unsigned char *>(ptr);
+  }
+  *out = g_scratch_ptr[dev];
+  return SFBHIP_OK;
+}
+
+// Consecutive 256-byte aligned pieces of the scratch array; `used` may pass
+// the capacity, which the caller checks before touching any piece.
+struct Carve {
+  unsigned char *base;
+  size_t used = 0;
+  template <class T>
+  T *take(size_t count) {
+    T *p = reinterpret_cast<T *>(base + used);
+    used += (count * sizeof(T) + 255) & ~size_t(255);
+    return p;
+  }
+};
+
+sfbhip_status noalloc_too_big(size_t need) {
+  return fail(SFBHIP_NOT_SUPPORTED, "no-allocation entry point: this call needs " + std::to_string(need) +
+                                        " bytes of device workspace, the library holds " +
+                                        std::to_string(NOALLOC_SCRATCH_BYTES) + " per GPU; use sfbhip_direct");
+}
+
+}  // namespace
+
+extern "C" {
+
+sfbhip_status sfbhip_direct_noalloc(const double *d_points, int64_t n_points, const sfbhip_params *p,
+                                    double *d_k_ln, double *d_c_ln, double *d_f_lmn, void *stream) {
+  g_last_error.clear();
+  CHK(validate(p));
+  if (p->precision != SFBHIP_PREC_FP64)
+    return fail(SFBHIP_NOT_SUPPORTED, "no-allocation entry point: precision FP64 only");
+  if (p->bessel == SFBHIP_BESSEL_REFERENCE || (p->bessel == SFBHIP_BESSEL_AUTO && n_points > 1000))
+    return fail(SFBHIP_NOT_SUPPORTED,
+                "no-allocation entry point: exact j_l only (bessel EXACT, or AUTO with at most 1000 points)");
+  if (n_points <= 0 || !d_points)
+    return fail(SFBHIP_INVALID_ARGUMENT, "empty catalogue: the normalisation divides by the number of points");
+  if (!d_k_ln || !d_c_ln) return fail(SFBHIP_INVALID_ARGUMENT, "k_ln / c_ln output is NULL");
+  if ((uint64_t)n_points > NOALLOC_SCRATCH_BYTES / (6 * sizeof(double))) return noalloc_too_big(48 * (size_t)n_points);
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  int dev = 0;
+  HIPCHK(hipGetDevice(&dev));
+  unsigned char *base = nullptr;
+  CHK(scratch_of_device(dev, &base));
+  if (p->verbose) {
+    std::printf("SFB decomposition (raw method): lmax=%d nmax=%d Rmax=%g\n", p->lmax, p->nmax, p->rmax);
+    std::printf("Catalog consists of %lld points.\n", (long long)n_points);
+  }
+  const int L = p->lmax, N = p->nmax, NZ = N + 1;
+  Carve cv{base};
+  int *derr = cv.take<int>(4);
+  double *dz = cv.take<double>((size_t)std::max(L, 1) * NZ);
+  if (cv.used > NOALLOC_SCRATCH_BYTES) return noalloc_too_big(cv.used);
+  fill_int_kernel<<<1, 64, 0, st>>>(derr, 4, 0);
+  HIPCHK(hipGetLastError());
+  // zeros of j_l on the device, k_ln and n_l on the host (as sfbhip_direct)
+  std::vector<double> zeros((size_t)std::max(L, 1) * NZ, 0.0), kln((size_t)std::max(L, 1) * N, 0.0);
+  std::vector<int32_t> nl(std::max(L, 1), 0);
+  if (L > 0) {
+    bessel_zeros_kernel<<<cdiv(L, 64), 64, 0, st>>>(L, NZ, p->zeros, dz);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(zeros.data(), dz, sizeof(double) * L * NZ, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+  }
+  const double kmax = p->k_max > 0 ? p->k_max : 0.0;
+  for (int l = 0; l < L; ++l) {
+    int cnt = 0;
+    for (int n = 0; n < N; ++n) {
+      const double z = zeros[(size_t)l * NZ + n];
+      kln[(size_t)l * N + n] = z / p->rmax;
+      if (kmax <= 0.0 || z < kmax * p->rmax) cnt = n + 1;
+    }
+    nl[l] = cnt;
+  }
+  // the work items of sfbhip_direct's FP64 path
+  const double norm = norm_factor(p, (double)n_points);
+  PlanCtx cx;
+  cx.nmax = N;
+  cx.norm = norm;
+  cx.kln = kln.data();
+  cx.R = p->rmax;
+  DirectPlan pl64;
+  plan_direct(nl, 0, L, n_points, F64_NT, F64_TP, F64_MC, 16, f64_target_items(n_points, false), cx, &pl64);
+  if (!pl64.items[5].empty()) return fail(SFBHIP_NOT_SUPPORTED, "FP64 plan class beyond 16 chunks");
+  std::vector<DirectItem> all;
+  for (auto &v : pl64.items) all.insert(all.end(), v.begin(), v.end());
+  const std::vector<ReduceGroup> &groups = pl64.groups;
+  const std::vector<int64_t> foff = flmn_offsets(L, N);
+  std::vector<double2> hs, hc;
+  std::vector<int32_t> ho;
+  legendre_host(std::max(L, 1), hs, hc, ho);
+  const size_t nf = sfbhip_flmn_count(L, N);
+  // the rest of the workspace
+  int32_t *dnl = cv.take<int32_t>(nl.size());
+  int64_t *dfoff = cv.take<int64_t>(foff.size());
+  double2 *dsect = cv.take<double2>(hs.size());
+  double2 *dcoef = cv.take<double2>(hc.size());
+  int32_t *dloff = cv.take<int32_t>(ho.size());
+  double *soa = cv.take<double>((size_t)6 * n_points);
+  DirectItem *ditems = cv.take<DirectItem>(std::max<size_t>(all.size(), 1));
+  ReduceGroup *dg = cv.take<ReduceGroup>(std::max<size_t>(groups.size(), 1));
+  double *partial = cv.take<double>((size_t)std::max<int64_t>(pl64.partial_size, 1));
+  double *f = d_f_lmn ? d_f_lmn : cv.take<double>(std::max<size_t>(2 * nf, 1));
+  if (cv.used > NOALLOC_SCRATCH_BYTES) return noalloc_too_big(cv.used);
+  if (L > 0)
+    HIPCHK(hipMemcpyAsync(d_k_ln, kln.data(), sizeof(double) * L * N, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(dnl, nl.data(), sizeof(int32_t) * nl.size(), hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(dfoff, foff.data(), sizeof(int64_t) * foff.size(), hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(dsect, hs.data(), sizeof(double2) * hs.size(), hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(dcoef, hc.data(), sizeof(double2) * hc.size(), hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(dloff, ho.data(), sizeof(int32_t) * ho.size(), hipMemcpyHostToDevice, st));
+  if (!all.empty())
+    HIPCHK(hipMemcpyAsync(ditems, all.data(), sizeof(DirectItem) * all.size(), hipMemcpyHostToDevice, st));
+  if (!groups.empty())
+    HIPCHK(hipMemcpyAsync(dg, groups.data(), sizeof(ReduceGroup) * groups.size(), hipMemcpyHostToDevice, st));
+  // point table; a bad point is refused before any kernel evaluates a Bessel function of it
+  const int64_t n = n_points;
+  double *r = soa, *x = soa + n, *s = soa + 2 * n, *ph = soa + 3 * n, *cp = soa + 4 * n, *sp = soa + 5 * n;
+  prep_points_kernel<<<cdiv(n, 256), 256, 0, st>>>(d_points, n, r, x, s, ph, cp, sp, derr,
+                                                    reinterpret_cast<unsigned long long *>(derr + 2));
+  HIPCHK(hipGetLastError());
+  int herr = 0;
+  HIPCHK(hipMemcpyAsync(&herr, derr, sizeof(int), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  if (herr & ERR_BADPOINT) return fail(SFBHIP_DOMAIN_ERROR, "point with r < 0 or a non-finite coordinate");
+  const PointsSoA pts{r, x, s, ph, cp, sp, n};
+  const LegendreTables lt{dsect, dcoef, dloff};
+  size_t off = 0;
+  for (int ci = 0; ci < 5; ++ci) {
+    const int k = (int)pl64.items[ci].size();
+    if (!k) continue;
+    const DirectItem *di = ditems + off;
+    switch (ci) {
+      case 0: launch_f64<1>(k, di, st, pts, nullptr, d_k_ln, N, lt, JL_EXACT, BesselTableRef{}, partial, derr); break;
+      case 1: launch_f64<2>(k, di, st, pts, nullptr, d_k_ln, N, lt, JL_EXACT, BesselTableRef{}, partial, derr); break;
+      case 2: launch_f64<4>(k, di, st, pts, nullptr, d_k_ln, N, lt, JL_EXACT, BesselTableRef{}, partial, derr); break;
+      case 3: launch_f64<8>(k, di, st, pts, nullptr, d_k_ln, N, lt, JL_EXACT, BesselTableRef{}, partial, derr); break;
+      default: launch_f64<16>(k, di, st, pts, nullptr, d_k_ln, N, lt, JL_EXACT, BesselTableRef{}, partial, derr); break;
+    }
+    HIPCHK(hipGetLastError());
+    off += k;
+  }
+  // f_lmn (zeroed, then every group's sum), c_ln
+  if (nf) {
+    fill_kernel<<<cdiv(2 * nf, 256), 256, 0, st>>>(f, (int64_t)(2 * nf), 0.0);
+    HIPCHK(hipGetLastError());
+  }
+  if (!groups.empty()) {
+    int chunks = 1;
+    for (const ReduceGroup &g : groups)
+      chunks = std::max<int>(chunks, (int)cdiv((int64_t)g.nrows * 2 * (g.l + 1), 256));
+    reduce_partials_kernel<<<dim3((unsigned)groups.size(), (unsigned)std::min(chunks, 64)), 256, 0, st>>>(
+        dg, partial, d_k_ln, f);
+    HIPCHK(hipGetLastError());
+  }
+  if (L > 0) {
+    cln_kernel<<<cdiv((int64_t)L * N, 256), 256, 0, st>>>(L, N, dnl, dfoff, f, d_c_ln);
+    HIPCHK(hipGetLastError());
+  }
+  HIPCHK(hipStreamSynchronize(st));  // the host vectors the copies read go out of scope
   return SFBHIP_OK;
 }
 

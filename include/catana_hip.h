@@ -117,6 +117,19 @@ sfbhip_status sfbhip_direct(sfbhip_handle *h, const double *d_points, int64_t n_
                             const sfbhip_params *p, double *d_k_ln, double *d_c_ln, double *d_f_lmn,
                             void *stream);
 
+/* sfbhip_direct without any device allocation, for callers that cannot
+ * allocate (serving loops that must not call hipMalloc, harnesses that forbid
+ * it).  No counterpart in the reference.  Precision FP64 with exact j_l only
+ * (bessel EXACT, or AUTO with at most 1000 points); results are bit-identical
+ * to sfbhip_direct's for the same arguments.  Every device buffer beyond the
+ * caller's points and outputs lives in one static 128 MiB array per GPU
+ * (held until the process ends); a call that would need more -- about 48
+ * bytes per point plus the partial sums -- fails with SFBHIP_NOT_SUPPORTED.
+ * Runs on the current device.  Calls that overlap on one GPU (other host
+ * threads, other streams) share that array and race: serialise them. */
+sfbhip_status sfbhip_direct_noalloc(const double *d_points, int64_t n_points, const sfbhip_params *p,
+                                    double *d_k_ln, double *d_c_ln, double *d_f_lmn, void *stream);
+
 /* One catalogue of a batched call (device buffers, shapes as for sfbhip_direct). */
 typedef struct sfbhip_catalog {
   const double *d_points;   /* AoS (r, theta, phi), n_points records */
