@@ -87,7 +87,7 @@ def make_points(name):
 # default CPU-baseline subsample per workload (points): each timed run is a few seconds on 16 threads
 CPU_SAMPLE = {"clustered1m": 1 << 17, "lognormal2p25": 4096, "uniform10k": 10000, "batched64": 1 << 15}
 CPU_RUNS = 3
-KERNELS = {"auto": 0, "256": 1, "512": 2}
+KERNELS = {"auto": 0, "256": 1, "512": 2, "grouped": 3}
 
 
 def log(msg):
@@ -154,7 +154,7 @@ def main():
     ap.add_argument("--chunks", type=int, default=4, choices=[1, 2, 4, 8],
                     help="mixed path: m chunks accumulated per work item")
     ap.add_argument("--fp64-below", type=int, default=2, help="mixed path: l below this run in FP64")
-    ap.add_argument("--kernel", default="auto", choices=["auto", "256", "512"],
+    ap.add_argument("--kernel", default="auto", choices=sorted(KERNELS),
                     help="mixed path, split-FP16 operands: the contraction kernel")
     ap.add_argument("--batch-mode", default="graph", choices=["graph", "plan", "call"],
                     help="batched64: replay a captured batch plan (graph), enqueue the plan (plan), or one "

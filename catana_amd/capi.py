@@ -16,7 +16,7 @@ PREC_FP64, PREC_MIXED = 0, 1
 BESSEL_EXACT, BESSEL_REFERENCE, BESSEL_AUTO = 0, 1, 2
 ZEROS_REFERENCE, ZEROS_CONVERGED = 0, 1
 OPERANDS_FP32, OPERANDS_SPLIT_FP16, OPERANDS_SPLIT3_FP16 = 0, 1, 2
-MIXED_KERNEL_AUTO, MIXED_KERNEL_256, MIXED_KERNEL_512 = 0, 1, 2
+MIXED_KERNEL_AUTO, MIXED_KERNEL_256, MIXED_KERNEL_512, MIXED_KERNEL_GROUPED = 0, 1, 2, 3
 
 EXPORTED = [
     "sfbhip_default_params", "sfbhip_create", "sfbhip_destroy", "sfbhip_flmn_count", "sfbhip_kln",

@@ -24,6 +24,7 @@
 #include "sfb_direct_f64.hpp"
 #include "sfb_direct_mixed.hpp"
 #include "sfb_direct_mixed2.hpp"
+#include "sfb_direct_split16.hpp"
 #include "sfb_lowl.hpp"
 #include "sfb_healpix.hpp"
 #include "sfb_healpix_ring.hpp"
@@ -160,7 +161,7 @@ sfbhip_status validate(const sfbhip_params *p) {
   if (p->mixed_chunks != 0 && p->mixed_chunks != 1 && p->mixed_chunks != 2 && p->mixed_chunks != 4 &&
       p->mixed_chunks != 8)
     return fail(SFBHIP_INVALID_ARGUMENT, "mixed_chunks must be 0, 1, 2, 4 or 8");
-  if (p->mixed_kernel < SFBHIP_MIXED_KERNEL_AUTO || p->mixed_kernel > SFBHIP_MIXED_KERNEL_512)
+  if (p->mixed_kernel < SFBHIP_MIXED_KERNEL_AUTO || p->mixed_kernel > SFBHIP_MIXED_KERNEL_GROUPED)
     return fail(SFBHIP_INVALID_ARGUMENT, "unknown mixed kernel");
   if (p->mixed_operands != SFBHIP_OPERANDS_FP32 && p->mixed_operands != SFBHIP_OPERANDS_SPLIT_FP16 &&
       p->mixed_operands != SFBHIP_OPERANDS_SPLIT3_FP16)
@@ -1058,7 +1059,7 @@ sfbhip_status launch_plan_mixed2(sfbhip_handle *h, const DirectPlan &pl, const s
 }
 
 // The 512-thread kernel runs the mixed path's split-FP16 contraction when it
-// is asked for; AUTO takes the 256-thread kernel, measured faster on MI355X
+// is asked for; it was measured slower than the 256-thread kernel on MI355X
 // (2^25 points, lmax 256: 4.34 s against 4.76 s; 1M clustered points: 6.70
 // against 6.96 ms): the 512-thread kernel issues 17 % fewer vector
 // instructions, but its eight waves per CU share one block's barriers and
@@ -1069,6 +1070,69 @@ bool use_mixed2(const sfbhip_params *p) {
          p->mixed_kernel == SFBHIP_MIXED_KERNEL_512;
 }
 int mixed2_qcap(const sfbhip_params *p) { return std::min(p->mixed_chunks ? p->mixed_chunks : 4, 4); }
+
+// The split-FP16 kernel with grouped Bessel evaluation (sfb_direct_split16.hpp)
+// runs the mixed path's contraction by default (AUTO): measured on MI355X
+// against the 256-thread kernel of sfb_direct_mixed.hpp, 2^25 points, lmax 256:
+// 3.52 s against 4.32 s; 1M clustered points: 5.60 against 6.68 ms.
+bool use_split16(const sfbhip_params *p) {
+  return p->precision == SFBHIP_PREC_MIXED && p->mixed_operands == SFBHIP_OPERANDS_SPLIT_FP16 &&
+         (p->mixed_kernel == SFBHIP_MIXED_KERNEL_GROUPED || p->mixed_kernel == SFBHIP_MIXED_KERNEL_AUTO);
+}
+
+template <int QMAX, int KS>
+sfbhip_status launch_split16(int nitems, const DirectItem *items, hipStream_t st, const PointsSoA &pts,
+                             const double *kln, const LegendreScaled &ls, const CubicTables &ct, double *partial) {
+  sfb_direct_split16_kernel<QMAX, KS><<<nitems, 256, 0, st>>>(pts, items, kln, ls, ct, partial);
+  HIPCHK(hipGetLastError());
+  return SFBHIP_OK;
+}
+
+template <int QMAX>
+sfbhip_status launch_split16_ks(int ksi, int nitems, const DirectItem *items, hipStream_t st, const PointsSoA &pts,
+                                const double *kln, const LegendreScaled &ls, const CubicTables &ct,
+                                double *partial) {
+  if (ksi == 2) return launch_split16<QMAX, 4>(nitems, items, st, pts, kln, ls, ct, partial);
+  if (ksi == 1) return launch_split16<QMAX, 2>(nitems, items, st, pts, kln, ls, ct, partial);
+  return launch_split16<QMAX, 1>(nitems, items, st, pts, kln, ls, ct, partial);
+}
+
+// All classes of a 256-thread mixed plan (plan_direct, uploaded in class
+// order) on the grouped-Bessel kernel, dealt in turn to the call's stream and
+// the two side streams.
+sfbhip_status launch_split16_classes(sfbhip_handle *h, const DirectPlan &pl, DirectItem *ditems, size_t *off,
+                                     hipStream_t st, const PointsSoA &pts, const double *dk, const LegendreScaled &ls,
+                                     const CubicTables &ct, double *partial, bool *used) {
+  const hipStream_t streams[3] = {st, h->mx_side[0], h->mx_side[1]};
+  int next = 0;
+  for (int idx = 0; idx < 18; ++idx) {
+    const int ci = idx % 6, ksi = idx / 6;
+    const int n = (int)pl.items[idx].size();
+    if (!n) continue;
+    if (ci > 3) return fail(SFBHIP_NOT_SUPPORTED, "mixed plan with more than 8 chunks per item");
+    const DirectItem *di = ditems + *off;
+    const int si = next;
+    next = (next + 1) % 3;
+    const hipStream_t s = streams[si];
+    if (si > 0) used[si - 1] = true;
+    switch (ci) {
+      case 0: CHK(launch_split16_ks<1>(ksi, n, di, s, pts, dk, ls, ct, partial)); break;
+      case 1: CHK(launch_split16_ks<2>(ksi, n, di, s, pts, dk, ls, ct, partial)); break;
+      case 2: CHK(launch_split16_ks<4>(ksi, n, di, s, pts, dk, ls, ct, partial)); break;
+      default: CHK(launch_split16_ks<8>(ksi, n, di, s, pts, dk, ls, ct, partial)); break;
+    }
+    *off += n;
+  }
+  return SFBHIP_OK;
+}
+
+sfbhip_status launch_plan_split16(sfbhip_handle *h, const DirectPlan &pl, DirectItem *ditems, size_t *off,
+                                  hipStream_t st, const PointsSoA &pts, const double *dk, const LegendreScaled &ls,
+                                  const CubicTables &ct, double *partial) {
+  CHK(fork_sides(h, st));
+  bool used[2] = {false, false};
+  return join_sides(h, st, launch_split16_classes(h, pl, ditems, off, st, pts, dk, ls, ct, partial, used), used);
+}
 
 // The low-l FP64 items (all QMAX class 1), j_l from the quintic table.
 sfbhip_status launch_lowl(const DirectPlan &pl, DirectItem *ditems, size_t *off, hipStream_t st,
@@ -1480,9 +1544,9 @@ sfbhip_status sfbhip_direct(sfbhip_handle *h, const double *d_points, int64_t n_
   HermiteTables ht{nullptr, nullptr, nullptr};
   CubicTables ct{nullptr, nullptr, nullptr};
   LegendreScaled ls{};
-  const bool m2 = use_mixed2(p);
+  const bool m2 = use_mixed2(p), s16 = use_split16(p);
   if (mixed && lsplit < p->lmax) {
-    if (m2) {
+    if (m2 || s16) {
       CHK(ensure_cubic(h, p, rdata, st, &ct));
       CHK(ensure_legendre_scaled(h, std::max(p->lmax, 1), st, &ls));
     } else {
@@ -1536,6 +1600,8 @@ sfbhip_status sfbhip_direct(sfbhip_handle *h, const double *d_points, int64_t n_
   HIPCHK(hipEventRecord(h->ev[1], st));
   if (mixed && m2)
     CHK(launch_plan_mixed2(h, plmx, h->kc.cub_nint, ditems, &off, st, pts, dk, ls, ct, partial));
+  else if (mixed && s16)
+    CHK(launch_plan_split16(h, plmx, ditems, &off, st, pts, dk, ls, ct, partial));
   else if (mixed)
     CHK(launch_plan_mixed(h, plmx, h->kc.herm_nint, ditems, &off, st, pts, dk, p->nmax, lt, ht,
                           p->mixed_operands, partial));
@@ -1727,15 +1793,15 @@ sfbhip_status sfbhip_direct_batched(sfbhip_handle *h, const sfbhip_catalog *cats
     tab.q5_nint = dni;
     HIPCHK(hipStreamSynchronize(st));  // host vectors go out of scope
   }
-  const bool m2 = use_mixed2(p);
+  const bool m2 = use_mixed2(p), s16 = use_split16(p);
   CubicTables ct{nullptr, nullptr, nullptr};
   LegendreScaled ls{};
   std::vector<int32_t> cnint;
-  if (mixed && lsplit < L && m2) {
+  if (mixed && lsplit < L && (m2 || s16)) {
     CHK(build_cubic(h, L, xmax, "b_cub", st, &ct, &cnint));
     CHK(ensure_legendre_scaled(h, std::max(L, 1), st, &ls));
   }
-  if (mixed && lsplit < L && !m2) {
+  if (mixed && lsplit < L && !m2 && !s16) {
     std::vector<int32_t> off(L, 0);
     int64_t o = 0;
     int32_t mx = 1;
@@ -1829,6 +1895,8 @@ sfbhip_status sfbhip_direct_batched(sfbhip_handle *h, const sfbhip_catalog *cats
   HIPCHK(hipEventRecord(h->ev[1], st));
   if (mixed && m2)
     CHK(launch_plan_mixed2(h, plmx, cnint, ditems, &off, st, pts, dk, ls, ct, partial));
+  else if (mixed && s16)
+    CHK(launch_plan_split16(h, plmx, ditems, &off, st, pts, dk, ls, ct, partial));
   else if (mixed)
     CHK(launch_plan_mixed(h, plmx, hnint, ditems, &off, st, pts, dk, 0, lt, ht, p->mixed_operands, partial));
   HIPCHK(hipEventRecord(h->ev[2], st));

@@ -85,10 +85,12 @@ typedef struct sfbhip_params {
 /* which contraction kernel the mixed path runs (split-FP16 operands; the other operand formats
  * always run the 256-thread kernel) */
 enum {
-  SFBHIP_MIXED_KERNEL_AUTO = 0,  /* the 256-thread kernel (measured faster on MI355X) */
+  SFBHIP_MIXED_KERNEL_AUTO = 0,  /* GROUPED (measured fastest on MI355X); batch plans: 256 */
   SFBHIP_MIXED_KERNEL_256 = 1,   /* 256-thread blocks, 64-row items, Bessel nodes in LDS */
-  SFBHIP_MIXED_KERNEL_512 = 2    /* 512-thread blocks, items of up to 128 rows, cubic Bessel
+  SFBHIP_MIXED_KERNEL_512 = 2,   /* 512-thread blocks, items of up to 128 rows, cubic Bessel
                                     coefficients in LDS, double-buffered Y* halves */
+  SFBHIP_MIXED_KERNEL_GROUPED = 3 /* 256-thread blocks, 64-row items, one cubic per (row, 8-point
+                                    group) for the Bessel values, two-operation Legendre steps */
 };
 
 typedef struct sfbhip_handle sfbhip_handle;

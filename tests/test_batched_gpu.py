@@ -49,7 +49,7 @@ def test_batched_fp64_vs_oracle(handle):
         assert _flmn_err(capi.split_flmn(f.cpu().numpy(), c["lmax"], c["nmax"]), fo) < 1e-10
 
 
-@pytest.mark.parametrize("kern", [capi.MIXED_KERNEL_256, capi.MIXED_KERNEL_512])
+@pytest.mark.parametrize("kern", [capi.MIXED_KERNEL_256, capi.MIXED_KERNEL_512, capi.MIXED_KERNEL_GROUPED])
 def test_batched_mixed_vs_fp64_and_single(handle, kern):
     cats = _catalogs()
     print(f"case: batched mixed vs batched FP64 and single calls, kernel {kern}", flush=True)
@@ -81,7 +81,7 @@ def test_batched_errors(handle):
         capi.sfb_direct_batched_torch(handle, [cats[0], bad], precision=capi.PREC_FP64)
 
 
-@pytest.mark.parametrize("kern", [capi.MIXED_KERNEL_256, capi.MIXED_KERNEL_512])
+@pytest.mark.parametrize("kern", [capi.MIXED_KERNEL_256, capi.MIXED_KERNEL_512, capi.MIXED_KERNEL_GROUPED])
 def test_batched_mixed_first_radius_not_smallest(handle, kern):
     # the batched sort orders by r with its lowest 7 bits dropped, so within a
     # key group points keep their input order.  Here the first key group holds

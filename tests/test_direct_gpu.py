@@ -73,7 +73,7 @@ def _rel(a, b):
     return np.abs(a[m] / b[m] - 1)
 
 
-@pytest.mark.parametrize("kern", [capi.MIXED_KERNEL_256, capi.MIXED_KERNEL_512])
+@pytest.mark.parametrize("kern", [capi.MIXED_KERNEL_256, capi.MIXED_KERNEL_512, capi.MIXED_KERNEL_GROUPED])
 def test_mixed_vs_fp64_uniform(handle, kern):
     pts = workloads.uniform_ball(20000, 500.0, seed=21)
     print(f"kernel {kern} case: mixed vs fp64, uniform N=20000 lmax=24 k_max=0.15 rmax=500", flush=True)
@@ -88,7 +88,7 @@ def test_mixed_vs_fp64_uniform(handle, kern):
     assert np.all((cm == 0) == (c64 == 0))
 
 
-@pytest.mark.parametrize("kern", [capi.MIXED_KERNEL_256, capi.MIXED_KERNEL_512])
+@pytest.mark.parametrize("kern", [capi.MIXED_KERNEL_256, capi.MIXED_KERNEL_512, capi.MIXED_KERNEL_GROUPED])
 def test_mixed_all_l_in_mixed_kernel(handle, kern):
     # mixed_fp64_below = 0: every l through the FP32-operand kernel; f_lmn stays
     # within 1e-6 of FP64 relative to each l block's largest coefficient
@@ -101,7 +101,7 @@ def test_mixed_all_l_in_mixed_kernel(handle, kern):
     assert np.median(_rel(cm, c64)) < 2e-7
 
 
-@pytest.mark.parametrize("kern", [capi.MIXED_KERNEL_256, capi.MIXED_KERNEL_512])
+@pytest.mark.parametrize("kern", [capi.MIXED_KERNEL_256, capi.MIXED_KERNEL_512, capi.MIXED_KERNEL_GROUPED])
 def test_mixed_vs_oracle_points_beyond_rmax(handle, kern):
     # r up to 1.3 rmax: Bessel arguments past the table fall back to FP64 j_l
     pts = workloads.uniform_ball(4000, 650.0, seed=22)
@@ -112,7 +112,7 @@ def test_mixed_vs_oracle_points_beyond_rmax(handle, kern):
     assert _flmn_err(f, fo) < 2e-6
 
 
-@pytest.mark.parametrize("kern", [capi.MIXED_KERNEL_256, capi.MIXED_KERNEL_512])
+@pytest.mark.parametrize("kern", [capi.MIXED_KERNEL_256, capi.MIXED_KERNEL_512, capi.MIXED_KERNEL_GROUPED])
 def test_mixed_vs_fp64_clustered_high_l(handle, kern):
     pts = workloads.clustered_ball(1500, 1000.0, mean_children=40, sigma=5.0, seed=23)
     print(f"kernel {kern} case: mixed vs fp64, clustered N={pts.shape[0]} lmax=140 k_max=0.2 rmax=1000", flush=True)
@@ -125,7 +125,7 @@ def test_mixed_vs_fp64_clustered_high_l(handle, kern):
     assert _flmn_err(fm, f64) < 1e-6
 
 
-@pytest.mark.parametrize("kern", [capi.MIXED_KERNEL_256, capi.MIXED_KERNEL_512])
+@pytest.mark.parametrize("kern", [capi.MIXED_KERNEL_256, capi.MIXED_KERNEL_512, capi.MIXED_KERNEL_GROUPED])
 def test_mixed_vs_fp64_two_row_groups(handle, kern):
     # n_l up to 76 > 64: multipoles with a second 64-row group (its 12 rows run
     # with the tile's points split four ways) and, for l >= 64, items whose m
@@ -163,7 +163,8 @@ def test_mixed_fp64_multipoles_tables(handle):
 @pytest.mark.parametrize("ops,chunks,kern", [(capi.OPERANDS_FP32, 4, 0), (capi.OPERANDS_SPLIT_FP16, 1, 1),
                                              (capi.OPERANDS_SPLIT_FP16, 8, 1), (capi.OPERANDS_SPLIT3_FP16, 2, 0),
                                              (capi.OPERANDS_SPLIT_FP16, 1, 2), (capi.OPERANDS_SPLIT_FP16, 2, 2),
-                                             (capi.OPERANDS_SPLIT_FP16, 4, 2)])
+                                             (capi.OPERANDS_SPLIT_FP16, 4, 2), (capi.OPERANDS_SPLIT_FP16, 1, 3),
+                                             (capi.OPERANDS_SPLIT_FP16, 2, 3), (capi.OPERANDS_SPLIT_FP16, 8, 3)])
 def test_mixed_operand_formats_and_chunks(handle, ops, chunks, kern):
     # every operand format and work-item chunk count against FP64; l up to 47
     # spans 3 chunks, so chunks=1 and 2 split the m range over several passes
@@ -215,7 +216,8 @@ def test_device_jl_vs_mpmath_goldens(handle, golden_dir):
 
 
 @pytest.mark.parametrize("n,kern", [(400, capi.MIXED_KERNEL_512), (400, capi.MIXED_KERNEL_256),
-                                    (6000, capi.MIXED_KERNEL_512)])
+                                    (6000, capi.MIXED_KERNEL_512), (400, capi.MIXED_KERNEL_GROUPED),
+                                    (6000, capi.MIXED_KERNEL_GROUPED)])
 def test_mixed_lmax256_third_row_group_and_global_table(handle, n, kern):
     # lmax 256, k_max 0.5, rmax 1000: n_l from 159 down to 52, so l >= 141 and rows
     # n >= 128 (a third 64-row group, or the rows after a 128-row item) are
