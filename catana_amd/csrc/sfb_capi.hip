@@ -1079,6 +1079,8 @@ bool use_split16(const sfbhip_params *p) {
   return p->precision == SFBHIP_PREC_MIXED && p->mixed_operands == SFBHIP_OPERANDS_SPLIT_FP16 &&
          (p->mixed_kernel == SFBHIP_MIXED_KERNEL_GROUPED || p->mixed_kernel == SFBHIP_MIXED_KERNEL_AUTO);
 }
+// chunks per item of a 256-thread-class plan
+int mixed_qcap(const sfbhip_params *p) { return p->mixed_chunks ? p->mixed_chunks : 4; }
 
 template <int QMAX, int KS>
 sfbhip_status launch_split16(int nitems, const DirectItem *items, hipStream_t st, const PointsSoA &pts,
@@ -1576,7 +1578,7 @@ sfbhip_status sfbhip_direct(sfbhip_handle *h, const double *d_points, int64_t n_
     if (m2)
       plan_mixed2(h->kc.nl, lsplit, p->lmax, n_points, mixed2_qcap(p), target, cx, &plmx);
     else
-      plan_direct(h->kc.nl, lsplit, p->lmax, n_points, MX_NT, MX_TP, MX_MC, p->mixed_chunks ? p->mixed_chunks : 4,
+      plan_direct(h->kc.nl, lsplit, p->lmax, n_points, MX_NT, MX_TP, MX_MC, mixed_qcap(p),
                   target, cx, &plmx, mixed_ks_cap(p->mixed_operands));
   }
   std::vector<ReduceGroup> groups = pllw.groups;
@@ -1869,7 +1871,7 @@ sfbhip_status sfbhip_direct_batched(sfbhip_handle *h, const sfbhip_catalog *cats
                     ctx(c), &plmx);
       else
         plan_direct(nl[c], std::min(lsplit, cats[c].lmax), cats[c].lmax, cats[c].n_points, MX_NT, MX_TP, MX_MC,
-                    p->mixed_chunks ? p->mixed_chunks : 4, target(c), ctx(c), &plmx, mixed_ks_cap(p->mixed_operands));
+                    mixed_qcap(p), target(c), ctx(c), &plmx, mixed_ks_cap(p->mixed_operands));
     }
   }
   std::vector<ReduceGroup> groups = pllw.groups;

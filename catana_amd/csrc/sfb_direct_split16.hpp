@@ -23,7 +23,14 @@
 //     Q^{m-1} = g_m cot(t) Q^m - Q^{m+1} (P^m = d_m Q^m, d_m applied to the
 //     FP64 sums at the end), carried binary exponent, FP64; azimuth by the
 //     Chebyshev recurrence in FP64; Y* 2^12 rounded once to FP32 and split,
-//     each FP16 remainder formed and rounded in one v_fma_mix{lo,hi}_f16.
+//     each FP16 remainder formed and rounded in one v_fma_mix{lo,hi}_f16; the
+//     chunk is staged point-major (64-bit stores) and read by transposed LDS
+//     loads.
+// Measured and not kept: generating chunk q+1 into a second buffer while
+// contracting chunk q in the same wave (11 % slower), and a 512-thread form in
+// which four waves generate Y* while four contract (1.5x slower): the two
+// 256-thread blocks per CU already overlap one block's vector work with the
+// other's matrix-core work.
 // Work item: (l, 64 rows of n, m chunks [qlo, qhi), point range), as for the
 // 256-thread kernel of sfb_direct_mixed.hpp, including the point split KS for
 // items of at most 32 (16) rows.
@@ -35,21 +42,70 @@
 
 namespace sfb {
 
-constexpr int S16_GPAD = 16;   // g_m table padding below m = 0
-constexpr int S16_LDP = MX_TP + 8;             // row pitch of a Y* column (FP16 elements)
-constexpr int S16_PLANE = 2 * MX_MC * S16_LDP;  // one FP16 piece of a chunk
+constexpr int S16_GPAD = 16;  // g_m table padding below m = 0
 constexpr float S16_FLO = -0.2f, S16_FHI = 1.2f;  // range of f served by one interval's cubic
+
+// Y* chunk in LDS, point-major: plane k (FP16 piece k) holds one row of 32
+// columns (2 mm: real, 2 mm + 1: imaginary part of m = mtop - mm) per tile
+// point, padded to S16_ROW elements (a 72-byte pitch keeps the row writes and
+// the transposed fragment reads spread over the banks).  A generating thread
+// writes its point's (re, im) pair of two m as one 64-bit store per plane
+// (16-bit stores measured at about 4.5 cycles per wave instruction); the
+// contraction reads each B fragment with two ds_read_b64_tr_b16, which hand
+// lane 4a + c element c of the 8-byte rows addressed by lanes a, a+4, a+8,
+// a+12 of its 16-lane group.
+constexpr int S16_ROW = 36;                // FP16 elements per point row of a plane
+constexpr int S16_PLANE = MX_TP * S16_ROW;  // one FP16 piece of a chunk
+constexpr int S16_YBUF = 2 * S16_PLANE;     // one Y* chunk buffer (both pieces), FP16 elements
 
 // The block's LDS areas.
 struct S16Shared {
-  unsigned short *y;  // Y* chunk: planes 0 (p0) and 1 (p1) of 32 columns x S16_LDP points
+  unsigned short *y;  // Y* chunk: planes 0 (p0) and 1 (p1), point-major
   double *rs;         // tile radii
   float *dg;          // r - r_mid of the point's 8-point group, FP32
   float *gs;          // per 8-point group: max |r - r_mid|
   double *cg;         // g_m of this l at cg[m] (cg[m] = 0 for m <= 0)
 };
 
-constexpr int S16_YBUF = 2 * S16_PLANE;  // one Y* chunk buffer (both pieces), FP16 elements
+typedef short s4v __attribute__((ext_vector_type(4)));
+typedef __attribute__((address_space(3))) s4v lds_s4v;
+
+// B fragment of v_mfma_f32_16x16x32_f16 from a point-major plane: lane 16 g + n
+// receives column col0 + n of points p0 + 8 g + e, e = 0..7 (the A fragment's
+// point order).  Lane 16 g + s supplies the row of point p0 + 8 g + 4 i + s / 4,
+// columns col0 + 4 (s % 4) .. +3, for the two reads i = 0, 1.
+__device__ __forceinline__ h8 s16_bfrag(const unsigned short *plane, int p0, int col0, int lane) {
+  const int g = lane >> 4, s = lane & 15;
+  const unsigned short *a = plane + (p0 + 8 * g + (s >> 2)) * S16_ROW + col0 + 4 * (s & 3);
+  const s4v x0 = __builtin_amdgcn_ds_read_tr16_b64_v4i16((lds_s4v *)(a));
+  const s4v x1 = __builtin_amdgcn_ds_read_tr16_b64_v4i16((lds_s4v *)(a + 4 * S16_ROW));
+  return __builtin_bit_cast(h8, __builtin_shufflevector(x0, x1, 0, 1, 2, 3, 4, 5, 6, 7));
+}
+
+// Y* 2^12 of this thread's point for m = mtop - mm, mm = m0 .. m0 + 2 K - 1,
+// into its rows of the two planes (64-bit stores of two m), stepping the
+// generator.  In the lowest chunk the steps past m = 0 use g = 0 and the
+// columns of m < 0 are computed and discarded.
+template <int K>
+__device__ __forceinline__ void s16_emit(YlmScaled &y, const double *cg, int mtop, int m0, unsigned short *Yw,
+                                         int tid) {
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    unsigned hi[2], lo[2];
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+      const int mm = m0 + 2 * k + u;
+      const double P = ldexp(y.v, y.E + 12);
+      const float re = (float)(P * y.cm), im = (float)(-P * y.sm);
+      split2(re, im, hi[u], lo[u]);
+      y.step(cg[mtop - mm]);
+      if ((mm & 7) == 7) y.rescale_nb();
+    }
+    const int e = tid * S16_ROW + 2 * (m0 + 2 * k);
+    *reinterpret_cast<uint2 *>(Yw + e) = make_uint2(hi[0], hi[1]);
+    *reinterpret_cast<uint2 *>(Yw + S16_PLANE + e) = make_uint2(lo[0], lo[1]);
+  }
+}
 
 // One phase of a tile.  MF: contract the chunk in Yr (this wave's 16 rows x 32
 // columns over its SPW 32-point steps: leading products p0 q0 in (a0, a1),
@@ -61,22 +117,19 @@ template <bool MF, bool GEN, int SPW>
 __device__ __forceinline__ void s16_phase(const h8 (&ah)[2][SPW], const unsigned short *Yr, v4d (&sum)[2],
                                           YlmScaled &y, const double *cg, int mtop, unsigned short *Yw, int tid,
                                           int lane, int kp, int g) {
-  constexpr int LDP = S16_LDP;
-  constexpr int PLANE = S16_PLANE;
   constexpr int MPS = MX_MC / SPW;  // m values generated per contraction step
+  static_assert(MPS % 2 == 0, "m values are stored in pairs");
   constexpr int FOLD = 64;          // points per FP32 leading-product partial sum
-  const int col = lane & 15;
   v4f a0 = v4f{0.f, 0.f, 0.f, 0.f}, a1 = v4f{0.f, 0.f, 0.f, 0.f};
   v4f c0 = v4f{0.f, 0.f, 0.f, 0.f}, c1 = v4f{0.f, 0.f, 0.f, 0.f};
-  const _Float16 *Yh = reinterpret_cast<const _Float16 *>(Yr);
 #pragma unroll
   for (int st = 0; st < SPW; ++st) {
     if constexpr (MF) {
-      const int o = col * LDP + 32 * (kp * SPW + st) + 8 * g;
-      const h8 b00 = *reinterpret_cast<const h8 *>(Yh + o);
-      const h8 b01 = *reinterpret_cast<const h8 *>(Yh + PLANE + o);
-      const h8 b10 = *reinterpret_cast<const h8 *>(Yh + o + 16 * LDP);
-      const h8 b11 = *reinterpret_cast<const h8 *>(Yh + PLANE + o + 16 * LDP);
+      const int p0 = 32 * (kp * SPW + st);
+      const h8 b00 = s16_bfrag(Yr, p0, 0, lane);
+      const h8 b01 = s16_bfrag(Yr + S16_PLANE, p0, 0, lane);
+      const h8 b10 = s16_bfrag(Yr, p0, 16, lane);
+      const h8 b11 = s16_bfrag(Yr + S16_PLANE, p0, 16, lane);
       c0 = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[0][st], b01, c0, 0, 0, 0);
       c0 = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[1][st], b00, c0, 0, 0, 0);
       c1 = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[0][st], b11, c1, 0, 0, 0);
@@ -93,24 +146,7 @@ __device__ __forceinline__ void s16_phase(const h8 (&ah)[2][SPW], const unsigned
         a1 = v4f{0.f, 0.f, 0.f, 0.f};
       }
     }
-    if constexpr (GEN) {
-      // m = mtop - mm; in the lowest chunk the steps past m = 0 use g = 0 and
-      // the columns of m < 0 are computed and discarded
-#pragma unroll
-      for (int k = 0; k < MPS; ++k) {
-        const int mm = st * MPS + k;
-        const double P = ldexp(y.v, y.E + 12);
-        const float re = (float)(P * y.cm), im = (float)(-P * y.sm);
-        unsigned hi, lo;
-        split2(re, im, hi, lo);
-        Yw[(2 * mm) * LDP + tid] = (unsigned short)hi;
-        Yw[(2 * mm + 1) * LDP + tid] = (unsigned short)(hi >> 16);
-        Yw[PLANE + (2 * mm) * LDP + tid] = (unsigned short)lo;
-        Yw[PLANE + (2 * mm + 1) * LDP + tid] = (unsigned short)(lo >> 16);
-        y.step(cg[mtop - mm]);
-        if ((mm & 7) == 7) y.rescale_nb();
-      }
-    }
+    if constexpr (GEN) s16_emit<MPS / 2>(y, cg, mtop, st * MPS, Yw, tid);
   }
   if constexpr (MF) {
 #pragma unroll
@@ -326,3 +362,4 @@ __global__ __launch_bounds__(256, 2) void sfb_direct_split16_kernel(PointsSoA pt
 }
 
 }  // namespace sfb
+

@@ -90,7 +90,8 @@ enum {
   SFBHIP_MIXED_KERNEL_512 = 2,   /* 512-thread blocks, items of up to 128 rows, cubic Bessel
                                     coefficients in LDS, double-buffered Y* halves */
   SFBHIP_MIXED_KERNEL_GROUPED = 3 /* 256-thread blocks, 64-row items, one cubic per (row, 8-point
-                                    group) for the Bessel values, two-operation Legendre steps */
+                                    group) for the Bessel values, two-operation Legendre steps,
+                                    point-major Y* chunks read by transposed LDS loads */
 };
 
 typedef struct sfbhip_handle sfbhip_handle;
