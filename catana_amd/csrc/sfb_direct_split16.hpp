@@ -27,10 +27,12 @@
 //     chunk is staged point-major (64-bit stores) and read by transposed LDS
 //     loads.
 // Measured and not kept: generating chunk q+1 into a second buffer while
-// contracting chunk q in the same wave (11 % slower), and a 512-thread form in
-// which four waves generate Y* while four contract (1.5x slower): the two
-// 256-thread blocks per CU already overlap one block's vector work with the
-// other's matrix-core work.
+// contracting chunk q in the same wave (11 % slower), a 512-thread form in
+// which four waves generate Y* while four contract (1.5x slower), and
+// 512-thread blocks for items of 65-128 rows, one Y* generation shared by all
+// eight waves (2^25 points: 3.52 against 3.24 s): the two 256-thread blocks
+// per CU already overlap one block's vector work with the other's
+// matrix-core work, where a single block's waves meet at every barrier.
 // Work item: (l, 64 rows of n, m chunks [qlo, qhi), point range), as for the
 // 256-thread kernel of sfb_direct_mixed.hpp, including the point split KS for
 // items of at most 32 (16) rows.
