@@ -805,6 +805,44 @@ sfbhip_status launch_plan_mixed(sfbhip_handle *h, const DirectPlan &pl, const st
 // downward recurrence P^{m-1} = -(a_m cot P^m + b_m P^{m+1}), d_l = d_{l-1} = 1,
 // d_{m-1} = b_m d_{m+1}, all d divided by their minimum (so |Q| <= |P|), and
 // g_m = -a_m d_m / d_{m-1}.
+// Host arrays of LegendreScaled for l < lmax: sect per l, g and d at off[l] + m.
+void legendre_scaled_host(int lmax, std::vector<double2> &hs, std::vector<double> &hg, std::vector<double> &hd,
+                          std::vector<int32_t> &ho) {
+  const size_t ncoef = (size_t)lmax * (lmax + 1) / 2 + 1;
+  hs.assign(std::max(lmax, 1), make_double2(0.0, 0.0));
+  hg.assign(ncoef, 0.0);
+  hd.assign(ncoef, 1.0);
+  ho.assign(std::max(lmax, 1), 0);
+  std::vector<double> a, b, dd;
+  double prod = 1.0;
+  int32_t o = 0;
+  for (int l = 0; l < lmax; ++l) {
+    if (l > 0) prod *= (2.0 * l - 1.0) / (2.0 * l);
+    double c = std::sqrt((2.0 * l + 1.0) / (4.0 * kPi) * prod);
+    if (l & 1) c = -c;
+    a.assign(l + 2, 0.0);
+    b.assign(l + 2, 0.0);
+    dd.assign(l + 2, 1.0);
+    for (int m = 1; m <= l; ++m) {
+      const double den = std::sqrt((double)(l + m) * (double)(l - m + 1));
+      a[m] = 2.0 * m / den;
+      b[m] = std::sqrt((double)(l - m) * (double)(l + m + 1)) / den;
+    }
+    for (int m = l - 1; m >= 1; --m) dd[m - 1] = b[m] * dd[m + 1];
+    double dmin = 1.0;
+    for (int m = 0; m <= l; ++m) dmin = std::min(dmin, dd[m]);
+    for (int m = 0; m <= l; ++m) dd[m] /= dmin;
+    ho[l] = o;
+    hg[o] = 0.0;
+    for (int m = 1; m <= l; ++m) hg[o + m] = -a[m] * dd[m] / dd[m - 1];
+    for (int m = 0; m <= l; ++m) hd[o + m] = dd[m];
+    int e;
+    const double mnt = std::frexp(c / dd[l], &e);
+    hs[l] = make_double2(mnt, (double)e);
+    o += l + 1;
+  }
+}
+
 sfbhip_status ensure_legendre_scaled(sfbhip_handle *h, int lmax, hipStream_t st, LegendreScaled *ls) {
   double2 *sect;
   double *g, *d;
@@ -815,36 +853,10 @@ sfbhip_status ensure_legendre_scaled(sfbhip_handle *h, int lmax, hipStream_t st,
   CHK(get_buf(h, "legs_d", ncoef, &d));
   CHK(get_buf(h, "legs_off", (size_t)std::max(lmax, 1), &off));
   if (h->legs_lmax < lmax) {
-    std::vector<double2> hs(std::max(lmax, 1));
-    std::vector<double> hg(ncoef, 0.0), hd(ncoef, 1.0), a, b, dd;
-    std::vector<int32_t> ho(std::max(lmax, 1));
-    double prod = 1.0;
-    int32_t o = 0;
-    for (int l = 0; l < lmax; ++l) {
-      if (l > 0) prod *= (2.0 * l - 1.0) / (2.0 * l);
-      double c = std::sqrt((2.0 * l + 1.0) / (4.0 * kPi) * prod);
-      if (l & 1) c = -c;
-      a.assign(l + 2, 0.0);
-      b.assign(l + 2, 0.0);
-      dd.assign(l + 2, 1.0);
-      for (int m = 1; m <= l; ++m) {
-        const double den = std::sqrt((double)(l + m) * (double)(l - m + 1));
-        a[m] = 2.0 * m / den;
-        b[m] = std::sqrt((double)(l - m) * (double)(l + m + 1)) / den;
-      }
-      for (int m = l - 1; m >= 1; --m) dd[m - 1] = b[m] * dd[m + 1];
-      double dmin = 1.0;
-      for (int m = 0; m <= l; ++m) dmin = std::min(dmin, dd[m]);
-      for (int m = 0; m <= l; ++m) dd[m] /= dmin;
-      ho[l] = o;
-      hg[o] = 0.0;
-      for (int m = 1; m <= l; ++m) hg[o + m] = -a[m] * dd[m] / dd[m - 1];
-      for (int m = 0; m <= l; ++m) hd[o + m] = dd[m];
-      int e;
-      const double mnt = std::frexp(c / dd[l], &e);
-      hs[l] = make_double2(mnt, (double)e);
-      o += l + 1;
-    }
+    std::vector<double2> hs;
+    std::vector<double> hg, hd;
+    std::vector<int32_t> ho;
+    legendre_scaled_host(lmax, hs, hg, hd, ho);
     HIPCHK(hipMemcpyAsync(sect, hs.data(), sizeof(double2) * hs.size(), hipMemcpyHostToDevice, st));
     HIPCHK(hipMemcpyAsync(g, hg.data(), sizeof(double) * hg.size(), hipMemcpyHostToDevice, st));
     HIPCHK(hipMemcpyAsync(d, hd.data(), sizeof(double) * hd.size(), hipMemcpyHostToDevice, st));
@@ -2502,9 +2514,9 @@ struct sfbhip_batch_plan {
   int32_t *dnl = nullptr;
   int64_t *dfoff = nullptr;
   LegendreTables lt{};
-  HermiteTables ht{};
+  LegendreScaled ls{};
+  CubicTables ct{};
   BesselTableRef q5{};
-  int lds_nodes = 0;
   DirectItem *ditems = nullptr;
   int nlow = 0;
   int n64[5] = {0, 0, 0, 0, 0};
@@ -2582,9 +2594,9 @@ void plan_grouped(const std::vector<int32_t> &nl, int lmin, int lmax, int64_t np
       groups->push_back(g);
       int32_t ks = 0;
       if (kscap >= 4 && g.nrows <= 16)
-        ks = ITEM_KS4;
+        ks = ITEM_SPLIT_KS4;
       else if (kscap >= 2 && g.nrows <= 32)
-        ks = ITEM_KS2;
+        ks = ITEM_SPLIT_KS2;
       const int32_t tab = (int32_t)std::min<double>(
           std::floor(cx.kln[(size_t)l * cx.nmax + n0 + g.nrows - 1] * cx.R / MX_H) + 2, 1e9);
       for (int64_t s = 0; s < S; ++s) {
@@ -2597,7 +2609,7 @@ void plan_grouped(const std::vector<int32_t> &nl, int lmin, int lmax, int64_t np
           it.qhi = std::min(Q, qlo + qcap);
           it.cat = cx.cat;
           it.tab = tab;
-          it.flags = (tab < (1 << 15) ? 0 : ITEM_FULL_WINDOW) | ks;
+          it.flags = ks;
           it.p0 = cx.pbase + s * chunk;
           it.p1 = cx.pbase + std::min(npts, (s + 1) * chunk);
           it.out = g.base + s * g.stride;
@@ -2662,8 +2674,8 @@ sfbhip_status plan_enqueue(sfbhip_batch_plan *pl, hipStream_t st, bool timed) {
   }
   if (timed) HIPCHK(hipEventRecord(pl->ev[2], st));
   if (pl->nmx) {
-    sfb_direct_mixed_grouped_kernel<<<pl->grid_mx, 256, sizeof(float2) * (size_t)pl->lds_nodes, st>>>(
-        pts, pl->ditems + off, pl->nmx, pl->derr + 4, pl->dk, pl->lt, pl->ht, pl->lds_nodes, pl->partial);
+    sfb_direct_split16_grouped_kernel<<<pl->grid_mx, 256, 0, st>>>(pts, pl->ditems + off, pl->nmx, pl->derr + 4,
+                                                                   pl->dk, pl->ls, pl->ct, pl->partial);
     HIPCHK(hipGetLastError());
   }
   if (timed) HIPCHK(hipEventRecord(pl->ev[3], st));
@@ -2691,7 +2703,7 @@ sfbhip_status plan_create(sfbhip_handle *h, const sfbhip_catalog *cats, int32_t 
     return fail(SFBHIP_NOT_SUPPORTED,
                 "batch plans run precision MIXED with SPLIT_FP16 operands; use sfbhip_direct_batched otherwise");
   if (p->mixed_kernel == SFBHIP_MIXED_KERNEL_512)
-    return fail(SFBHIP_NOT_SUPPORTED, "batch plans run the 256-thread mixed kernel");
+    return fail(SFBHIP_NOT_SUPPORTED, "batch plans run the 256-thread split-FP16 kernel");
   std::vector<sfbhip_params> cp(ncat, *p);
   std::vector<double> R(ncat);
   int64_t ntot = 0;
@@ -2809,28 +2821,42 @@ sfbhip_status plan_create(sfbhip_handle *h, const sfbhip_catalog *cats, int32_t 
     pl->q5.q5_nint = dni;
   }
   if (lsplit < L) {
-    std::vector<int32_t> hoff(L, 0), hni(L, 1);
-    int64_t ho = 0;
+    // the split-FP16 kernel's tables, owned by the plan: cubic j_l coefficients
+    // over each l's largest argument and the scaled Legendre constants
+    std::vector<int32_t> coff(L, 0), cni(L, 1);
+    int64_t co = 0;
     int32_t mx = 1;
     for (int l = 0; l < L; ++l) {
       const int64_t k = (int64_t)std::floor(xmax[l] / MX_H) + 2;
       if (k > MX_TAB_MAX)
         return fail(SFBHIP_NOT_SUPPORTED, "mixed path: largest Bessel argument k r exceeds " +
                                               std::to_string(MX_TAB_MAX * MX_H) + "; use precision FP64");
-      hni[l] = (int32_t)k;
-      hoff[l] = (int32_t)ho;
-      ho += k;
+      cni[l] = (int32_t)k;
+      coff[l] = (int32_t)co;
+      co += k;
       mx = std::max<int32_t>(mx, (int32_t)k);
-      if (ho > INT32_MAX) return fail(SFBHIP_NOT_SUPPORTED, "mixed path: Bessel tables too large");
+      if (co > INT32_MAX) return fail(SFBHIP_NOT_SUPPORTED, "mixed path: Bessel tables too large");
     }
     float4 *tb;
     int32_t *doff, *dni;
-    CHK(plan_alloc(pl, (size_t)ho, &tb));
-    CHK(plan_upload(pl, hoff, &doff));
-    CHK(plan_upload(pl, hni, &dni));
-    hermite_table_kernel<<<dim3((unsigned)cdiv(mx, 256), (unsigned)L), 256, 0, nullptr>>>(L, doff, dni, tb);
+    CHK(plan_alloc(pl, (size_t)co, &tb));
+    CHK(plan_upload(pl, coff, &doff));
+    CHK(plan_upload(pl, cni, &dni));
+    cubic_table_kernel<<<dim3((unsigned)cdiv(mx, 256), (unsigned)L), 256, 0, nullptr>>>(L, doff, dni, tb);
     HIPCHK(hipGetLastError());
-    pl->ht = HermiteTables{tb, doff, dni};
+    pl->ct = CubicTables{tb, doff, dni};
+    std::vector<double2> hs;
+    std::vector<double> hg, hd;
+    std::vector<int32_t> ho;
+    legendre_scaled_host(L, hs, hg, hd, ho);
+    double2 *ds2;
+    double *dg, *dd;
+    int32_t *dso;
+    CHK(plan_upload(pl, hs, &ds2));
+    CHK(plan_upload(pl, hg, &dg));
+    CHK(plan_upload(pl, hd, &dd));
+    CHK(plan_upload(pl, ho, &dso));
+    pl->ls = LegendreScaled{ds2, dg, dd, dso};
   }
 
   // work items: FP64 low l, FP64 l < lsplit beyond the low-l kernel's range, grouped mixed
@@ -2919,13 +2945,7 @@ sfbhip_status plan_create(sfbhip_handle *h, const sfbhip_catalog *cats, int32_t 
     CHK(plan_alloc(pl, (size_t)hb, &pl->hist));
     pl->sort_nb = nbmax;
   }
-  int nodes = 0;
-  for (const DirectItem &it : mx) nodes = std::max(nodes, std::min(it.tab + 2, MX_LDS_NODES));
-  pl->lds_nodes = nodes;
   pl->grid_mx = (int)std::max<int64_t>(1, std::min<int64_t>(pl->nmx, 2LL * h->ncu));
-  // the grouped kernel's dynamic LDS limit is set to the largest any plan uses
-  HIPCHK(hipFuncSetAttribute(reinterpret_cast<const void *>(&sfb_direct_mixed_grouped_kernel),
-                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)(sizeof(float2) * MX_LDS_NODES)));
   // catalogue descriptors
   std::vector<CatPrep> prep(ncat);
   std::vector<CatOut> outs(ncat);

@@ -532,48 +532,4 @@ __global__ __launch_bounds__(256) void sfb_direct_mixed_kernel(
   mixed_item<QMAX, NP, KS>(it, pts, kln, leg, ht, lds_nodes, partial, ItemShared{ybuf, rs, cfs, TN});
 }
 
-// DirectItem::flags of the grouped kernel's items: the point split KS
-enum : int32_t { ITEM_KS2 = 2, ITEM_KS4 = 4 };
-
-// Persistent grouped launch of the split-FP16 contraction: every block pulls
-// items from *counter (zeroed before the launch) until the list is exhausted,
-// and runs each with the class its m-chunk count (1, 2 or up to 4) and point
-// split (flags ITEM_KS2 / ITEM_KS4) select, so that items of every class and
-// catalogue share one launch with no tail per class.  Every block leaves the
-// loop once the counter passes nitems.
-__global__ __launch_bounds__(256, 2) void sfb_direct_mixed_grouped_kernel(
-    PointsSoA pts, const DirectItem *__restrict__ items, int nitems, int *__restrict__ counter,
-    const double *__restrict__ kln, LegendreTables leg, HermiteTables ht, int lds_nodes,
-    double *__restrict__ partial) {
-  __shared__ __attribute__((aligned(16))) char ybuf[MixedLds<2>::YBYTES];
-  __shared__ double rs[MX_TP];
-  __shared__ double2 cfs[MX_MC + MX_LMAX];
-  __shared__ int next;
-  extern __shared__ float2 TN[];
-  const ItemShared sh{ybuf, rs, cfs, TN};
-  for (;;) {
-    if (threadIdx.x == 0) next = atomicAdd(counter, 1);
-    __syncthreads();  // next written; the previous item's LDS readers are done
-    const int i = __builtin_amdgcn_readfirstlane(next);
-    __syncthreads();  // every thread has read next before thread 0 overwrites it
-    if (i >= nitems) break;
-    const DirectItem it = items[i];
-    const int nq = it.qhi - it.qlo;
-    const int ks = it.flags & (ITEM_KS2 | ITEM_KS4);
-    if (ks == ITEM_KS4) {
-      if (nq == 1) mixed_item<1, 2, 4>(it, pts, kln, leg, ht, lds_nodes, partial, sh);
-      else if (nq == 2) mixed_item<2, 2, 4>(it, pts, kln, leg, ht, lds_nodes, partial, sh);
-      else mixed_item<4, 2, 4>(it, pts, kln, leg, ht, lds_nodes, partial, sh);
-    } else if (ks == ITEM_KS2) {
-      if (nq == 1) mixed_item<1, 2, 2>(it, pts, kln, leg, ht, lds_nodes, partial, sh);
-      else if (nq == 2) mixed_item<2, 2, 2>(it, pts, kln, leg, ht, lds_nodes, partial, sh);
-      else mixed_item<4, 2, 2>(it, pts, kln, leg, ht, lds_nodes, partial, sh);
-    } else {
-      if (nq == 1) mixed_item<1, 2, 1>(it, pts, kln, leg, ht, lds_nodes, partial, sh);
-      else if (nq == 2) mixed_item<2, 2, 1>(it, pts, kln, leg, ht, lds_nodes, partial, sh);
-      else mixed_item<4, 2, 1>(it, pts, kln, leg, ht, lds_nodes, partial, sh);
-    }
-  }
-}
-
 }  // namespace sfb

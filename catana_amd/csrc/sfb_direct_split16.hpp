@@ -365,3 +365,51 @@ __global__ __launch_bounds__(256, 2) void sfb_direct_split16_kernel(PointsSoA pt
 
 }  // namespace sfb
 
+
+namespace sfb {
+
+// DirectItem::flags of the grouped kernel's items: the point split KS
+enum : int32_t { ITEM_SPLIT_KS2 = 2, ITEM_SPLIT_KS4 = 4 };
+
+// Persistent grouped launch of split16_item: every block pulls items from
+// *counter (zeroed before the launch) until the list is exhausted and runs
+// each with the class its m-chunk count (1, 2 or up to 4) and point split
+// (flags ITEM_SPLIT_KS2 / ITEM_SPLIT_KS4) select, so that items of every class
+// and catalogue share one launch with no tail per class.  Every block leaves
+// the loop once the counter passes nitems.
+__global__ __launch_bounds__(256, 2) void sfb_direct_split16_grouped_kernel(
+    PointsSoA pts, const DirectItem *__restrict__ items, int nitems, int *__restrict__ counter,
+    const double *__restrict__ kln, LegendreScaled leg, CubicTables ct, double *__restrict__ partial) {
+  __shared__ __attribute__((aligned(16))) unsigned short ybuf[S16_YBUF];
+  __shared__ double rs[MX_TP];
+  __shared__ __attribute__((aligned(16))) float dg[MX_TP];
+  __shared__ float gs[MX_TP / 8];
+  __shared__ double cgs[S16_GPAD + MX_LMAX];
+  __shared__ int next;
+  const S16Shared sh{ybuf, rs, dg, gs, cgs};
+  for (;;) {
+    if (threadIdx.x == 0) next = atomicAdd(counter, 1);
+    __syncthreads();  // next written; the previous item's LDS readers are done
+    const int i = __builtin_amdgcn_readfirstlane(next);
+    __syncthreads();  // every thread has read next before thread 0 overwrites it
+    if (i >= nitems) break;
+    const DirectItem it = items[i];
+    const int nq = it.qhi - it.qlo;
+    const int ks = it.flags & (ITEM_SPLIT_KS2 | ITEM_SPLIT_KS4);
+    if (ks == ITEM_SPLIT_KS4) {
+      if (nq == 1) split16_item<1, 4>(it, pts, kln, leg, ct, partial, sh);
+      else if (nq == 2) split16_item<2, 4>(it, pts, kln, leg, ct, partial, sh);
+      else split16_item<4, 4>(it, pts, kln, leg, ct, partial, sh);
+    } else if (ks == ITEM_SPLIT_KS2) {
+      if (nq == 1) split16_item<1, 2>(it, pts, kln, leg, ct, partial, sh);
+      else if (nq == 2) split16_item<2, 2>(it, pts, kln, leg, ct, partial, sh);
+      else split16_item<4, 2>(it, pts, kln, leg, ct, partial, sh);
+    } else {
+      if (nq == 1) split16_item<1, 1>(it, pts, kln, leg, ct, partial, sh);
+      else if (nq == 2) split16_item<2, 1>(it, pts, kln, leg, ct, partial, sh);
+      else split16_item<4, 1>(it, pts, kln, leg, ct, partial, sh);
+    }
+  }
+}
+
+}  // namespace sfb
