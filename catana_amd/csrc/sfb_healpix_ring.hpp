@@ -25,6 +25,7 @@
 #include "sfb_common.hpp"
 #include "sfb_direct_mixed.hpp"
 #include "sfb_direct_mixed2.hpp"
+#include "sfb_direct_split16.hpp"
 #include "sfb_healpix.hpp"
 #include "sfb_ylm.hpp"
 
@@ -112,6 +113,11 @@ __device__ __forceinline__ float cubic_at(const float4 *C, double t) {
   return fmaf(fmaf(fmaf(c.w, f, c.z), f, c.y), f, c.x);
 }
 
+// Sixteen lanes per pixel, lane j evaluating row j: a wave's lanes then loop
+// over the radii of 4 pixels (counts ~ Poisson, a wave waits on the fullest of
+// 4) instead of 64 (one pixel per lane: 12.1 ms of map time at nside 256).  A
+// block takes 64 consecutive pixels per pass, 16 at a time, stages the 16 x 64
+// sums in LDS and writes each row's 64 pixels contiguously.
 template <class F>
 __device__ __forceinline__ void map_pixels(const MapItem &it, const int64_t *__restrict__ off,
                                            const double *__restrict__ radii, int64_t npix,
@@ -119,7 +125,9 @@ __device__ __forceinline__ void map_pixels(const MapItem &it, const int64_t *__r
                                            F jl) {
   // one pixel per lane, in pixel order: a wave's radius reads and map writes
   // stay contiguous (ordering each tile's pixels by point count, so that a
-  // wave's lanes loop over equally many radii, measured slower)
+  // wave's lanes loop over equally many radii, measured slower; so did sixteen
+  // lanes per pixel, one row each, with the rows staged in LDS for the writes:
+  // 12.1 -> 23 ms at nside 256, one evaluation per radius load instead of 16)
   for (int64_t p = it.p0 + threadIdx.x; p < it.p1; p += blockDim.x) {
     // FP64 pixel sums (a pixel may hold hundreds of radii whose j_l values
     // cancel), rounded to FP32 once
@@ -177,49 +185,50 @@ struct RingPixels {
 };
 
 // Y* 2^12 of one pixel for the 16 m of a chunk (from mtop down) into the
-// chunk buffer: two FP16 planes of 32 columns (2 mm: real, 2 mm + 1:
-// imaginary) x LDP pixels.  Leaves the azimuth 16 steps lower.  Columns of
-// m < 0 and padding pixels are zero.
+// chunk buffer: two point-major FP16 planes (row tid: 32 columns, 2 mm: real,
+// 2 mm + 1: imaginary; the layout of sfb_direct_split16.hpp), stored as one
+// 64-bit word per plane and two m.  Leaves the azimuth 16 steps lower.
+// Columns of m < 0 and padding pixels are zero.
 // Pq[sgn * mm] is P_l^m at m = mtop - mm (sgn = 1: the LDS copy, already zero
 // for m < 0; sgn = -1: the global table, read at max(m, 0)).
-template <bool QUIRK, int LDP>
-__device__ __forceinline__ void ring_emit(_Float16 *Yh, int tid, int mtop, bool pok, const double *Pq, int sgn,
+template <bool QUIRK>
+__device__ __forceinline__ void ring_emit(unsigned short *Yw, int tid, int mtop, bool pok, const double *Pq, int sgn,
                                           int nr, int &smod, bool quirk, double &cm, double &sm, double &cp,
                                           double &sp, double c2, double &cm2, double &sm2, double cph2, double sph2) {
-  constexpr int PLANE = 2 * MX_MC * LDP;
 #pragma unroll
-  for (int mm = 0; mm < MX_MC; ++mm) {
-    const int m = mtop - mm;
-    // staged: index mm (the LDS copy holds zeros for m < 0, all inside it);
-    // global: index -(mm) = P at m, or P at m = 0 for m < 0 (masked below)
-    const double pr = sgn > 0 ? Pq[mm] : Pq[-(m >= 0 ? mm : mtop)];
-    const double pv = (pok && m >= 0) ? pr : 0.0;
-    double c = cm, sn = sm;
-    if constexpr (QUIRK) {
-      const bool flip = quirk && m > 0 && 2 * smod > nr;
-      c = flip ? cm2 : cm;
-      sn = flip ? sm2 : sm;
+  for (int mp = 0; mp < MX_MC; mp += 2) {
+    unsigned hi[2], lo[2];
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+      const int mm = mp + u;
+      const int m = mtop - mm;
+      // staged: index mm (the LDS copy holds zeros for m < 0, all inside it);
+      // global: index -(mm) = P at m, or P at m = 0 for m < 0 (masked below)
+      const double pr = sgn > 0 ? Pq[mm] : Pq[-(m >= 0 ? mm : mtop)];
+      const double pv = (pok && m >= 0) ? pr : 0.0;
+      double c = cm, sn = sm;
+      if constexpr (QUIRK) {
+        const bool flip = quirk && m > 0 && 2 * smod > nr;
+        c = flip ? cm2 : cm;
+        sn = flip ? sm2 : sm;
+      }
+      split2((float)(pv * c), (float)(-pv * sn), hi[u], lo[u]);
+      // m -> m - 1
+      const double cn = fma(c2, cm, -cp), snn = fma(c2, sm, -sp);
+      cp = cm;
+      sp = sm;
+      cm = cn;
+      sm = snn;
+      if constexpr (QUIRK) {
+        const double cn2 = fma(cm2, cph2, sm2 * sph2), sn2 = fma(sm2, cph2, -cm2 * sph2);
+        cm2 = cn2;
+        sm2 = sn2;
+        smod = smod == 0 ? nr - 1 : smod - 1;
+      }
     }
-    const float re = (float)(pv * c), im = (float)(-pv * sn);
-    unsigned hi, lo;
-    split2(re, im, hi, lo);
-    unsigned short *Yu = reinterpret_cast<unsigned short *>(Yh);
-    Yu[(2 * mm) * LDP + tid] = (unsigned short)hi;
-    Yu[(2 * mm + 1) * LDP + tid] = (unsigned short)(hi >> 16);
-    Yu[PLANE + (2 * mm) * LDP + tid] = (unsigned short)lo;
-    Yu[PLANE + (2 * mm + 1) * LDP + tid] = (unsigned short)(lo >> 16);
-    // m -> m - 1
-    const double cn = fma(c2, cm, -cp), snn = fma(c2, sm, -sp);
-    cp = cm;
-    sp = sm;
-    cm = cn;
-    sm = snn;
-    if constexpr (QUIRK) {
-      const double cn2 = fma(cm2, cph2, sm2 * sph2), sn2 = fma(sm2, cph2, -cm2 * sph2);
-      cm2 = cn2;
-      sm2 = sn2;
-      smod = smod == 0 ? nr - 1 : smod - 1;
-    }
+    const int e = tid * S16_ROW + 2 * mp;
+    *reinterpret_cast<uint2 *>(Yw + e) = make_uint2(hi[0], hi[1]);
+    *reinterpret_cast<uint2 *>(Yw + S16_PLANE + e) = make_uint2(lo[0], lo[1]);
   }
 }
 
@@ -232,11 +241,10 @@ __global__ __launch_bounds__(256, 2) void healpix_ring_contract_kernel(RingPixel
                                                                     const float *__restrict__ maps, int64_t npix,
                                                                     const int64_t *__restrict__ moff, double unscale,
                                                                     double *__restrict__ partial) {
-  constexpr int LDP = HR_TP + 8;
-  constexpr int PLANE = 2 * MX_MC * LDP;
   constexpr int FOLD = 64;
   constexpr int SPW = HR_TP / 32;
-  __shared__ __attribute__((aligned(16))) _Float16 Yh[2 * PLANE];
+  static_assert(HR_TP == MX_TP, "the point-major Y* planes hold MX_TP rows");
+  __shared__ __attribute__((aligned(16))) unsigned short Yh[S16_YBUF];
   // ring Legendre values of the tile's rings for the item's m range:
   // Ps[(rho - rho_lo) 16 QMAX + (mtop0 - m)]
   __shared__ double Ps[HR_RINGS * MX_MC * QMAX];
@@ -349,20 +357,18 @@ __global__ __launch_bounds__(256, 2) void healpix_ring_contract_kernel(RingPixel
         const double *Pq = staged ? Pl + MX_MC * qq : Pr + mtop;
         const int sgn = staged ? 1 : -1;
         if (wave_quirk)
-          ring_emit<true, LDP>(Yh, tid, mtop, pok, Pq, sgn, nr, smod, quirk, cm, sm, cp, sp, c2, cm2, sm2, cph2, sph2);
+          ring_emit<true>(Yh, tid, mtop, pok, Pq, sgn, nr, smod, quirk, cm, sm, cp, sp, c2, cm2, sm2, cph2, sph2);
         else
-          ring_emit<false, LDP>(Yh, tid, mtop, pok, Pq, sgn, nr, smod, quirk, cm, sm, cp, sp, c2, cm2, sm2, cph2, sph2);
+          ring_emit<false>(Yh, tid, mtop, pok, Pq, sgn, nr, smod, quirk, cm, sm, cp, sp, c2, cm2, sm2, cph2, sph2);
         __syncthreads();
         if (wave_rows) {
-          const int col = lane & 15;
           v4f a0 = v4f{0.f, 0.f, 0.f, 0.f}, a1 = a0, c0 = a0, c1 = a0;
 #pragma unroll
           for (int s = 0; s < SPW; ++s) {
-            const int o = col * LDP + 32 * s + 8 * g;
-            const h8 b00 = *reinterpret_cast<const h8 *>(Yh + o);
-            const h8 b01 = *reinterpret_cast<const h8 *>(Yh + o + 16 * LDP);
-            const h8 b10 = *reinterpret_cast<const h8 *>(Yh + PLANE + o);
-            const h8 b11 = *reinterpret_cast<const h8 *>(Yh + PLANE + o + 16 * LDP);
+            const h8 b00 = s16_bfrag(Yh, 32 * s, 0, lane);
+            const h8 b01 = s16_bfrag(Yh, 32 * s, 16, lane);
+            const h8 b10 = s16_bfrag(Yh + S16_PLANE, 32 * s, 0, lane);
+            const h8 b11 = s16_bfrag(Yh + S16_PLANE, 32 * s, 16, lane);
             c0 = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[0][s], b10, c0, 0, 0, 0);
             c0 = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[1][s], b00, c0, 0, 0, 0);
             c1 = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[0][s], b11, c1, 0, 0, 0);
