@@ -1,7 +1,7 @@
 // catana_api.cpp -- host C++ API of catana_api.hpp over the C-ABI.
 #include "catana_api.hpp"
 
-#include <hip/hip_runtime.h>
+#include <hip/hip_runtime_api.h>
 
 #include <algorithm>
 #include <atomic>
