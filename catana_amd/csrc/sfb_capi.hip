@@ -1252,6 +1252,12 @@ sfbhip_status healpix_ring(sfbhip_handle *h, const int64_t *d_offsets, const dou
   rp.ptab = ptab;
   rp.poff = dpoff;
   rp.alias_compat = p->alias_compat;
+  // slot order of the maps and the contraction: pixels by point count within each tile
+  int32_t *slot_perm;
+  CHK(get_buf(h, "hr_slot_perm", (size_t)std::max<int64_t>(npix, 1), &slot_perm));
+  ring_slot_order_kernel<<<(unsigned)cdiv(npix, HR_TP), HR_TP, 0, st>>>(d_offsets, npix, slot_perm);
+  HIPCHK(hipGetLastError());
+  rp.perm = slot_perm;
 
   // batches of multipoles whose maps fit the workspace
   std::vector<std::pair<int, int>> batches;
@@ -1369,8 +1375,8 @@ sfbhip_status healpix_ring(sfbhip_handle *h, const int64_t *d_offsets, const dou
     if (!mitems.empty()) {
       HIPCHK(hipFuncSetAttribute(reinterpret_cast<const void *>(&ring_maps_kernel),
                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
-      ring_maps_kernel<<<(unsigned)mitems.size(), 256, shm, st>>>(dmi, d_offsets, d_radii, npix, dk, ct, nodes, scale,
-                                                                 maps);
+      ring_maps_kernel<<<(unsigned)mitems.size(), 256, shm, st>>>(dmi, d_offsets, slot_perm, d_radii, npix, dk, ct,
+                                                                 nodes, scale, maps);
       HIPCHK(hipGetLastError());
     }
     if (batches.size() == 1) HIPCHK(hipEventRecord(h->ev[1], st));

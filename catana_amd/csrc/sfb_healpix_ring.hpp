@@ -73,6 +73,27 @@ __global__ void ring_setup_kernel(int nside, double *__restrict__ rx, double *__
   }
 }
 
+// Slot order of the maps and of the contraction: within each tile of HR_TP
+// consecutive pixels (the tiles of the map and contraction items), slot
+// HR_TP b + i holds the pixel of rank i by decreasing point count (ties in
+// pixel order).  A wave's 64 lanes then loop over about equally many radii in
+// the map kernel, and a tile still spans the same few rings.
+__global__ __launch_bounds__(256) void ring_slot_order_kernel(const int64_t *__restrict__ off, int64_t npix,
+                                                            int32_t *__restrict__ perm) {
+  __shared__ int64_t cnt[HR_TP];
+  const int64_t b0 = (int64_t)blockIdx.x * HR_TP;
+  const int n = (int)min<int64_t>(HR_TP, npix - b0);
+  const int t = threadIdx.x;
+  if (t < n) cnt[t] = off[b0 + t + 1] - off[b0 + t];
+  __syncthreads();
+  if (t < n) {
+    const int64_t c = cnt[t];
+    int rank = 0;
+    for (int k = 0; k < n; ++k) rank += (cnt[k] > c) || (cnt[k] == c && k < t);
+    perm[b0 + rank] = (int32_t)(b0 + t);
+  }
+}
+
 // ptab[poff[l] + rho (l + 1) + m] = P_l^m(cos theta_rho) 2^12 for m = 0..l (the
 // orthonormal functions with the Condon-Shortley phase, gsl_sf_legendre_sphPlm
 // as in sfb_reverse_fft.cpp:119), by the downward recurrence of YlmDown;
@@ -120,7 +141,8 @@ __device__ __forceinline__ float cubic_at(const float4 *C, double t) {
 // sums in LDS and writes each row's 64 pixels contiguously.
 template <class F>
 __device__ __forceinline__ void map_pixels(const MapItem &it, const int64_t *__restrict__ off,
-                                           const double *__restrict__ radii, int64_t npix,
+                                           const int32_t *__restrict__ perm, const double *__restrict__ radii,
+                                           int64_t npix,
                                            const double (&khr)[HR_NB], double scale, float *__restrict__ maps,
                                            F jl) {
   // one pixel per lane, in pixel order: a wave's radius reads and map writes
@@ -130,12 +152,13 @@ __device__ __forceinline__ void map_pixels(const MapItem &it, const int64_t *__r
   // 12.1 -> 23 ms at nside 256, one evaluation per radius load instead of 16)
   for (int64_t p = it.p0 + threadIdx.x; p < it.p1; p += blockDim.x) {
     // FP64 pixel sums (a pixel may hold hundreds of radii whose j_l values
-    // cancel), rounded to FP32 once
+    // cancel), rounded to FP32 once; slot p holds pixel perm[p]
     double acc[HR_NB];
 #pragma unroll
     for (int j = 0; j < HR_NB; ++j) acc[j] = 0.0;
-    const int64_t q1 = off[p + 1];
-    for (int64_t q = off[p]; q < q1; ++q) {
+    const int64_t px = perm[p];
+    const int64_t q1 = off[px + 1];
+    for (int64_t q = off[px]; q < q1; ++q) {
       const double r = radii[q];
 #pragma unroll
       for (int j = 0; j < HR_NB; ++j) acc[j] += (double)jl(r * khr[j]);
@@ -152,6 +175,7 @@ __device__ __forceinline__ void map_pixels(const MapItem &it, const int64_t *__r
 // k = 0 (t = 0) and are not written.
 __global__ __launch_bounds__(256) void ring_maps_kernel(const MapItem *__restrict__ items,
                                                         const int64_t *__restrict__ off,
+                                                        const int32_t *__restrict__ perm,
                                                         const double *__restrict__ radii, int64_t npix,
                                                         const double *__restrict__ kln, CubicTables ct,
                                                         int lds_nodes, double scale, float *__restrict__ maps) {
@@ -168,9 +192,9 @@ __global__ __launch_bounds__(256) void ring_maps_kernel(const MapItem *__restric
 #pragma unroll
   for (int j = 0; j < HR_NB; ++j) khr[j] = j < it.nrows ? kln[it.kofs + j] * (1.0 / MX_H) : 0.0;
   if (tl)
-    map_pixels(it, off, radii, npix, khr, scale, maps, [&](double t) { return cubic_at(TM, t); });
+    map_pixels(it, off, perm, radii, npix, khr, scale, maps, [&](double t) { return cubic_at(TM, t); });
   else
-    map_pixels(it, off, radii, npix, khr, scale, maps, [&](double t) { return cubic_at(T, t); });
+    map_pixels(it, off, perm, radii, npix, khr, scale, maps, [&](double t) { return cubic_at(T, t); });
 }
 
 // Per-pixel data of the ring contraction.
@@ -181,6 +205,7 @@ struct RingPixels {
   const int32_t *ring;                 // ring index (0-based) of the pixel
   const double *ptab;                  // ring Legendre table (ring_legendre_kernel)
   const int64_t *poff;
+  const int32_t *perm;                 // slot order (ring_slot_order_kernel): slot i holds pixel perm[i]
   int32_t alias_compat;
 };
 
@@ -318,11 +343,14 @@ __global__ __launch_bounds__(256, 2) void healpix_ring_contract_kernel(RingPixel
     double cm2 = 0.0, sm2 = 0.0, cph2 = 0.0, sph2 = 0.0;
     bool quirk = false;
     if (pok) {
-      const int rho = rp.ring[gp];
+      // slot gp holds pixel px (the maps are stored in slot order); the slots of
+      // a tile hold the tile's pixels, so its rings are those of [base, plast]
+      const int64_t px = rp.perm[gp];
+      const int rho = rp.ring[px];
       Pr = P + (int64_t)rho * (l + 1);
       if (staged) Pl = Ps + (rho - rho_lo) * MX_MC * QMAX;
-      nr = rp.np[gp];
-      const double phi = rp.phi[gp], cph = rp.cphi[gp], sph = rp.sphi[gp];
+      nr = rp.np[px];
+      const double phi = rp.phi[px], cph = rp.cphi[px], sph = rp.sphi[px];
       c2 = cph + cph;
       const double hi = (double)mtop0 * phi, lo = fma((double)mtop0, phi, -hi);
       double S, C;
@@ -334,13 +362,13 @@ __global__ __launch_bounds__(256, 2) void healpix_ring_contract_kernel(RingPixel
       // the mirrored azimuth only matters where some m <= l folds past n_ring / 2
       quirk = alias && 2 * l > nr;
       if (quirk) {
-        const double phi2 = rp.phi2[gp];
+        const double phi2 = rp.phi2[px];
         const double h2 = (double)mtop0 * phi2, l2 = fma((double)mtop0, phi2, -h2);
         sincos(h2, &S, &C);
         cm2 = C - l2 * S;
         sm2 = S + l2 * C;
-        cph2 = rp.cphi2[gp];
-        sph2 = rp.sphi2[gp];
+        cph2 = rp.cphi2[px];
+        sph2 = rp.sphi2[px];
       }
       smod = mtop0 % nr;
     }
