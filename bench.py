@@ -65,11 +65,13 @@ def make_batch(seed=0, ncat=64):
     return cats
 
 
-# HBM bytes per call of the mixed contraction kernels (all launches of one call), from
-# rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE on this build, scaled by what the counters read
-# for a 512 MiB device-to-device copy (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0).  Counters cannot
-# be read from inside the timed run, so the figure is carried here per workload.
-PROFILED_TRAFFIC_BYTES = {("clustered1m", "mixed", "split16", 2, 4): 3.551e9,  # (workload, precision, operands,
+# HBM bytes per call of the dominant kernels (all launches of one call), measured with
+# rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE and scaled by what the counters read for a 512 MiB
+# device-to-device copy (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0).  Counters cannot be read from
+# inside the timed run, so the figure is carried here per workload: (workload, precision,
+# operands, fp64_below, chunks) for the point path, (workload, precision) for the others.
+PROFILED_TRAFFIC_BYTES = {("clustered1m", "mixed", "split16", 2, 4): 3.551e9,  # split16 classes
+                          ("healpix4m", "mixed"): 3.971e10,                    # maps + ring contraction
                           ("ingest2p25",): 1.337e9}                            #  fp64_below, chunks)
 
 
@@ -88,6 +90,17 @@ def make_points(name):
 CPU_SAMPLE = {"clustered1m": 1 << 17, "lognormal2p25": 4096, "uniform10k": 10000, "batched64": 1 << 15}
 CPU_RUNS = 3
 KERNELS = {"auto": 0, "256": 1, "512": 2, "grouped": 3}
+
+
+def contraction_kernel(prec_fp64, operands, kernel):
+    """Name of the kernel that runs the point-mode contraction for these options."""
+    if prec_fp64:
+        return "sfb_direct_f64_kernel"
+    if operands == "split16" and kernel in ("auto", "grouped"):
+        return "sfb_direct_split16_kernel"
+    if operands == "split16" and kernel == "512":
+        return "sfb_direct_mixed2_kernel"
+    return "sfb_direct_mixed_kernel"
 
 
 def log(msg):
@@ -265,7 +278,7 @@ def main():
                      "frac": achieved / peak,
                      "traffic": PROFILED_TRAFFIC_BYTES.get((args.workload, args.precision, args.operands, args.fp64_below,
                                                               args.chunks)),
-                     "kernel": "sfb_direct_%s_kernel" % ("f64" if prec == capi.PREC_FP64 else "mixed"),
+                     "kernel": contraction_kernel(prec == capi.PREC_FP64, args.operands, args.kernel),
                      "kernel_ms": kms, "algorithmic_flops_per_launch": flops_dom,
                      "l_range": [lo, cfg["lmax"]],
                      "phases_ms": {"fp64_contraction": float(ph[0]), "mixed_contraction": float(ph[1]),
@@ -601,7 +614,7 @@ def run_batched(args):
         ms = e0.elapsed_time(e1) / args.steps
         kms = float(np.mean(kern))
         ph = np.mean(np.array(phases), axis=0)
-        kname = "sfb_direct_%s_kernel" % ("f64" if prec == capi.PREC_FP64 else "mixed")
+        kname = contraction_kernel(prec == capi.PREC_FP64, args.operands, args.kernel)
     else:
         pkw = dict(mixed_fp64_below=args.fp64_below, zeros=kw.get("zeros", capi.ZEROS_REFERENCE))
         if args.autotune:
@@ -639,7 +652,7 @@ def run_batched(args):
         ph = np.mean(np.array(phs), axis=0)
         kms = float(ph[1])
         res = plan.outputs
-        kname = "sfb_direct_mixed_grouped_kernel"
+        kname = "sfb_direct_split16_grouped_kernel"
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         capi.sfb_direct_batched_torch(h, cats, **kw)

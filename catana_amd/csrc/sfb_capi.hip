@@ -1086,7 +1086,7 @@ int mixed2_qcap(const sfbhip_params *p) { return std::min(p->mixed_chunks ? p->m
 // The split-FP16 kernel with grouped Bessel evaluation (sfb_direct_split16.hpp)
 // runs the mixed path's contraction by default (AUTO): measured on MI355X
 // against the 256-thread kernel of sfb_direct_mixed.hpp, 2^25 points, lmax 256:
-// 3.52 s against 4.32 s; 1M clustered points: 5.60 against 6.68 ms.
+// 3.20 s against 4.29 s for the mixed phase; 1M clustered points: 4.6 against 5.6 ms.
 bool use_split16(const sfbhip_params *p) {
   return p->precision == SFBHIP_PREC_MIXED && p->mixed_operands == SFBHIP_OPERANDS_SPLIT_FP16 &&
          (p->mixed_kernel == SFBHIP_MIXED_KERNEL_GROUPED || p->mixed_kernel == SFBHIP_MIXED_KERNEL_AUTO);
@@ -2709,7 +2709,7 @@ sfbhip_status plan_create(sfbhip_handle *h, const sfbhip_catalog *cats, int32_t 
     return fail(SFBHIP_NOT_SUPPORTED,
                 "batch plans run precision MIXED with SPLIT_FP16 operands; use sfbhip_direct_batched otherwise");
   if (p->mixed_kernel == SFBHIP_MIXED_KERNEL_512)
-    return fail(SFBHIP_NOT_SUPPORTED, "batch plans run the 256-thread split-FP16 kernel");
+    return fail(SFBHIP_NOT_SUPPORTED, "batch plans run the grouped split-FP16 kernel (256 threads); MIXED_KERNEL_512 is not one");
   std::vector<sfbhip_params> cp(ncat, *p);
   std::vector<double> R(ncat);
   int64_t ntot = 0;

@@ -363,11 +363,6 @@ __global__ __launch_bounds__(256, 2) void sfb_direct_split16_kernel(PointsSoA pt
   split16_item<QMAX, KS>(it, pts, kln, leg, ct, partial, S16Shared{ybuf, rs, dg, gs, cgs});
 }
 
-}  // namespace sfb
-
-
-namespace sfb {
-
 // DirectItem::flags of the grouped kernel's items: the point split KS
 enum : int32_t { ITEM_SPLIT_KS2 = 2, ITEM_SPLIT_KS4 = 4 };
 

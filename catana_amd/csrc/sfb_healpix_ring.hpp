@@ -134,22 +134,20 @@ __device__ __forceinline__ float cubic_at(const float4 *C, double t) {
   return fmaf(fmaf(fmaf(c.w, f, c.z), f, c.y), f, c.x);
 }
 
-// Sixteen lanes per pixel, lane j evaluating row j: a wave's lanes then loop
-// over the radii of 4 pixels (counts ~ Poisson, a wave waits on the fullest of
-// 4) instead of 64 (one pixel per lane: 12.1 ms of map time at nside 256).  A
-// block takes 64 consecutive pixels per pass, 16 at a time, stages the 16 x 64
-// sums in LDS and writes each row's 64 pixels contiguously.
+// The map values of one item: one slot (pixel) per lane, each lane looping
+// over its pixel's radii and evaluating the item's rows for each.
 template <class F>
 __device__ __forceinline__ void map_pixels(const MapItem &it, const int64_t *__restrict__ off,
                                            const int32_t *__restrict__ perm, const double *__restrict__ radii,
                                            int64_t npix,
                                            const double (&khr)[HR_NB], double scale, float *__restrict__ maps,
                                            F jl) {
-  // one pixel per lane, in pixel order: a wave's radius reads and map writes
-  // stay contiguous (ordering each tile's pixels by point count, so that a
-  // wave's lanes loop over equally many radii, measured slower; so did sixteen
-  // lanes per pixel, one row each, with the rows staged in LDS for the writes:
-  // 12.1 -> 23 ms at nside 256, one evaluation per radius load instead of 16)
+  // one slot per lane: the slots of a 256-pixel tile hold its pixels by
+  // decreasing point count, so a wave's lanes loop over about equally many
+  // radii (12.0 -> 11.0 ms of maps at nside 256 against pixel order), and the
+  // map writes stay contiguous.  Measured slower: sixteen lanes per pixel, one
+  // row each, with the rows staged in LDS for the writes (12.1 -> 23 ms, one
+  // evaluation per radius load instead of 16).
   for (int64_t p = it.p0 + threadIdx.x; p < it.p1; p += blockDim.x) {
     // FP64 pixel sums (a pixel may hold hundreds of radii whose j_l values
     // cancel), rounded to FP32 once; slot p holds pixel perm[p]

@@ -164,8 +164,10 @@ sfbhip_status sfbhip_direct_batched(sfbhip_handle *h, const sfbhip_catalog *cats
  * replayed whenever the points in the bound buffers change.  The contraction
  * is one persistent grouped launch: every block pulls work items of any
  * catalogue, multipole and shape from a device counter, in order of
- * decreasing cost.  Points stay in input order (no radius sort); each
- * catalogue's Bessel tables cover r <= r_bound (default: its rmax).
+ * decreasing cost.  Each enqueue puts every catalogue's points in radius
+ * order on the device (a 16-bit radius key, two 8-bit counting-sort passes,
+ * one gather); each catalogue's Bessel tables cover r <= r_bound (default:
+ * its rmax).
  * Precision MIXED with SPLIT_FP16 operands only (the FP64 and other operand
  * formats run through sfbhip_direct_batched).  No counterpart in the
  * reference, where a caller loops over catana::sfb_decomposition
