@@ -356,8 +356,12 @@ struct PlanCtx {
 
 // ks_cap > 1 (mixed kernel, 64-row items): an item of at most 16 rows goes to
 // point split KS = min(4, ks_cap), of at most 32 rows to KS = min(2, ks_cap).
+// cuts (optional): segment boundaries 0 = c_0 < c_1 < ... = npts, multiples of
+// tp; no point range of an item then crosses one (each segment gets its share
+// of the ranges).
 void plan_direct(const std::vector<int32_t> &nl, int lmin, int lmax, int64_t npts, int nt, int tp, int mc,
-                 int qmax_cap, int64_t target_items, const PlanCtx &cx, DirectPlan *pl, int ks_cap = 1) {
+                 int qmax_cap, int64_t target_items, const PlanCtx &cx, DirectPlan *pl, int ks_cap = 1,
+                 const std::vector<int64_t> *cuts = nullptr) {
   int64_t ngroups = 0, passes_total = 0;
   for (int l = lmin; l < lmax; ++l) {
     const int64_t g = cdiv(nl[l], nt);
@@ -365,10 +369,23 @@ void plan_direct(const std::vector<int32_t> &nl, int lmin, int lmax, int64_t npt
     passes_total += g * cdiv(l / mc + 1, qmax_cap);
   }
   if (ngroups == 0) return;
-  int64_t S = std::max<int64_t>(1, cdiv(target_items, passes_total));
-  S = std::min<int64_t>(S, cdiv(npts, tp));
-  const int64_t chunk = cdiv(cdiv(npts, S), tp) * tp;
-  S = cdiv(npts, chunk);
+  int64_t S0 = std::max<int64_t>(1, cdiv(target_items, passes_total));
+  S0 = std::min<int64_t>(S0, cdiv(npts, tp));
+  // point ranges [r0[s], r1[s]) of the splits
+  std::vector<int64_t> r0, r1;
+  const std::vector<int64_t> whole{0, npts};
+  const std::vector<int64_t> &cv = cuts ? *cuts : whole;
+  for (size_t k = 0; k + 1 < cv.size(); ++k) {
+    const int64_t a = cv[k], len = cv[k + 1] - cv[k];
+    if (len <= 0) continue;
+    const int64_t Sk = std::max<int64_t>(1, (S0 * len + npts / 2) / std::max<int64_t>(npts, 1));
+    const int64_t chunk = cdiv(cdiv(len, Sk), tp) * tp;
+    for (int64_t q = 0; q < len; q += chunk) {
+      r0.push_back(a + q);
+      r1.push_back(a + std::min(len, q + chunk));
+    }
+  }
+  const int64_t S = (int64_t)r0.size();
   for (int l = lmax - 1; l >= lmin; --l) {
     const int Q = l / mc + 1;
     const int ncol = 2 * (l + 1);
@@ -400,8 +417,8 @@ void plan_direct(const std::vector<int32_t> &nl, int lmin, int lmax, int64_t npt
                                 std::floor(cx.kln[(size_t)l * cx.nmax + n0 + g.nrows - 1] * cx.R / MX_H) + 2, 1e9)
                           : 0;
           it.flags = 0;
-          it.p0 = cx.pbase + s * chunk;
-          it.p1 = cx.pbase + std::min(npts, (s + 1) * chunk);
+          it.p0 = cx.pbase + r0[s];
+          it.p1 = cx.pbase + r1[s];
           it.out = g.base + s * g.stride;
           it.kofs = g.kofs;
           const int qc = qclass(it.qhi - it.qlo);
@@ -682,19 +699,19 @@ sfbhip_status launch_mixed_ops(int ops, int ksi, int nitems, const DirectItem *i
 
 // Launch every class of a plan; items[] on the device holds the classes back to back from `*off`.
 // The FP64 classes dealt in turn to st and the handle's two side streams
-// (forked and joined by launch_plan_f64).
+// (forked and joined by launch_plan_f64), or to the side streams only.
 sfbhip_status launch_f64_classes(sfbhip_handle *h, const DirectPlan &pl, DirectItem *ditems, size_t *off,
                                  hipStream_t st, const PointsSoA &pts, const PixelsSoA *pix, const double *dk,
                                  int nmax, const LegendreTables &lt, int use_table, const BesselTableRef &tab,
-                                 double *partial, int *derr, bool *used) {
+                                 double *partial, int *derr, bool *used, bool sides_only = false) {
   const hipStream_t streams[3] = {st, h->mx_side[0], h->mx_side[1]};
-  int next = 0;
+  int next = sides_only ? 1 : 0;
   for (int ci = 0; ci < 5; ++ci) {
     const int n = (int)pl.items[ci].size();
     if (!n) continue;
     const DirectItem *di = ditems + *off;
     const int si = next;
-    next = (next + 1) % 3;
+    next = sides_only ? 3 - next : (next + 1) % 3;
     const hipStream_t s = streams[si];
     if (si > 0) used[si - 1] = true;
     switch (ci) {
@@ -1181,10 +1198,20 @@ constexpr int64_t HR_MAP_CAP_BYTES = 16LL << 30;
 // Cubic-table intervals the map kernel stages in LDS at most (64 KB).
 constexpr int HR_LDS_NODES = 4096;
 
-template <int QMAX>
-void launch_ring_contract(int n, const DirectItem *di, hipStream_t s, const RingPixels &rp, const float *maps,
-                          int64_t npix, const int64_t *dmoff, double unscale, double *partial) {
-  healpix_ring_contract_kernel<QMAX><<<n, 256, 0, s>>>(rp, di, maps, npix, dmoff, unscale, partial);
+// One class of ring items: EQ, the equatorial kernel with the azimuth table
+// (shm bytes) in dynamic LDS; otherwise the general ring kernel.
+template <int QMAX, bool EQ>
+sfbhip_status launch_ring_contract(int n, const DirectItem *di, hipStream_t s, const RingPixels &rp, const float *maps,
+                                   int64_t npix, const int64_t *dmoff, double unscale, double *partial, size_t shm) {
+  if constexpr (EQ) {
+    HIPCHK(hipFuncSetAttribute(reinterpret_cast<const void *>(&healpix_ring_eq_kernel<QMAX>),
+                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
+    healpix_ring_eq_kernel<QMAX><<<n, 256, shm, s>>>(rp, di, maps, npix, dmoff, unscale, partial);
+  } else {
+    healpix_ring_contract_kernel<QMAX><<<n, 256, 0, s>>>(rp, di, maps, npix, dmoff, unscale, partial);
+  }
+  HIPCHK(hipGetLastError());
+  return SFBHIP_OK;
 }
 
 sfbhip_status healpix_ring(sfbhip_handle *h, const int64_t *d_offsets, const double *d_radii, int32_t nside,
@@ -1203,8 +1230,18 @@ sfbhip_status healpix_ring(sfbhip_handle *h, const int64_t *d_offsets, const dou
   CHK(get_buf(h, "hr_rx", (size_t)nrings, &rx));
   CHK(get_buf(h, "hr_rs", (size_t)nrings, &rsn));
   CHK(get_buf(h, "hr_ring_of", (size_t)npix, &ring_of));
+  // equatorial tiles take their azimuth factors from a table of 8 nside
+  // entries when every such tile lies in one ring (nside a multiple of 128, so
+  // that the cap pixel count and the ring length are multiples of the 256-pixel
+  // tile), the table fits LDS beside the kernel's own (nside <= 512), and no m
+  // folds on the equatorial rings (2 (lmax - 1) <= 4 nside)
+  const bool eq_path = nside % 128 == 0 && nside <= 512 && 2 * (L - 1) <= 4 * nside;
+  uint2 *eqtab;
+  CHK(get_buf(h, "hr_eqtab", (size_t)8 * nside, &eqtab));
   if (h->ring_nside != nside) {
     ring_setup_kernel<<<cdiv(npix, 256), 256, 0, st>>>(nside, rx, rsn, ring_of);
+    HIPCHK(hipGetLastError());
+    ring_eq_table_kernel<<<cdiv(8 * nside, 256), 256, 0, st>>>(nside, eqtab);
     HIPCHK(hipGetLastError());
     h->ring_nside = nside;
   }
@@ -1252,6 +1289,11 @@ sfbhip_status healpix_ring(sfbhip_handle *h, const int64_t *d_offsets, const dou
   rp.ptab = ptab;
   rp.poff = dpoff;
   rp.alias_compat = p->alias_compat;
+  rp.eqtab = eqtab;
+  rp.eq_ns = eq_path ? nside : 0;
+  rp.eq_lo = 2LL * nside * (nside - 1);  // the equatorial pixels [eq_lo, eq_hi)
+  rp.eq_hi = npix - rp.eq_lo;
+  const size_t eq_shm = eq_path ? sizeof(uint2) * 8 * (size_t)nside : 0;
   // slot order of the maps and the contraction: pixels by point count within each tile
   int32_t *slot_perm;
   CHK(get_buf(h, "hr_slot_perm", (size_t)std::max<int64_t>(npix, 1), &slot_perm));
@@ -1294,114 +1336,159 @@ sfbhip_status healpix_ring(sfbhip_handle *h, const int64_t *d_offsets, const dou
   if (lsplit > 0) plan_direct(nl, 0, lsplit, npix, F64_NT, F64_TP, F64_MC, 16, 4096, cx, &pl64);
   pl.partial_size = pl64.partial_size;
   pl.groups = pl64.groups;
+  // items never straddle the equatorial boundaries, so that each is all
+  // equatorial (healpix_ring_eq_kernel) or all other (healpix_ring_contract_kernel);
+  // per batch the classes are uploaded as [other Q = 1, 2, 4][equatorial Q = 1, 2, 4]
+  const int64_t eq_lo = 2LL * nside * (nside - 1), eq_hi = npix - eq_lo;
+  const std::vector<int64_t> cuts = eq_path ? std::vector<int64_t>{0, eq_lo, eq_hi, npix} : std::vector<int64_t>{0, npix};
   std::vector<std::vector<DirectItem>> bitems(batches.size());
-  std::vector<std::array<int, 3>> bcounts(batches.size());
+  std::vector<std::array<int, 6>> bcounts(batches.size());
   for (size_t b = 0; b < batches.size(); ++b) {
     DirectPlan pb;
     pb.partial_size = pl.partial_size;
     plan_direct(nl, batches[b].first, batches[b].second, npix, HR_NT, HR_TP, MX_MC, 4, f64_target_items(npix, true) / 2,
-                cx, &pb);
+                cx, &pb, 1, &cuts);
     pl.partial_size = pb.partial_size;
     pl.groups.insert(pl.groups.end(), pb.groups.begin(), pb.groups.end());
-    for (int ci = 0; ci < 3; ++ci) {
-      bitems[b].insert(bitems[b].end(), pb.items[ci].begin(), pb.items[ci].end());
-      bcounts[b][ci] = (int)pb.items[ci].size();
-    }
     if (!pb.items[3].empty() || !pb.items[4].empty()) return fail(SFBHIP_NOT_SUPPORTED, "ring plan class");
+    for (int e = 0; e < 2; ++e)
+      for (int ci = 0; ci < 3; ++ci) {
+        int n = 0;
+        for (const DirectItem &it : pb.items[ci]) {
+          const bool is_eq = eq_path && it.p0 >= eq_lo && it.p1 <= eq_hi;
+          if (is_eq == (e == 1)) {
+            bitems[b].push_back(it);
+            ++n;
+          }
+        }
+        bcounts[b][3 * e + ci] = n;
+      }
   }
   double *partial;
   CHK(get_buf(h, "partial", (size_t)std::max<int64_t>(pl.partial_size, 1), &partial));
 
+  // host arrays the asynchronous copies read stay alive until the stream is synchronised
+  std::vector<DirectItem> lo;
+  for (auto &v : pl64.items) lo.insert(lo.end(), v.begin(), v.end());
+  DirectItem *dlo = nullptr;
+  if (!lo.empty()) {
+    CHK(get_buf(h, "hr_lowl_items", lo.size(), &dlo));
+    HIPCHK(hipMemcpyAsync(dlo, lo.data(), sizeof(DirectItem) * lo.size(), hipMemcpyHostToDevice, st));
+  }
+  std::vector<std::vector<int64_t>> bmoff(batches.size());
+  std::vector<std::vector<MapItem>> bmitems(batches.size());
+
+  // Streams: the maps and the largest contraction class run on st; l < lsplit
+  // (FP64, exact j_l: some of these C_l sit at cancellation level, where even
+  // the quintic table's 1e-10 of the amplitude shows) runs on the handle's two
+  // side streams beside the maps; the smaller contraction classes run on the
+  // side streams after the maps.  The side streams are forked from st here and
+  // joined back before the reduction.
   HIPCHK(hipEventRecord(h->ev[0], st));
-  if (lsplit > 0) {
-    std::vector<DirectItem> lo;
-    for (auto &v : pl64.items) lo.insert(lo.end(), v.begin(), v.end());
-    DirectItem *dlo;
-    CHK(get_buf(h, "hr_lowl_items", std::max<size_t>(lo.size(), 1), &dlo));
-    if (!lo.empty())
-      HIPCHK(hipMemcpyAsync(dlo, lo.data(), sizeof(DirectItem) * lo.size(), hipMemcpyHostToDevice, st));
-    // exact j_l: some of these C_l sit at cancellation level, where even the
-    // quintic table's 1e-10 of the amplitude shows (a C_0n at nside 64 moved by 8e-5)
+  CHK(fork_sides(h, st));
+  bool used[2] = {false, false};
+  sfbhip_status rc = SFBHIP_OK;
+  if (!lo.empty()) {
     size_t off = 0;
-    CHK(launch_plan_f64(h, pl64, dlo, &off, st, PointsSoA{}, &pix, dk, p->nmax, lt, JL_EXACT, BesselTableRef{}, partial,
-                        derr));
-    HIPCHK(hipStreamSynchronize(st));  // lo goes out of scope
+    rc = launch_f64_classes(h, pl64, dlo, &off, st, PointsSoA{}, &pix, dk, p->nmax, lt, JL_EXACT, BesselTableRef{},
+                            partial, derr, used, true);
   }
-  float map_ms = 0.f;
-  for (size_t b = 0; b < batches.size(); ++b) {
-    const int lb0 = batches[b].first, lb1 = batches[b].second;
-    // map offsets within the batch's workspace, and the map items
-    std::vector<int64_t> moff(std::max(L, 1), 0);
-    std::vector<MapItem> mitems;
-    int64_t o = 0, nmaprows = 0;
-    for (int l = lb0; l < lb1; ++l) {
-      moff[l] = o;
-      o += (int64_t)nl[l] * npix;
-      nmaprows += cdiv(nl[l], HR_NB);
-    }
-    const int64_t target = 8192;
-    int64_t S = std::max<int64_t>(1, cdiv(target, std::max<int64_t>(nmaprows, 1)));
-    S = std::min<int64_t>(S, cdiv(npix, 256));
-    const int64_t chunk = cdiv(cdiv(npix, S), 256) * 256;
-    int nodes = 1;
-    for (int l = lb0; l < lb1; ++l) {
-      if (nl[l] == 0) continue;
-      nodes = std::max(nodes, h->kc.cub_nint[l]);
-      for (int n0 = 0; n0 < nl[l]; n0 += HR_NB)
-        for (int64_t p0 = 0; p0 < npix; p0 += chunk) {
-          MapItem mi;
-          mi.l = l;
-          mi.n0 = n0;
-          mi.nrows = std::min(HR_NB, nl[l] - n0);
-          mi.pad = 0;
-          mi.p0 = p0;
-          mi.p1 = std::min(npix, p0 + chunk);
-          mi.moff = moff[l];
-          mi.kofs = (int64_t)l * p->nmax + n0;
-          mitems.push_back(mi);
+  for (size_t b = 0; b < batches.size() && rc == SFBHIP_OK; ++b) {
+    rc = [&]() -> sfbhip_status {
+      const int lb0 = batches[b].first, lb1 = batches[b].second;
+      // map offsets within the batch's workspace, and the map items
+      std::vector<int64_t> &moff = bmoff[b];
+      std::vector<MapItem> &mitems = bmitems[b];
+      moff.assign(std::max(L, 1), 0);
+      int64_t o = 0, nmaprows = 0;
+      for (int l = lb0; l < lb1; ++l) {
+        moff[l] = o;
+        o += (int64_t)nl[l] * npix;
+        nmaprows += cdiv(nl[l], HR_NB);
+      }
+      const int64_t target = 8192;
+      int64_t S = std::max<int64_t>(1, cdiv(target, std::max<int64_t>(nmaprows, 1)));
+      S = std::min<int64_t>(S, cdiv(npix, 256));
+      const int64_t chunk = cdiv(cdiv(npix, S), 256) * 256;
+      int nodes = 1;
+      for (int l = lb0; l < lb1; ++l) {
+        if (nl[l] == 0) continue;
+        nodes = std::max(nodes, h->kc.cub_nint[l]);
+        for (int n0 = 0; n0 < nl[l]; n0 += HR_NB)
+          for (int64_t p0 = 0; p0 < npix; p0 += chunk) {
+            MapItem mi;
+            mi.l = l;
+            mi.n0 = n0;
+            mi.nrows = std::min(HR_NB, nl[l] - n0);
+            mi.pad = 0;
+            mi.p0 = p0;
+            mi.p1 = std::min(npix, p0 + chunk);
+            mi.moff = moff[l];
+            mi.kofs = (int64_t)l * p->nmax + n0;
+            mitems.push_back(mi);
+          }
+      }
+      nodes = std::min(nodes, HR_LDS_NODES);
+      MapItem *dmi;
+      DirectItem *ditems;
+      CHK(get_buf(h, "hr_mitems", std::max<size_t>(mitems.size(), 1), &dmi));
+      CHK(get_buf(h, "items", std::max<size_t>(bitems[b].size(), 1), &ditems));
+      HIPCHK(hipMemcpyAsync(dmoff, moff.data(), sizeof(int64_t) * moff.size(), hipMemcpyHostToDevice, st));
+      if (!mitems.empty())
+        HIPCHK(hipMemcpyAsync(dmi, mitems.data(), sizeof(MapItem) * mitems.size(), hipMemcpyHostToDevice, st));
+      if (!bitems[b].empty())
+        HIPCHK(hipMemcpyAsync(ditems, bitems[b].data(), sizeof(DirectItem) * bitems[b].size(), hipMemcpyHostToDevice,
+                              st));
+      const size_t shm = sizeof(float4) * (size_t)nodes;
+      if (!mitems.empty()) {
+        HIPCHK(hipFuncSetAttribute(reinterpret_cast<const void *>(&ring_maps_kernel),
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
+        ring_maps_kernel<<<(unsigned)mitems.size(), 256, shm, st>>>(dmi, d_offsets, slot_perm, d_radii, npix, dk, ct,
+                                                                   nodes, scale, maps);
+        HIPCHK(hipGetLastError());
+      }
+      if (batches.size() == 1) HIPCHK(hipEventRecord(h->ev[1], st));
+      // the contraction classes start once the maps are written: the classes
+      // of 4 chunks (the largest) on st, of 2 on side stream 0, of 1 on side
+      // stream 1; equatorial classes after the others of their stream
+      HIPCHK(hipEventRecord(h->mx_fork, st));
+      for (int k = 0; k < 2; ++k) HIPCHK(hipStreamWaitEvent(h->mx_side[k], h->mx_fork, 0));
+      size_t off = 0;
+      for (int k = 0; k < 6; ++k) {
+        const int n = bcounts[b][k];
+        if (!n) continue;
+        const DirectItem *di = ditems + off;
+        const int ci = k % 3;
+        const hipStream_t s = ci == 2 ? st : h->mx_side[1 - ci];
+        if (ci < 2) used[1 - ci] = true;
+        switch (k) {
+          case 0: CHK((launch_ring_contract<1, false>(n, di, s, rp, maps, npix, dmoff, unscale, partial, 0))); break;
+          case 1: CHK((launch_ring_contract<2, false>(n, di, s, rp, maps, npix, dmoff, unscale, partial, 0))); break;
+          case 2: CHK((launch_ring_contract<4, false>(n, di, s, rp, maps, npix, dmoff, unscale, partial, 0))); break;
+          case 3: CHK((launch_ring_contract<1, true>(n, di, s, rp, maps, npix, dmoff, unscale, partial, eq_shm))); break;
+          case 4: CHK((launch_ring_contract<2, true>(n, di, s, rp, maps, npix, dmoff, unscale, partial, eq_shm))); break;
+          default: CHK((launch_ring_contract<4, true>(n, di, s, rp, maps, npix, dmoff, unscale, partial, eq_shm))); break;
         }
-    }
-    nodes = std::min(nodes, HR_LDS_NODES);
-    MapItem *dmi;
-    DirectItem *ditems;
-    CHK(get_buf(h, "hr_mitems", std::max<size_t>(mitems.size(), 1), &dmi));
-    CHK(get_buf(h, "items", std::max<size_t>(bitems[b].size(), 1), &ditems));
-    HIPCHK(hipMemcpyAsync(dmoff, moff.data(), sizeof(int64_t) * moff.size(), hipMemcpyHostToDevice, st));
-    if (!mitems.empty())
-      HIPCHK(hipMemcpyAsync(dmi, mitems.data(), sizeof(MapItem) * mitems.size(), hipMemcpyHostToDevice, st));
-    if (!bitems[b].empty())
-      HIPCHK(hipMemcpyAsync(ditems, bitems[b].data(), sizeof(DirectItem) * bitems[b].size(), hipMemcpyHostToDevice, st));
-    const size_t shm = sizeof(float4) * (size_t)nodes;
-    if (!mitems.empty()) {
-      HIPCHK(hipFuncSetAttribute(reinterpret_cast<const void *>(&ring_maps_kernel),
-                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
-      ring_maps_kernel<<<(unsigned)mitems.size(), 256, shm, st>>>(dmi, d_offsets, slot_perm, d_radii, npix, dk, ct,
-                                                                 nodes, scale, maps);
-      HIPCHK(hipGetLastError());
-    }
-    if (batches.size() == 1) HIPCHK(hipEventRecord(h->ev[1], st));
-    size_t off = 0;
-    for (int ci = 0; ci < 3; ++ci) {
-      const int n = bcounts[b][ci];
-      if (!n) continue;
-      const DirectItem *di = ditems + off;
-      if (ci == 0) launch_ring_contract<1>(n, di, st, rp, maps, npix, dmoff, unscale, partial);
-      if (ci == 1) launch_ring_contract<2>(n, di, st, rp, maps, npix, dmoff, unscale, partial);
-      if (ci == 2) launch_ring_contract<4>(n, di, st, rp, maps, npix, dmoff, unscale, partial);
-      HIPCHK(hipGetLastError());
-      off += n;
-    }
-    if (batches.size() > 1) HIPCHK(hipStreamSynchronize(st));  // the next batch rewrites the maps and items
+        off += n;
+      }
+      if (b + 1 < batches.size()) {
+        // the next batch rewrites the maps and items: every stream drains first
+        for (int k = 0; k < 2; ++k) HIPCHK(hipStreamSynchronize(h->mx_side[k]));
+        HIPCHK(hipStreamSynchronize(st));
+      }
+      return SFBHIP_OK;
+    }();
   }
+  CHK(join_sides(h, st, rc, used));
   if (batches.size() != 1) HIPCHK(hipEventRecord(h->ev[1], st));
   HIPCHK(hipEventRecord(h->ev[2], st));
   CHK(finalize(h, p, pl.groups, partial, norm, d_k_ln, d_c_ln, d_f_lmn, st));
   HIPCHK(hipEventRecord(h->ev[3], st));
   HIPCHK(hipStreamSynchronize(st));
-  // phases: [0] maps, [1] ring contraction, [2] reduction and C_l; the dominant
-  // kernels are the map and contraction kernels together
+  // phases (events on st): [0] maps (l < lsplit runs beside them on the side
+  // streams), [1] ring contraction up to the join of every stream, [2]
+  // reduction and C_l; the dominant kernels are the map and contraction kernels
   for (int i = 0; i < 3; ++i) HIPCHK(hipEventElapsedTime(&h->phase_ms[i], h->ev[i], h->ev[i + 1]));
-  (void)map_ms;
   h->last_ms = h->phase_ms[0] + h->phase_ms[1];
   return SFBHIP_OK;
 }

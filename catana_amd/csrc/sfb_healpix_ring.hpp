@@ -205,17 +205,40 @@ struct RingPixels {
   const int64_t *poff;
   const int32_t *perm;                 // slot order (ring_slot_order_kernel): slot i holds pixel perm[i]
   int32_t alias_compat;
+  // equatorial tiles (ring_eq_table_kernel): eq_ns = nside when the path is on
+  // (0 otherwise); tiles of [eq_lo, eq_hi) take their azimuth factors from
+  // eqtab (8 nside entries, staged in the kernel's dynamic LDS)
+  const uint2 *eqtab;
+  int32_t eq_ns;
+  int64_t eq_lo, eq_hi;
 };
+
+// Azimuth factors of the equatorial rings: E[k] = e^{-i 2 pi k / (8 nside)} 2^12
+// as its two FP16 pieces, .x = (re, im) of piece 0, .y = (re, im) of piece 1.
+// Every equatorial pixel's azimuth is phi_j = (2 j + s) pi / (4 nside) (s = 1 on
+// shifted rings), so e^{-i m phi_j} = E[(m (2 j + s)) mod 8 nside] exactly.
+__global__ void ring_eq_table_kernel(int nside, uint2 *__restrict__ E) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= 8 * nside) return;
+  double sn, cs;
+  sincospi((double)k / (4.0 * nside), &sn, &cs);
+  unsigned hi, lo;
+  split2((float)(cs * 4096.0), (float)(-sn * 4096.0), hi, lo);
+  E[k] = make_uint2(hi, lo);
+}
 
 // Y* 2^12 of one pixel for the 16 m of a chunk (from mtop down) into the
 // chunk buffer: two point-major FP16 planes (row tid: 32 columns, 2 mm: real,
 // 2 mm + 1: imaginary; the layout of sfb_direct_split16.hpp), stored as one
 // 64-bit word per plane and two m.  Leaves the azimuth 16 steps lower.
 // Columns of m < 0 and padding pixels are zero.
-// Pq[sgn * mm] is P_l^m at m = mtop - mm (sgn = 1: the LDS copy, already zero
-// for m < 0; sgn = -1: the global table, read at max(m, 0)).
-template <bool QUIRK>
-__device__ __forceinline__ void ring_emit(unsigned short *Yw, int tid, int mtop, bool pok, const double *Pq, int sgn,
+// STAGED: Pq[mm] is P_l^m at m = mtop - mm from the LDS copy (already zero for
+// m < 0); otherwise Pq[-mm] from the global table, read at max(m, 0).  The two
+// are separate instantiations so that each read has one address space: a
+// pointer that may be either becomes a flat load, and waiting for a flat load
+// waits for every global load in flight (here the tile's map values).
+template <bool QUIRK, bool STAGED>
+__device__ __forceinline__ void ring_emit(unsigned short *Yw, int tid, int mtop, bool pok, const double *Pq,
                                           int nr, int &smod, bool quirk, double &cm, double &sm, double &cp,
                                           double &sp, double c2, double &cm2, double &sm2, double cph2, double sph2) {
 #pragma unroll
@@ -227,7 +250,7 @@ __device__ __forceinline__ void ring_emit(unsigned short *Yw, int tid, int mtop,
       const int m = mtop - mm;
       // staged: index mm (the LDS copy holds zeros for m < 0, all inside it);
       // global: index -(mm) = P at m, or P at m = 0 for m < 0 (masked below)
-      const double pr = sgn > 0 ? Pq[mm] : Pq[-(m >= 0 ? mm : mtop)];
+      const double pr = STAGED ? Pq[mm] : Pq[-(m >= 0 ? mm : mtop)];
       const double pv = (pok && m >= 0) ? pr : 0.0;
       double c = cm, sn = sm;
       if constexpr (QUIRK) {
@@ -252,6 +275,101 @@ __device__ __forceinline__ void ring_emit(unsigned short *Yw, int tid, int mtop,
     const int e = tid * S16_ROW + 2 * mp;
     *reinterpret_cast<uint2 *>(Yw + e) = make_uint2(hi[0], hi[1]);
     *reinterpret_cast<uint2 *>(Yw + S16_PLANE + e) = make_uint2(lo[0], lo[1]);
+  }
+}
+
+// Equatorial tile: the chunk's Y* columns are e^{-i m phi} 2^12 alone (the
+// tile's pixels share one ring; its Legendre values scale the FP64 sums per
+// column afterwards), read from the LDS table E at index (m a) mod 8 nside,
+// a = 2 j + s.  Columns of m < 0 and padding pixels are zero.
+__device__ __forceinline__ void ring_emit_eq(unsigned short *Yw, int tid, int mtop, bool pok, int a, int mask,
+                                             const uint2 *E) {
+  int idx = (mtop * a) & mask;
+#pragma unroll
+  for (int mp = 0; mp < MX_MC; mp += 2) {
+    unsigned hi[2], lo[2];
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+      const int m = mtop - mp - u;
+      const uint2 e = E[idx];
+      const bool ok = pok && m >= 0;
+      hi[u] = ok ? e.x : 0u;
+      lo[u] = ok ? e.y : 0u;
+      idx = (idx - a) & mask;
+    }
+    const int e = tid * S16_ROW + 2 * mp;
+    *reinterpret_cast<uint2 *>(Yw + e) = make_uint2(hi[0], hi[1]);
+    *reinterpret_cast<uint2 *>(Yw + S16_PLANE + e) = make_uint2(lo[0], lo[1]);
+  }
+}
+
+// A fragments of a tile from its map values (this lane: row, and points
+// base + 32 s + 8 g + 4 hq .. + 3), zero past the item's pixels or rows.
+template <int SPW>
+__device__ __forceinline__ void ring_split_a(const float4 (&araw)[SPW][2], h8 (&ah)[2][SPW], int64_t base, int g,
+                                             bool rowok, int64_t p1) {
+#pragma unroll
+  for (int s = 0; s < SPW; ++s) {
+    unsigned hw[4], lw[4];
+#pragma unroll
+    for (int hq = 0; hq < 2; ++hq) {
+      const int64_t p = base + 32 * s + 8 * g + 4 * hq;
+      const bool ok = rowok && p < p1;
+      const float4 v = ok ? araw[s][hq] : make_float4(0.f, 0.f, 0.f, 0.f);
+      split2(v.x, v.y, hw[2 * hq], lw[2 * hq]);
+      split2(v.z, v.w, hw[2 * hq + 1], lw[2 * hq + 1]);
+    }
+    ah[0][s] = __builtin_bit_cast(h8, (u4v){hw[0], hw[1], hw[2], hw[3]});
+    ah[1][s] = __builtin_bit_cast(h8, (u4v){lw[0], lw[1], lw[2], lw[3]});
+  }
+}
+
+// The contraction of one chunk: this wave's 16 rows x 32 columns over the
+// tile's SPW 32-point steps, three products per step (leading p0 q0 in (a0,
+// a1), folded every 64 points; corrections in (c0, c1), folded once).  EQ: the
+// folds go to FP64 tile sums t, added to sum scaled by the column's Legendre
+// value pc (one per column half); otherwise straight into sum.  All MFMAs and
+// the reads of their results form one basic block.
+template <int SPW, bool EQ>
+__device__ __forceinline__ void ring_mfma_chunk(const h8 (&ah)[2][SPW], const unsigned short *Yh, int lane,
+                                                v4d (&sum)[2], double pc0, double pc1) {
+  constexpr int FOLD = 64;
+  v4f a0 = v4f{0.f, 0.f, 0.f, 0.f}, a1 = a0, c0 = a0, c1 = a0;
+  v4d t[2] = {v4d{0.0, 0.0, 0.0, 0.0}, v4d{0.0, 0.0, 0.0, 0.0}};
+  v4d(&acc)[2] = EQ ? t : sum;
+#pragma unroll
+  for (int s = 0; s < SPW; ++s) {
+    const h8 b00 = s16_bfrag(Yh, 32 * s, 0, lane);
+    const h8 b01 = s16_bfrag(Yh, 32 * s, 16, lane);
+    const h8 b10 = s16_bfrag(Yh + S16_PLANE, 32 * s, 0, lane);
+    const h8 b11 = s16_bfrag(Yh + S16_PLANE, 32 * s, 16, lane);
+    c0 = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[0][s], b10, c0, 0, 0, 0);
+    c0 = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[1][s], b00, c0, 0, 0, 0);
+    c1 = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[0][s], b11, c1, 0, 0, 0);
+    c1 = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[1][s], b01, c1, 0, 0, 0);
+    a0 = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[0][s], b00, a0, 0, 0, 0);
+    a1 = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[0][s], b01, a1, 0, 0, 0);
+    if ((32 * (s + 1)) % FOLD == 0 && s + 1 < SPW) {
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        acc[0][j] += (double)a0[j];
+        acc[1][j] += (double)a1[j];
+      }
+      a0 = v4f{0.f, 0.f, 0.f, 0.f};
+      a1 = a0;
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    acc[0][j] += (double)a0[j] + (double)c0[j];
+    acc[1][j] += (double)a1[j] + (double)c1[j];
+  }
+  if constexpr (EQ) {
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      sum[0][j] = fma(pc0, t[0][j], sum[0][j]);
+      sum[1][j] = fma(pc1, t[1][j], sum[1][j]);
+    }
   }
 }
 
@@ -380,12 +498,21 @@ __global__ __launch_bounds__(256, 2) void healpix_ring_contract_kernel(RingPixel
         __syncthreads();  // previous chunk's readers are done
         const int mtop = l - MX_MC * q;
         // Legendre values from LDS (staged) at index (mtop0 - m), else from the global table at m
-        const double *Pq = staged ? Pl + MX_MC * qq : Pr + mtop;
-        const int sgn = staged ? 1 : -1;
-        if (wave_quirk)
-          ring_emit<true>(Yh, tid, mtop, pok, Pq, sgn, nr, smod, quirk, cm, sm, cp, sp, c2, cm2, sm2, cph2, sph2);
-        else
-          ring_emit<false>(Yh, tid, mtop, pok, Pq, sgn, nr, smod, quirk, cm, sm, cp, sp, c2, cm2, sm2, cph2, sph2);
+        if (staged) {
+          if (wave_quirk)
+            ring_emit<true, true>(Yh, tid, mtop, pok, Pl + MX_MC * qq, nr, smod, quirk, cm, sm, cp, sp, c2, cm2, sm2,
+                                  cph2, sph2);
+          else
+            ring_emit<false, true>(Yh, tid, mtop, pok, Pl + MX_MC * qq, nr, smod, quirk, cm, sm, cp, sp, c2, cm2, sm2,
+                                   cph2, sph2);
+        } else {
+          if (wave_quirk)
+            ring_emit<true, false>(Yh, tid, mtop, pok, Pr + mtop, nr, smod, quirk, cm, sm, cp, sp, c2, cm2, sm2, cph2,
+                                   sph2);
+          else
+            ring_emit<false, false>(Yh, tid, mtop, pok, Pr + mtop, nr, smod, quirk, cm, sm, cp, sp, c2, cm2, sm2, cph2,
+                                    sph2);
+        }
         __syncthreads();
         if (wave_rows) {
           v4f a0 = v4f{0.f, 0.f, 0.f, 0.f}, a1 = a0, c0 = a0, c1 = a0;
@@ -417,6 +544,119 @@ __global__ __launch_bounds__(256, 2) void healpix_ring_contract_kernel(RingPixel
             sum[qq][1][j] += (double)a1[j] + (double)c1[j];
           }
         }
+      }
+    }
+  }
+
+  // ---- the item's partial block (64 rows x its columns), FP64
+  double *out = partial + it.out;
+#pragma unroll
+  for (int qq = 0; qq < QMAX; ++qq) {
+    const int q = it.qlo + qq;
+    if (q < it.qhi && wave_rows) {
+#pragma unroll
+      for (int c = 0; c < 2; ++c) {
+        const int lc = 16 * c + (lane & 15);
+        const int m = l - MX_MC * q - (lc >> 1);
+        if (m >= 0) {
+          const int cc = 2 * m + (lc & 1);
+          // v_mfma_f32_16x16x32_f16 result: element j of lane i is row 4(i/16) + j, column i%16
+#pragma unroll
+          for (int j = 0; j < 4; ++j) out[(size_t)(16 * wave + 4 * g + j) * ncol + cc] = sum[qq][c][j] * unscale;
+        }
+      }
+    }
+  }
+}
+
+// The ring contraction of equatorial items (every tile in [eq_lo, eq_hi), so
+// each 256-pixel tile lies in one ring of 4 nside pixels; rp.eq_ns != 0).  Per
+// tile the B operand is e^{-i m phi} 2^12 alone, from the LDS table of 8 nside
+// azimuth factors: no per-pixel recurrence or set-up.  The tile's FP64 sums are
+// scaled per column by the ring's Legendre value and added to the item's
+// sums.  Same items, layout and output as healpix_ring_contract_kernel.
+// Dynamic LDS: the azimuth table (8 eq_ns uint2).
+template <int QMAX>
+__global__ __launch_bounds__(256, 2) void healpix_ring_eq_kernel(RingPixels rp, const DirectItem *__restrict__ items,
+                                                              const float *__restrict__ maps, int64_t npix,
+                                                              const int64_t *__restrict__ moff, double unscale,
+                                                              double *__restrict__ partial) {
+  constexpr int SPW = HR_TP / 32;
+  static_assert(HR_TP == MX_TP, "the point-major Y* planes hold MX_TP rows");
+  __shared__ __attribute__((aligned(16))) unsigned short Yh[S16_YBUF];
+  __shared__ double Pe[MX_MC * QMAX];  // the tile ring's P_l^m 2^-12 at index mtop0 - m (0 for m < 0)
+  extern __shared__ uint2 Eq[];
+
+  const DirectItem it = items[blockIdx.x];
+  const int l = it.l;
+  const int ncol = 2 * (l + 1);
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
+  const int g = lane >> 4;
+  const int row = 16 * wave + (lane & 15);
+  const bool wave_rows = 16 * wave < it.nrows;  // wave-uniform
+  const bool rowok = row < it.nrows;
+  const float *__restrict__ M = maps + moff[l] + (int64_t)(it.n0 + (rowok ? row : 0)) * npix;
+  const double *__restrict__ P = rp.ptab + rp.poff[l];
+  const int64_t ns = rp.eq_ns;
+  const int mask = (int)(8 * ns - 1);
+  for (int k = tid; k < 8 * ns; k += 256) Eq[k] = rp.eqtab[k];  // visible after the first chunk's barrier
+
+  v4d sum[QMAX][2];
+#pragma unroll
+  for (int q = 0; q < QMAX; ++q)
+#pragma unroll
+    for (int c = 0; c < 2; ++c) sum[q][c] = v4d{0.0, 0.0, 0.0, 0.0};
+
+  const int mtop0 = l - MX_MC * it.qlo;
+  const int mspan = MX_MC * (it.qhi - it.qlo);
+  for (int64_t base = it.p0; base < it.p1; base += HR_TP) {
+    // this thread's pixel (slot gp holds pixel perm[gp], inside the tile) and
+    // the tile's ring, then its Legendre values, then the map values (issued
+    // last: a wait on a global load also waits on those issued before it)
+    const int64_t gp = base + tid;
+    const bool pok = gp < it.p1;
+    const int64_t px = rp.perm[pok ? gp : base];
+    const int rho = rp.ring[base];
+    const int mk = mtop0 - tid;
+    const bool pk = tid < mspan && mk >= 0;
+    const double pv = P[pk ? (int64_t)rho * (l + 1) + mk : 0];
+    // the tile's A fragments (the wait for the map values is exposed once
+    // per tile: the table generation below is too short to cover it)
+    h8 ah[2][SPW];
+    {
+      float4 araw[SPW][2];
+      const int64_t last = it.p1 - 4;
+#pragma unroll
+      for (int s = 0; s < SPW; ++s)
+#pragma unroll
+        for (int hq = 0; hq < 2; ++hq) {
+          const int64_t p = base + 32 * s + 8 * g + 4 * hq;
+          araw[s][hq] = *reinterpret_cast<const float4 *>(M + (p < last ? p : last));
+        }
+      if (wave_rows) ring_split_a(araw, ah, base, g, rowok, it.p1);
+    }
+    // (the previous tile's readers of Pe are past its last chunk barrier)
+    if (tid < MX_MC * QMAX) Pe[tid] = pk ? pv * 0x1p-12 : 0.0;
+    // azimuth phi_j = (2 j + s) pi / (4 nside) of the pixel: table step a = 2 j + s
+    const int64_t r1 = (int64_t)rho + 1 - ns;  // ring number - nside
+    const int64_t start = 2 * ns * (ns - 1) + r1 * 4 * ns;
+    const int ea = (int)(2 * (px - start)) + ((r1 & 1) == 0 ? 1 : 0);
+#pragma unroll
+    for (int qq = 0; qq < QMAX; ++qq) {
+      const int q = it.qlo + qq;
+      if (q < it.qhi) {  // block-uniform
+        __syncthreads();  // previous chunk's readers are done; Pe and Eq written
+        const int mtop = l - MX_MC * q;
+        ring_emit_eq(Yh, tid, mtop, pok, ea, mask, Eq);
+        // the Legendre values of this lane's two result columns, read before
+        // the barrier below (after which the next tile may rewrite Pe)
+        const int m0 = mtop - ((lane & 15) >> 1), m1 = m0 - MX_MC / 2;
+        const double pc0 = m0 >= 0 ? Pe[mtop0 - m0] : 0.0;
+        const double pc1 = m1 >= 0 ? Pe[mtop0 - m1] : 0.0;
+        __syncthreads();
+        if (wave_rows) ring_mfma_chunk<SPW, true>(ah, Yh, lane, sum[qq], pc0, pc1);
       }
     }
   }

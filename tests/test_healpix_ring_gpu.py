@@ -56,8 +56,9 @@ def _cmp(cr, c64, fr, f64, what, low_l_tol=TOL_LOW_L):
     return full.max()
 
 
+# nside 128 and 256 run the equatorial tiles through healpix_ring_eq_kernel
 @pytest.mark.parametrize("nside,lmax,alias", [(4, 20, True), (8, 30, True), (8, 30, False), (32, 64, True),
-                                              (64, 40, False)])
+                                              (64, 40, False), (128, 90, True), (256, 48, False)])
 def test_ring_vs_fp64_healpix(handle, nside, lmax, alias):
     pts = workloads.radial_selection(60000, 800.0, r0=300.0, seed=50 + nside)
     print(f"case: ring path nside={nside} lmax={lmax} alias={alias}", flush=True)
