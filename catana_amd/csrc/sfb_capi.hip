@@ -2586,6 +2586,10 @@ struct sfbhip_batch_plan {
   std::vector<void *> mem;         // device allocations owned by the plan
   hipStream_t own = nullptr;       // NULL stream argument: enqueue / launch here (synchronises with stream 0)
   hipStream_t cap = nullptr;       // NULL stream argument: capture here
+  // the grouped launches of point splits 2 and 4 run on these (default
+  // priority), forked from and joined back to the enqueue's stream by events
+  hipStream_t side[2] = {nullptr, nullptr};
+  hipEvent_t fork = nullptr, join[2] = {nullptr, nullptr};
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
@@ -2614,7 +2618,8 @@ struct sfbhip_batch_plan {
   int nlow = 0;
   int n64[5] = {0, 0, 0, 0, 0};
   int nmx = 0;
-  int grid_mx = 1;
+  int nmx_ks[3] = {0, 0, 0};   // grouped items with point split 1, 2, 4 (in this order after the FP64 items)
+  int grid_ks[3] = {1, 1, 1};
   ReduceGroup *dgroups = nullptr;
   int ngroups = 0, red_chunks = 1;
   double *partial = nullptr;
@@ -2766,10 +2771,28 @@ sfbhip_status plan_enqueue(sfbhip_batch_plan *pl, hipStream_t st, bool timed) {
     off += k;
   }
   if (timed) HIPCHK(hipEventRecord(pl->ev[2], st));
-  if (pl->nmx) {
-    sfb_direct_split16_grouped_kernel<<<pl->grid_mx, 256, 0, st>>>(pts, pl->ditems + off, pl->nmx, pl->derr + 4,
-                                                                   pl->dk, pl->ls, pl->ct, pl->partial);
+  // one persistent launch per point split, each with its own item counter:
+  // point split 1 on st, 2 and 4 on the plan's side streams, so that each
+  // launch's blocks take over the compute units the others' blocks leave
+  // (running the FP64 launches there too, beside the grouped ones, measured
+  // no faster: 166.3 ms per 64-catalogue replay either way)
+  HIPCHK(hipEventRecord(pl->fork, st));
+  for (int k = 0; k < 2; ++k) HIPCHK(hipStreamWaitEvent(pl->side[k], pl->fork, 0));
+  for (int k = 0; k < 3; ++k) {
+    const int n = pl->nmx_ks[k];
+    const DirectItem *di = pl->ditems + off;
+    int *ctr = pl->derr + 4 + k;
+    const unsigned gx = (unsigned)pl->grid_ks[k];
+    const hipStream_t s = k == 0 ? st : pl->side[k - 1];
+    if (n && k == 0) sfb_direct_split16_grouped_kernel<1><<<gx, 256, 0, s>>>(pts, di, n, ctr, pl->dk, pl->ls, pl->ct, pl->partial);
+    if (n && k == 1) sfb_direct_split16_grouped_kernel<2><<<gx, 256, 0, s>>>(pts, di, n, ctr, pl->dk, pl->ls, pl->ct, pl->partial);
+    if (n && k == 2) sfb_direct_split16_grouped_kernel<4><<<gx, 256, 0, s>>>(pts, di, n, ctr, pl->dk, pl->ls, pl->ct, pl->partial);
     HIPCHK(hipGetLastError());
+    off += n;
+  }
+  for (int k = 0; k < 2; ++k) {
+    HIPCHK(hipEventRecord(pl->join[k], pl->side[k]));
+    HIPCHK(hipStreamWaitEvent(st, pl->join[k], 0));
   }
   if (timed) HIPCHK(hipEventRecord(pl->ev[3], st));
   if (pl->ftot) {
@@ -2995,6 +3018,18 @@ sfbhip_status plan_create(sfbhip_handle *h, const sfbhip_catalog *cats, int32_t 
                  &mx, &groups, &psize);
   std::stable_sort(mx.begin(), mx.end(),
                    [](const DirectItem &a, const DirectItem &b) { return item_cost(a) > item_cost(b); });
+  // by point split (1, 2, 4), each part still by decreasing cost
+  {
+    std::vector<DirectItem> byks;
+    const int32_t ksf[3] = {0, ITEM_SPLIT_KS2, ITEM_SPLIT_KS4};
+    for (int k = 0; k < 3; ++k) {
+      const size_t before = byks.size();
+      for (const DirectItem &it : mx)
+        if ((it.flags & (ITEM_SPLIT_KS2 | ITEM_SPLIT_KS4)) == ksf[k]) byks.push_back(it);
+      pl->nmx_ks[k] = (int)(byks.size() - before);
+    }
+    mx.swap(byks);
+  }
   std::vector<DirectItem> all;
   for (auto &v : pllw.items) all.insert(all.end(), v.begin(), v.end());
   pl->nlow = (int)all.size();
@@ -3038,7 +3073,7 @@ sfbhip_status plan_create(sfbhip_handle *h, const sfbhip_catalog *cats, int32_t 
     CHK(plan_alloc(pl, (size_t)hb, &pl->hist));
     pl->sort_nb = nbmax;
   }
-  pl->grid_mx = (int)std::max<int64_t>(1, std::min<int64_t>(pl->nmx, 2LL * h->ncu));
+  for (int k = 0; k < 3; ++k) pl->grid_ks[k] = (int)std::max<int64_t>(1, std::min<int64_t>(pl->nmx_ks[k], 2LL * h->ncu));
   // catalogue descriptors
   std::vector<CatPrep> prep(ncat);
   std::vector<CatOut> outs(ncat);
@@ -3068,6 +3103,11 @@ void plan_release(sfbhip_batch_plan *pl) {
     if (e) (void)hipEventDestroy(e);
   if (pl->own) (void)hipStreamDestroy(pl->own);
   if (pl->cap) (void)hipStreamDestroy(pl->cap);
+  if (pl->fork) (void)hipEventDestroy(pl->fork);
+  for (int k = 0; k < 2; ++k) {
+    if (pl->join[k]) (void)hipEventDestroy(pl->join[k]);
+    if (pl->side[k]) (void)hipStreamDestroy(pl->side[k]);
+  }
   delete pl;
 }
 
@@ -3122,6 +3162,11 @@ sfbhip_status sfbhip_batch_plan_create(sfbhip_handle *h, const sfbhip_catalog *c
   for (auto &e : pl->ev) step(hipEventCreate(&e), "hipEventCreate");
   step(hipStreamCreateWithFlags(&pl->own, hipStreamDefault), "hipStreamCreate");
   step(hipStreamCreateWithFlags(&pl->cap, hipStreamNonBlocking), "hipStreamCreate");
+  step(hipEventCreateWithFlags(&pl->fork, hipEventDisableTiming), "hipEventCreate");
+  for (int k = 0; k < 2; ++k) {
+    step(hipEventCreateWithFlags(&pl->join[k], hipEventDisableTiming), "hipEventCreate");
+    step(hipStreamCreateWithFlags(&pl->side[k], hipStreamNonBlocking), "hipStreamCreate");
+  }
   if (rc == SFBHIP_OK) rc = plan_create(h, cats, ncat, p, r_bound, pl);
   if (rc != SFBHIP_OK) {
     (void)hipDeviceSynchronize();

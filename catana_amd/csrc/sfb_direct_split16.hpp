@@ -366,12 +366,15 @@ __global__ __launch_bounds__(256, 2) void sfb_direct_split16_kernel(PointsSoA pt
 // DirectItem::flags of the grouped kernel's items: the point split KS
 enum : int32_t { ITEM_SPLIT_KS2 = 2, ITEM_SPLIT_KS4 = 4 };
 
-// Persistent grouped launch of split16_item: every block pulls items from
-// *counter (zeroed before the launch) until the list is exhausted and runs
-// each with the class its m-chunk count (1, 2 or up to 4) and point split
-// (flags ITEM_SPLIT_KS2 / ITEM_SPLIT_KS4) select, so that items of every class
-// and catalogue share one launch with no tail per class.  Every block leaves
-// the loop once the counter passes nitems.
+// Persistent grouped launch of split16_item for the items of one point split
+// KS: every block pulls items from *counter (zeroed before the launch) until
+// the list is exhausted and runs each with the class its m-chunk count (1, 2
+// or up to 4) selects, so that items of every chunk count and catalogue share
+// one launch with no tail per class.  Every block leaves the loop once the
+// counter passes nitems.  One kernel per KS: a single kernel over all nine
+// (chunks, KS) classes needs more than the 256 registers of two blocks per CU
+// and spills (196 bytes per lane), one KS with its three chunk counts does not.
+template <int KS>
 __global__ __launch_bounds__(256, 2) void sfb_direct_split16_grouped_kernel(
     PointsSoA pts, const DirectItem *__restrict__ items, int nitems, int *__restrict__ counter,
     const double *__restrict__ kln, LegendreScaled leg, CubicTables ct, double *__restrict__ partial) {
@@ -390,20 +393,9 @@ __global__ __launch_bounds__(256, 2) void sfb_direct_split16_grouped_kernel(
     if (i >= nitems) break;
     const DirectItem it = items[i];
     const int nq = it.qhi - it.qlo;
-    const int ks = it.flags & (ITEM_SPLIT_KS2 | ITEM_SPLIT_KS4);
-    if (ks == ITEM_SPLIT_KS4) {
-      if (nq == 1) split16_item<1, 4>(it, pts, kln, leg, ct, partial, sh);
-      else if (nq == 2) split16_item<2, 4>(it, pts, kln, leg, ct, partial, sh);
-      else split16_item<4, 4>(it, pts, kln, leg, ct, partial, sh);
-    } else if (ks == ITEM_SPLIT_KS2) {
-      if (nq == 1) split16_item<1, 2>(it, pts, kln, leg, ct, partial, sh);
-      else if (nq == 2) split16_item<2, 2>(it, pts, kln, leg, ct, partial, sh);
-      else split16_item<4, 2>(it, pts, kln, leg, ct, partial, sh);
-    } else {
-      if (nq == 1) split16_item<1, 1>(it, pts, kln, leg, ct, partial, sh);
-      else if (nq == 2) split16_item<2, 1>(it, pts, kln, leg, ct, partial, sh);
-      else split16_item<4, 1>(it, pts, kln, leg, ct, partial, sh);
-    }
+    if (nq == 1) split16_item<1, KS>(it, pts, kln, leg, ct, partial, sh);
+    else if (nq == 2) split16_item<2, KS>(it, pts, kln, leg, ct, partial, sh);
+    else split16_item<4, KS>(it, pts, kln, leg, ct, partial, sh);
   }
 }
 

@@ -2,8 +2,8 @@
 //
 // The reference (sfb_reverse_fft.cpp:115-153, :174-209) evaluates Y_lm once per
 // ring, and for every (l, n) builds the map F(p) = sum_{r in p} j_l(k_ln r),
-// Fourier-transforms each ring and sums ring by ring.  Here the same three steps
-// are three kernels:
+// Fourier-transforms each ring and sums ring by ring.  Here the same steps are
+// these kernels:
 //   1. ring_legendre_kernel: P_l^m(cos theta) once per (l, ring) -- 4 nside - 1
 //      FP64 recurrences per l instead of one per pixel;
 //   2. ring_maps_kernel: the FP32 maps F_l(n, p), streamed from the CSR radii,
@@ -16,7 +16,13 @@
 //      with the maps (A operand, read from HBM) and Y* = P_ring e^{-i m phi_j}
 //      (B operand, the ring table times the pixel azimuth, FP64 Chebyshev
 //      recurrence) split into FP16 pieces: v_mfma_f32_16x16x32_f16, three
-//      products, FP32 partial sums folded into FP64 every 64 pixels.
+//      products, FP32 partial sums folded into FP64 every 64 pixels;
+//   4. healpix_ring_eq_kernel: the same contraction for the equatorial tiles
+//      (nside a multiple of 128: every 256-pixel tile there lies in one ring of
+//      4 nside pixels at phi_j = (2 j + s) pi / (4 nside)): Y* is e^{-i m phi}
+//      alone, read from an LDS table of the 8 nside factors e^{-2 pi i k /
+//      (8 nside)}, and the ring's Legendre values scale the tile's FP64 sums
+//      per column -- no per-pixel recurrence.
 // With alias_compat the m whose folded FFT bin the reference reads without
 // conjugation ((m mod n_ring) > n_ring / 2) use the mirrored azimuth
 // phi' = 2 phi_0 - phi, as the FP64 pixel kernel does (YlmDownPix).
