@@ -340,6 +340,7 @@ struct DirectPlan {
   std::vector<DirectItem> items[18];
   std::vector<ReduceGroup> groups;
   int64_t partial_size = 0;
+  int64_t chunk = 0;  // length of the point ranges (uniform splits; the last may be shorter)
 };
 
 // Where one catalogue's items and groups point: its k_ln rows (k_{l,n} at
@@ -386,6 +387,7 @@ void plan_direct(const std::vector<int32_t> &nl, int lmin, int lmax, int64_t npt
     }
   }
   const int64_t S = (int64_t)r0.size();
+  pl->chunk = cuts ? 0 : (S > 0 ? r1[0] - r0[0] : 0);
   for (int l = lmax - 1; l >= lmin; --l) {
     const int Q = l / mc + 1;
     const int ncol = 2 * (l + 1);
@@ -1131,6 +1133,50 @@ sfbhip_status launch_split16_ks(int ksi, int nitems, const DirectItem *items, hi
 // All classes of a 256-thread mixed plan (plan_direct, uploaded in class
 // order) on the grouped-Bessel kernel, dealt in turn to the call's stream and
 // the two side streams.
+// Items of one class reordered for the chip's eight XCDs (each with its own
+// L2): the dispatcher hands workgroup i to XCD i mod 8, so position 8 k + x
+// takes the k-th item of the point ranges given to XCD x (range rank mod 8),
+// range after range, each range's items in their planned order (l
+// descending).  The blocks resident on one XCD then work on one point range
+// across many l, so its records are read from HBM about once per XCD instead
+// of once per item.
+constexpr int GFX950_XCDS = 8;
+void xcd_order(std::vector<DirectItem> &v, int64_t chunk) {
+  if (v.size() < 2 || chunk <= 0) return;
+  // lane x: the items of the ranges r with r mod 8 = x, range after range
+  // (the plan emits a range's items in l order; a stable pass keeps it)
+  int64_t rmax = 0;
+  for (const DirectItem &it : v) rmax = std::max(rmax, it.p0 / chunk);
+  std::vector<int64_t> cnt(rmax + 2, 0);
+  for (const DirectItem &it : v) ++cnt[it.p0 / chunk + 1];
+  for (int64_t r = 0; r <= rmax; ++r) cnt[r + 1] += cnt[r];
+  std::vector<DirectItem> byr(v.size());
+  for (const DirectItem &it : v) byr[cnt[it.p0 / chunk]++] = it;  // cnt[r] now = end of range r
+  // lane starts/ends in byr: ranges are contiguous there, lanes interleave them
+  std::vector<DirectItem> out;
+  out.reserve(v.size());
+  std::vector<int64_t> rr(GFX950_XCDS), pos(GFX950_XCDS);
+  for (int x = 0; x < GFX950_XCDS; ++x) {
+    rr[x] = x;
+    pos[x] = x == 0 ? 0 : (x <= rmax ? cnt[x - 1] : (int64_t)v.size());
+  }
+  for (bool any = true; any;) {
+    any = false;
+    for (int x = 0; x < GFX950_XCDS; ++x) {
+      // advance lane x past exhausted ranges
+      while (rr[x] <= rmax && pos[x] >= cnt[rr[x]]) {
+        rr[x] += GFX950_XCDS;
+        if (rr[x] <= rmax) pos[x] = rr[x] == 0 ? 0 : cnt[rr[x] - 1];
+      }
+      if (rr[x] <= rmax) {
+        out.push_back(byr[pos[x]++]);
+        any = true;
+      }
+    }
+  }
+  v.swap(out);
+}
+
 sfbhip_status launch_split16_classes(sfbhip_handle *h, const DirectPlan &pl, DirectItem *ditems, size_t *off,
                                      hipStream_t st, const PointsSoA &pts, const double *dk, const LegendreScaled &ls,
                                      const CubicTables &ct, double *partial, bool *used) {
@@ -1685,6 +1731,8 @@ sfbhip_status sfbhip_direct(sfbhip_handle *h, const double *d_points, int64_t n_
     else
       plan_direct(h->kc.nl, lsplit, p->lmax, n_points, MX_NT, MX_TP, MX_MC, mixed_qcap(p),
                   target, cx, &plmx, mixed_ks_cap(p->mixed_operands));
+    if (s16)
+      for (auto &v : plmx.items) xcd_order(v, plmx.chunk);
   }
   std::vector<ReduceGroup> groups = pllw.groups;
   groups.insert(groups.end(), pl64.groups.begin(), pl64.groups.end());
