@@ -341,6 +341,7 @@ struct DirectPlan {
   std::vector<ReduceGroup> groups;
   int64_t partial_size = 0;
   int64_t chunk = 0;  // length of the point ranges (uniform splits; the last may be shorter)
+  std::vector<int64_t> starts;  // first point of each range, ascending
 };
 
 // Where one catalogue's items and groups point: its k_ln rows (k_{l,n} at
@@ -388,6 +389,8 @@ void plan_direct(const std::vector<int32_t> &nl, int lmin, int lmax, int64_t npt
   }
   const int64_t S = (int64_t)r0.size();
   pl->chunk = cuts ? 0 : (S > 0 ? r1[0] - r0[0] : 0);
+  pl->starts.resize(S);
+  for (int64_t k = 0; k < S; ++k) pl->starts[k] = cx.pbase + r0[k];
   for (int l = lmax - 1; l >= lmin; --l) {
     const int Q = l / mc + 1;
     const int ncol = 2 * (l + 1);
@@ -1141,40 +1144,67 @@ sfbhip_status launch_split16_ks(int ksi, int nitems, const DirectItem *items, hi
 // across many l, so its records are read from HBM about once per XCD instead
 // of once per item.
 constexpr int GFX950_XCDS = 8;
-void xcd_order(std::vector<DirectItem> &v, int64_t chunk) {
-  if (v.size() < 2 || chunk <= 0) return;
-  // lane x: the items of the ranges r with r mod 8 = x, range after range
-  // (the plan emits a range's items in l order; a stable pass keeps it)
-  int64_t rmax = 0;
-  for (const DirectItem &it : v) rmax = std::max(rmax, it.p0 / chunk);
-  std::vector<int64_t> cnt(rmax + 2, 0);
-  for (const DirectItem &it : v) ++cnt[it.p0 / chunk + 1];
-  for (int64_t r = 0; r <= rmax; ++r) cnt[r + 1] += cnt[r];
-  std::vector<DirectItem> byr(v.size());
-  for (const DirectItem &it : v) byr[cnt[it.p0 / chunk]++] = it;  // cnt[r] now = end of range r
-  // lane starts/ends in byr: ranges are contiguous there, lanes interleave them
-  std::vector<DirectItem> out;
-  out.reserve(v.size());
-  std::vector<int64_t> rr(GFX950_XCDS), pos(GFX950_XCDS);
-  for (int x = 0; x < GFX950_XCDS; ++x) {
-    rr[x] = x;
-    pos[x] = x == 0 ? 0 : (x <= rmax ? cnt[x - 1] : (int64_t)v.size());
+// Items whose point (pixel) range starts at p0: with chunk > 0 the ranges are
+// [r chunk, (r + 1) chunk) and r = p0 / chunk; otherwise r is the index of p0
+// in starts (the plan's ascending range starts) or, without them, its rank
+// among the items' distinct starts.  Appends v to dst in XCD order.
+template <class Item>
+void append_xcd(std::vector<Item> &dst, const std::vector<Item> &v, int64_t chunk,
+                const std::vector<int64_t> *plan_starts = nullptr) {
+  const size_t n = v.size();
+  if (n < 2) {
+    dst.insert(dst.end(), v.begin(), v.end());
+    return;
   }
-  for (bool any = true; any;) {
-    any = false;
-    for (int x = 0; x < GFX950_XCDS; ++x) {
-      // advance lane x past exhausted ranges
-      while (rr[x] <= rmax && pos[x] >= cnt[rr[x]]) {
-        rr[x] += GFX950_XCDS;
-        if (rr[x] <= rmax) pos[x] = rr[x] == 0 ? 0 : cnt[rr[x] - 1];
-      }
-      if (rr[x] <= rmax) {
-        out.push_back(byr[pos[x]++]);
-        any = true;
-      }
+  std::vector<int64_t> r(n);
+  if (chunk > 0) {
+    // p0 is a multiple of chunk: the rounded quotient in FP64 is exact
+    const double inv = 1.0 / (double)chunk;
+    for (size_t i = 0; i < n; ++i) r[i] = (int64_t)((double)v[i].p0 * inv + 0.5);
+  } else {
+    std::vector<int64_t> own;
+    if (!plan_starts) {
+      own.resize(n);
+      for (size_t i = 0; i < n; ++i) own[i] = v[i].p0;
+      std::sort(own.begin(), own.end());
+      own.erase(std::unique(own.begin(), own.end()), own.end());
     }
+    const std::vector<int64_t> &st = plan_starts ? *plan_starts : own;
+    for (size_t i = 0; i < n; ++i) r[i] = std::lower_bound(st.begin(), st.end(), v[i].p0) - st.begin();
   }
-  v.swap(out);
+  int64_t rmax = 0;
+  for (size_t i = 0; i < n; ++i) rmax = std::max(rmax, r[i]);
+  // the items of each range in plan order (counting sort): range q holds
+  // byr[beg[q] .. beg[q + 1])
+  std::vector<int64_t> beg(rmax + 2, 0);
+  for (size_t i = 0; i < n; ++i) ++beg[r[i] + 1];
+  for (int64_t q = 0; q <= rmax; ++q) beg[q + 1] += beg[q];
+  std::vector<int32_t> byr(n);
+  {
+    std::vector<int64_t> fill(beg.begin(), beg.end() - 1);
+    for (size_t i = 0; i < n; ++i) byr[fill[r[i]]++] = (int32_t)i;
+  }
+  // lane x: ranges x, x + 8, ... in order; the output interleaves the lanes
+  // round-robin, skipping exhausted ones, so lane x's j-th item lands after
+  // sum_y min(len_y, j) items and the lanes y < x that still have a j-th
+  int64_t len[GFX950_XCDS] = {0};
+  for (int64_t q = 0; q <= rmax; ++q) len[q % GFX950_XCDS] += beg[q + 1] - beg[q];
+  const int64_t lmin = *std::min_element(len, len + GFX950_XCDS);
+  std::vector<int32_t> perm(n);  // output position -> item
+  for (int x = 0; x < GFX950_XCDS; ++x) {
+    int64_t j = 0;
+    for (int64_t q = x; q <= rmax; q += GFX950_XCDS)
+      for (int64_t k = beg[q]; k < beg[q + 1]; ++k, ++j) {
+        int64_t pos = GFX950_XCDS * j + x;  // while every lane still has items
+        if (j >= lmin) {
+          pos = 0;
+          for (int y = 0; y < GFX950_XCDS; ++y) pos += std::min(len[y], j) + (y < x && len[y] > j ? 1 : 0);
+        }
+        perm[pos] = byr[k];
+      }
+  }
+  dst.reserve(dst.size() + n);
+  for (size_t k = 0; k < n; ++k) dst.push_back(v[perm[k]]);
 }
 
 sfbhip_status launch_split16_classes(sfbhip_handle *h, const DirectPlan &pl, DirectItem *ditems, size_t *off,
@@ -1399,15 +1429,13 @@ sfbhip_status healpix_ring(sfbhip_handle *h, const int64_t *d_offsets, const dou
     if (!pb.items[3].empty() || !pb.items[4].empty()) return fail(SFBHIP_NOT_SUPPORTED, "ring plan class");
     for (int e = 0; e < 2; ++e)
       for (int ci = 0; ci < 3; ++ci) {
-        int n = 0;
+        std::vector<DirectItem> cls;
         for (const DirectItem &it : pb.items[ci]) {
           const bool is_eq = eq_path && it.p0 >= eq_lo && it.p1 <= eq_hi;
-          if (is_eq == (e == 1)) {
-            bitems[b].push_back(it);
-            ++n;
-          }
+          if (is_eq == (e == 1)) cls.push_back(it);
         }
-        bcounts[b][3 * e + ci] = n;
+        append_xcd(bitems[b], cls, 0, &pb.starts);
+        bcounts[b][3 * e + ci] = (int)cls.size();
       }
   }
   double *partial;
@@ -1475,6 +1503,11 @@ sfbhip_status healpix_ring(sfbhip_handle *h, const int64_t *d_offsets, const dou
           }
       }
       nodes = std::min(nodes, HR_LDS_NODES);
+      {
+        std::vector<MapItem> ordered;
+        append_xcd(ordered, mitems, chunk);
+        mitems.swap(ordered);
+      }
       MapItem *dmi;
       DirectItem *ditems;
       CHK(get_buf(h, "hr_mitems", std::max<size_t>(mitems.size(), 1), &dmi));
@@ -1731,8 +1764,6 @@ sfbhip_status sfbhip_direct(sfbhip_handle *h, const double *d_points, int64_t n_
     else
       plan_direct(h->kc.nl, lsplit, p->lmax, n_points, MX_NT, MX_TP, MX_MC, mixed_qcap(p),
                   target, cx, &plmx, mixed_ks_cap(p->mixed_operands));
-    if (s16)
-      for (auto &v : plmx.items) xcd_order(v, plmx.chunk);
   }
   std::vector<ReduceGroup> groups = pllw.groups;
   groups.insert(groups.end(), pl64.groups.begin(), pl64.groups.end());
@@ -1742,7 +1773,12 @@ sfbhip_status sfbhip_direct(sfbhip_handle *h, const double *d_points, int64_t n_
   std::vector<DirectItem> all;
   for (auto &v : pllw.items) all.insert(all.end(), v.begin(), v.end());
   for (auto &v : pl64.items) all.insert(all.end(), v.begin(), v.end());
-  for (auto &v : plmx.items) all.insert(all.end(), v.begin(), v.end());
+  // split-FP16 classes in XCD order (each class keeps its place in the list)
+  for (auto &v : plmx.items)
+    if (mixed && s16)
+      append_xcd(all, v, plmx.chunk);
+    else
+      all.insert(all.end(), v.begin(), v.end());
   DirectItem *ditems;
   CHK(get_buf(h, "items", std::max<size_t>(all.size(), 1), &ditems));
   if (!all.empty())
