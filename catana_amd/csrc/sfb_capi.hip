@@ -2680,7 +2680,7 @@ struct sfbhip_batch_plan {
   int *h_err = nullptr;            // pinned copy of the error flags
   int timed = 0;                   // last run: 0 none, 1 eager enqueue (phases), 2 graph launch (total)
   // device data
-  int *derr = nullptr;             // [0] error bits, [4] work counter of the grouped launch
+  int *derr = nullptr;             // [0] error bits, [8 + 8 k + x] counter of lane x of grouped launch k
   double *soa = nullptr;           // point table in radius order (what the kernels read)
   double *soa_in = nullptr;        // point table in input order
   int64_t ntot = 0;
@@ -2703,6 +2703,7 @@ struct sfbhip_batch_plan {
   int n64[5] = {0, 0, 0, 0, 0};
   int nmx = 0;
   int nmx_ks[3] = {0, 0, 0};   // grouped items with point split 1, 2, 4 (in this order after the FP64 items)
+  GroupedLanes lanes_ks[3] = {};  // their XCD lanes
   int grid_ks[3] = {1, 1, 1};
   ReduceGroup *dgroups = nullptr;
   int ngroups = 0, red_chunks = 1;
@@ -2813,7 +2814,7 @@ double item_cost(const DirectItem &it) {
 
 sfbhip_status plan_enqueue(sfbhip_batch_plan *pl, hipStream_t st, bool timed) {
   if (timed) HIPCHK(hipEventRecord(pl->ev[0], st));
-  fill_int_kernel<<<1, 64, 0, st>>>(pl->derr, 8, 0);
+  fill_int_kernel<<<1, 64, 0, st>>>(pl->derr, 8 + 3 * GROUPED_LANES, 0);
   HIPCHK(hipGetLastError());
   const dim3 gcat((unsigned)pl->prep_bx, (unsigned)pl->ncat);
   prep_points_batch_kernel<<<gcat, 256, 0, st>>>(pl->dprep, pl->soa_in, pl->ntot, pl->keys[0], pl->derr);
@@ -2865,12 +2866,13 @@ sfbhip_status plan_enqueue(sfbhip_batch_plan *pl, hipStream_t st, bool timed) {
   for (int k = 0; k < 3; ++k) {
     const int n = pl->nmx_ks[k];
     const DirectItem *di = pl->ditems + off;
-    int *ctr = pl->derr + 4 + k;
+    int *ctr = pl->derr + 8 + GROUPED_LANES * k;
+    const GroupedLanes &ln = pl->lanes_ks[k];
     const unsigned gx = (unsigned)pl->grid_ks[k];
     const hipStream_t s = k == 0 ? st : pl->side[k - 1];
-    if (n && k == 0) sfb_direct_split16_grouped_kernel<1><<<gx, 256, 0, s>>>(pts, di, n, ctr, pl->dk, pl->ls, pl->ct, pl->partial);
-    if (n && k == 1) sfb_direct_split16_grouped_kernel<2><<<gx, 256, 0, s>>>(pts, di, n, ctr, pl->dk, pl->ls, pl->ct, pl->partial);
-    if (n && k == 2) sfb_direct_split16_grouped_kernel<4><<<gx, 256, 0, s>>>(pts, di, n, ctr, pl->dk, pl->ls, pl->ct, pl->partial);
+    if (n && k == 0) sfb_direct_split16_grouped_kernel<1><<<gx, 256, 0, s>>>(pts, di, ln, ctr, pl->dk, pl->ls, pl->ct, pl->partial);
+    if (n && k == 1) sfb_direct_split16_grouped_kernel<2><<<gx, 256, 0, s>>>(pts, di, ln, ctr, pl->dk, pl->ls, pl->ct, pl->partial);
+    if (n && k == 2) sfb_direct_split16_grouped_kernel<4><<<gx, 256, 0, s>>>(pts, di, ln, ctr, pl->dk, pl->ls, pl->ct, pl->partial);
     HIPCHK(hipGetLastError());
     off += n;
   }
@@ -3106,10 +3108,24 @@ sfbhip_status plan_create(sfbhip_handle *h, const sfbhip_catalog *cats, int32_t 
   {
     std::vector<DirectItem> byks;
     const int32_t ksf[3] = {0, ITEM_SPLIT_KS2, ITEM_SPLIT_KS4};
+    // ... and each part in XCD lanes: lane x the items of the point ranges of
+    // rank x mod 8 (ranks of the ranges' starts over all catalogues)
+    std::vector<int64_t> starts;
+    for (const DirectItem &it : mx) starts.push_back(it.p0);
+    std::sort(starts.begin(), starts.end());
+    starts.erase(std::unique(starts.begin(), starts.end()), starts.end());
     for (int k = 0; k < 3; ++k) {
+      GroupedLanes &ln = pl->lanes_ks[k];
       const size_t before = byks.size();
-      for (const DirectItem &it : mx)
-        if ((it.flags & (ITEM_SPLIT_KS2 | ITEM_SPLIT_KS4)) == ksf[k]) byks.push_back(it);
+      for (int x = 0; x < GROUPED_LANES; ++x) {
+        ln.off[x] = (int32_t)(byks.size() - before);
+        for (const DirectItem &it : mx) {
+          if ((it.flags & (ITEM_SPLIT_KS2 | ITEM_SPLIT_KS4)) != ksf[k]) continue;
+          const int64_t r = std::lower_bound(starts.begin(), starts.end(), it.p0) - starts.begin();
+          if (r % GROUPED_LANES == x) byks.push_back(it);
+        }
+      }
+      ln.off[GROUPED_LANES] = (int32_t)(byks.size() - before);
       pl->nmx_ks[k] = (int)(byks.size() - before);
     }
     mx.swap(byks);
@@ -3142,7 +3158,7 @@ sfbhip_status plan_create(sfbhip_handle *h, const sfbhip_catalog *cats, int32_t 
     CHK(plan_alloc(pl, (size_t)ntot, &pl->keys[k]));
     CHK(plan_alloc(pl, (size_t)ntot, &pl->idx[k]));
   }
-  CHK(plan_alloc(pl, 8, &pl->derr));
+  CHK(plan_alloc(pl, 8 + 3 * GROUPED_LANES, &pl->derr));
   {
     std::vector<SortSeg> seg(ncat);
     int64_t hb = 0;

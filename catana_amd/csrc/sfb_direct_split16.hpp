@@ -366,17 +366,28 @@ __global__ __launch_bounds__(256, 2) void sfb_direct_split16_kernel(PointsSoA pt
 // DirectItem::flags of the grouped kernel's items: the point split KS
 enum : int32_t { ITEM_SPLIT_KS2 = 2, ITEM_SPLIT_KS4 = 4 };
 
+// Work lanes of a persistent grouped launch: lane x holds items
+// [off[x], off[x + 1]) of the launch's list and counter x hands them out.
+constexpr int GROUPED_LANES = 8;
+struct GroupedLanes {
+  int32_t off[GROUPED_LANES + 1];
+};
+
 // Persistent grouped launch of split16_item for the items of one point split
-// KS: every block pulls items from *counter (zeroed before the launch) until
-// the list is exhausted and runs each with the class its m-chunk count (1, 2
-// or up to 4) selects, so that items of every chunk count and catalogue share
-// one launch with no tail per class.  Every block leaves the loop once the
-// counter passes nitems.  One kernel per KS: a single kernel over all nine
-// (chunks, KS) classes needs more than the 256 registers of two blocks per CU
-// and spills (196 bytes per lane), one KS with its three chunk counts does not.
+// KS.  The items come in GROUPED_LANES lanes, one per XCD (the dispatcher
+// places workgroup b on XCD b mod 8, and lane x holds the items of every
+// eighth point range, so the blocks of one XCD share their points in its L2):
+// a block pulls items of its own lane from that lane's counter (zeroed before
+// the launch), then helps the other lanes in turn, and leaves once it has
+// found all of them exhausted.  Each item runs with the class its m-chunk
+// count (1, 2 or up to 4) selects, so items of every chunk count and
+// catalogue share one launch with no tail per class.  One kernel per KS: a
+// single kernel over all nine (chunks, KS) classes needs more than the 256
+// registers of two blocks per CU and spills (196 bytes per lane), one KS with
+// its three chunk counts does not.
 template <int KS>
 __global__ __launch_bounds__(256, 2) void sfb_direct_split16_grouped_kernel(
-    PointsSoA pts, const DirectItem *__restrict__ items, int nitems, int *__restrict__ counter,
+    PointsSoA pts, const DirectItem *__restrict__ items, GroupedLanes lanes, int *__restrict__ counters,
     const double *__restrict__ kln, LegendreScaled leg, CubicTables ct, double *__restrict__ partial) {
   __shared__ __attribute__((aligned(16))) unsigned short ybuf[S16_YBUF];
   __shared__ double rs[MX_TP];
@@ -385,13 +396,18 @@ __global__ __launch_bounds__(256, 2) void sfb_direct_split16_grouped_kernel(
   __shared__ double cgs[S16_GPAD + MX_LMAX];
   __shared__ int next;
   const S16Shared sh{ybuf, rs, dg, gs, cgs};
-  for (;;) {
-    if (threadIdx.x == 0) next = atomicAdd(counter, 1);
+  int x = (int)(blockIdx.x % GROUPED_LANES);
+  for (int tried = 0; tried < GROUPED_LANES;) {
+    if (threadIdx.x == 0) next = atomicAdd(counters + x, 1);
     __syncthreads();  // next written; the previous item's LDS readers are done
     const int i = __builtin_amdgcn_readfirstlane(next);
     __syncthreads();  // every thread has read next before thread 0 overwrites it
-    if (i >= nitems) break;
-    const DirectItem it = items[i];
+    if (i >= lanes.off[x + 1] - lanes.off[x]) {  // lane exhausted: the next one
+      x = x + 1 == GROUPED_LANES ? 0 : x + 1;
+      ++tried;
+      continue;
+    }
+    const DirectItem it = items[lanes.off[x] + i];
     const int nq = it.qhi - it.qlo;
     if (nq == 1) split16_item<1, KS>(it, pts, kln, leg, ct, partial, sh);
     else if (nq == 2) split16_item<2, KS>(it, pts, kln, leg, ct, partial, sh);
