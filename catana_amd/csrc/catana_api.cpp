@@ -147,7 +147,6 @@ KClkk run_healpix(const PixelizedPointContainer &pix, unsigned short lmax, unsig
     hip_check(hipMemcpy(c.rad.p, rad.data(), rad.size() * sizeof(double), hipMemcpyHostToDevice), "copy radii");
   reserve_outputs(c, lmax, nmax, store);
   sfbhip_params p = make_params(lmax, nmax, rmax, verbose, o);
-  p.precision = SFBHIP_PREC_FP64;
   throw_status(sfbhip_healpix(c.h, static_cast<const int64_t *>(c.off.p), static_cast<const double *>(c.rad.p),
                               (int32_t)pix.get_nside(), &p, static_cast<double *>(c.kln.p),
                               static_cast<double *>(c.cln.p), store ? static_cast<double *>(c.flmn.p) : nullptr,
@@ -586,8 +585,10 @@ std::vector<KClkk> sfb_decomposition_reverse_batched(const std::vector<CatalogSp
 }
 
 KClkk sfb_decomposition_healpix(const PixelizedPointContainer &pix, unsigned short lmax, double k_max, double rmax,
-                                unsigned short nmax, double window_volume, bool store_flmn, bool verbose) {
+                                unsigned short nmax, double window_volume, bool store_flmn, bool verbose,
+                                int precision) {
   Options o;
+  o.precision = precision;
   o.interpolated = 0;
   o.k_max = k_max;
   o.window_volume = window_volume;
@@ -596,9 +597,9 @@ KClkk sfb_decomposition_healpix(const PixelizedPointContainer &pix, unsigned sho
 
 KClkk sfb_decomposition_healpix(const PointContainer &points, unsigned int nside, unsigned short lmax, double k_max,
                                 double rmax, unsigned short nmax, double window_volume, bool store_flmn,
-                                bool verbose) {
+                                bool verbose, int precision) {
   return sfb_decomposition_healpix(PixelizedPointContainer(nside, points), lmax, k_max, rmax, nmax, window_volume,
-                                   store_flmn, verbose);
+                                   store_flmn, verbose, precision);
 }
 
 }  // namespace catana
@@ -1210,7 +1211,6 @@ KClkk Analyzer::compute_sfb_pixelized(unsigned short lmax, unsigned short nmax, 
   reserve_outputs(c, lmax, nmax, store_flmn);
   Options o;
   sfbhip_params p = make_params(lmax, nmax, rmax, verbose, o);
-  p.precision = SFBHIP_PREC_FP64;
   throw_status(sfbhip_healpix(c.h, static_cast<const int64_t *>(c.off.p), static_cast<const double *>(c.rad.p),
                               (int32_t)nside, &p, static_cast<double *>(c.kln.p), static_cast<double *>(c.cln.p),
                               store_flmn ? static_cast<double *>(c.flmn.p) : nullptr, nullptr),

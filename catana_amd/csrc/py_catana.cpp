@@ -230,21 +230,27 @@ PYBIND11_MODULE(_catana, m) {
   m.def(
       "sfb_decomposition_healpix",
       [](const PixelizedPointContainer &pc, unsigned short lmax, double k_max, double rmax, unsigned short nmax,
-         double window_volume, bool store, bool verbose) {
-        return sfb_decomposition_healpix(pc, lmax, k_max, rmax, nmax, window_volume, store, verbose);
+         double window_volume, bool store, bool verbose, const std::string &precision) {
+        const int pr = precision_code(precision);
+        py::gil_scoped_release rel;
+        return sfb_decomposition_healpix(pc, lmax, k_max, rmax, nmax, window_volume, store, verbose, pr);
       },
       py::arg("pixelized_point_container"), py::arg("lmax"), py::arg("k_max"), py::arg("rmax"), py::arg("nmax"),
       py::arg("window_volume") = 0.0, py::arg("store_flmn") = false, py::arg("verbose") = true,
-      py::call_guard<py::gil_scoped_release>());
+      py::arg("precision") = "fp64",
+      "HEALPix SFB with per-l truncation and an optional window volume.  precision 'fp64' (the pixel path) or "
+      "'mixed' (the ring path: FP32 maps, split-FP16 ring contraction).");
   m.def(
       "sfb_decomposition_healpix",
       [](const PointContainer &pc, unsigned int nside, unsigned short lmax, double k_max, double rmax,
-         unsigned short nmax, double window_volume, bool store, bool verbose) {
-        return sfb_decomposition_healpix(pc, nside, lmax, k_max, rmax, nmax, window_volume, store, verbose);
+         unsigned short nmax, double window_volume, bool store, bool verbose, const std::string &precision) {
+        const int pr = precision_code(precision);
+        py::gil_scoped_release rel;
+        return sfb_decomposition_healpix(pc, nside, lmax, k_max, rmax, nmax, window_volume, store, verbose, pr);
       },
       py::arg("point_container"), py::arg("nside"), py::arg("lmax"), py::arg("k_max"), py::arg("rmax"),
       py::arg("nmax"), py::arg("window_volume") = 0.0, py::arg("store_flmn") = false, py::arg("verbose") = true,
-      py::call_guard<py::gil_scoped_release>());
+      py::arg("precision") = "fp64");
 
   m.def(
       "sfb_decomposition_reverse_batched",

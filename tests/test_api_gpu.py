@@ -157,6 +157,13 @@ def test_k_max_entry_points():
     np.testing.assert_array_equal(h1.c_ln, h2.c_ln)
     np.testing.assert_allclose(h1.c_ln, co, rtol=1e-10)
     assert _flmn_err(h1.f_lmn, fo) < 1e-10
+    # the ring path through the same entry point (precision 'mixed'); l < 2 run in FP64,
+    # the stated tolerances are 1e-6 per-l f_lmn and 5e-6 C_l below l = 8
+    hm = catana.sfb_decomposition_healpix(pc, 8, lmax, k_max, rmax, nmax, vol, True, False, "mixed")
+    nz = h1.c_ln > 0
+    assert np.all(hm.c_ln[~nz] == 0)
+    assert np.max(np.abs(hm.c_ln[nz] / h1.c_ln[nz] - 1)) < 5e-6
+    assert _flmn_err(hm.f_lmn, h1.f_lmn) < 1e-6
 
 
 def test_error_types():
