@@ -11,6 +11,7 @@
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
+#include <memory>
 #include <array>
 #include <cmath>
 #include <cstdio>
@@ -96,6 +97,10 @@ struct KlnCache {
   std::vector<int32_t> cub_nint;
 };
 
+namespace {
+struct RingPlanCache;
+}
+
 struct sfbhip_handle {
   int device = 0;
   std::map<std::string, DevBuf> bufs;
@@ -114,6 +119,7 @@ struct sfbhip_handle {
   int lds_max = 65536;  // largest LDS allocation of one block (bytes)
   int pix_nside = -1;  // pixel tables on the device are for this nside
   int ring_nside = -1;  // ring tables (cos/sin theta per ring, ring of each pixel) are for this nside
+  std::shared_ptr<RingPlanCache> ring_plan;  // the last single-batch ring plan and its shape
   int rleg_nside = -1, rleg_lmax = -1;  // ring Legendre table is for this (nside, lmax)
   int ig_L = -1, ig_nz = -1;  // zero rows of the sbessel integrals on the device
   int ncu = 256;  // compute units of the device
@@ -342,6 +348,21 @@ struct DirectPlan {
   int64_t partial_size = 0;
   int64_t chunk = 0;  // length of the point ranges (uniform splits; the last may be shorter)
   std::vector<int64_t> starts;  // first point of each range, ascending
+};
+
+// The last single-batch plan of the ring path: its work items are on the device
+// (hr_lowl_items, hr_items, hr_mitems, hr_moff), so a call of the same shape
+// plans and uploads nothing.  The shape: nside, lmax, nmax, the FP64 split,
+// the equatorial path and n_l.
+struct RingPlanCache {
+  int32_t nside = 0, L = 0, nmax = 0, lsplit = 0;
+  bool eq = false;
+  std::vector<int32_t> nl;
+  DirectPlan pl64;                   // the FP64 low-l classes (items on the device)
+  std::vector<ReduceGroup> groups;   // every group of the call (norm set per call)
+  int64_t partial_size = 0;
+  std::array<int, 6> counts{};       // ring item classes on the device
+  int64_t nmap = 0;                  // map items on the device
 };
 
 // Where one catalogue's items and groups point: its k_ln rows (k_{l,n} at
@@ -1404,53 +1425,144 @@ sfbhip_status healpix_ring(sfbhip_handle *h, const int64_t *d_offsets, const dou
   CHK(get_buf(h, "hr_maps", (size_t)maxmap, &maps));
   CHK(get_buf(h, "hr_moff", (size_t)std::max(L, 1), &dmoff));
 
-  // one plan over all multipoles (the partial blocks of every batch), items launched per batch
+  // Map items of batch [lb0, lb1): per l and 16 rows, pixel ranges of one
+  // chunk, appended in XCD order; moff[l] = offset of l's maps in the workspace.
+  auto map_items = [&](int lb0, int lb1, std::vector<int64_t> &moff, std::vector<MapItem> &mitems) {
+    moff.assign(std::max(L, 1), 0);
+    mitems.clear();
+    int64_t o = 0, nmaprows = 0;
+    for (int l = lb0; l < lb1; ++l) {
+      moff[l] = o;
+      o += (int64_t)nl[l] * npix;
+      nmaprows += cdiv(nl[l], HR_NB);
+    }
+    const int64_t target = 8192;
+    int64_t S = std::max<int64_t>(1, cdiv(target, std::max<int64_t>(nmaprows, 1)));
+    S = std::min<int64_t>(S, cdiv(npix, 256));
+    const int64_t chunk = cdiv(cdiv(npix, S), 256) * 256;
+    std::vector<MapItem> plain;
+    for (int l = lb0; l < lb1; ++l)
+      for (int n0 = 0; n0 < nl[l]; n0 += HR_NB)
+        for (int64_t p0 = 0; p0 < npix; p0 += chunk) {
+          MapItem mi;
+          mi.l = l;
+          mi.n0 = n0;
+          mi.nrows = std::min(HR_NB, nl[l] - n0);
+          mi.pad = 0;
+          mi.p0 = p0;
+          mi.p1 = std::min(npix, p0 + chunk);
+          mi.moff = moff[l];
+          mi.kofs = (int64_t)l * p->nmax + n0;
+          plain.push_back(mi);
+        }
+    append_xcd(mitems, plain, chunk);
+  };
+  // cubic-table intervals the map kernel stages in LDS for batch [lb0, lb1)
+  auto map_nodes = [&](int lb0, int lb1) {
+    int nodes = 1;
+    for (int l = lb0; l < lb1; ++l)
+      if (nl[l] > 0) nodes = std::max(nodes, h->kc.cub_nint[l]);
+    return std::min(nodes, HR_LDS_NODES);
+  };
+
+  // One plan over all multipoles (the partial blocks of every batch), items
+  // launched per batch.  A single-batch plan and its device copies are kept in
+  // the handle: a call of the same shape plans and uploads nothing.
   PlanCtx cx;
   cx.nmax = p->nmax;
   cx.norm = norm;
-  DirectPlan pl64, pl;
-  if (lsplit > 0) plan_direct(nl, 0, lsplit, npix, F64_NT, F64_TP, F64_MC, 16, 4096, cx, &pl64);
-  pl.partial_size = pl64.partial_size;
-  pl.groups = pl64.groups;
-  // items never straddle the equatorial boundaries, so that each is all
-  // equatorial (healpix_ring_eq_kernel) or all other (healpix_ring_contract_kernel);
-  // per batch the classes are uploaded as [other Q = 1, 2, 4][equatorial Q = 1, 2, 4]
-  const int64_t eq_lo = 2LL * nside * (nside - 1), eq_hi = npix - eq_lo;
-  const std::vector<int64_t> cuts = eq_path ? std::vector<int64_t>{0, eq_lo, eq_hi, npix} : std::vector<int64_t>{0, npix};
+  const bool single = batches.size() == 1;
+  if (!h->ring_plan) h->ring_plan = std::make_shared<RingPlanCache>();
+  RingPlanCache &rc_plan = *h->ring_plan;
+  const bool hit = single && rc_plan.nside == nside && rc_plan.L == L && rc_plan.nmax == p->nmax &&
+                   rc_plan.lsplit == lsplit && rc_plan.eq == eq_path && rc_plan.nl == nl;
+  DirectPlan pl64local, pl;
   std::vector<std::vector<DirectItem>> bitems(batches.size());
   std::vector<std::array<int, 6>> bcounts(batches.size());
-  for (size_t b = 0; b < batches.size(); ++b) {
-    DirectPlan pb;
-    pb.partial_size = pl.partial_size;
-    plan_direct(nl, batches[b].first, batches[b].second, npix, HR_NT, HR_TP, MX_MC, 4, f64_target_items(npix, true) / 2,
-                cx, &pb, 1, &cuts);
-    pl.partial_size = pb.partial_size;
-    pl.groups.insert(pl.groups.end(), pb.groups.begin(), pb.groups.end());
-    if (!pb.items[3].empty() || !pb.items[4].empty()) return fail(SFBHIP_NOT_SUPPORTED, "ring plan class");
-    for (int e = 0; e < 2; ++e)
-      for (int ci = 0; ci < 3; ++ci) {
-        std::vector<DirectItem> cls;
-        for (const DirectItem &it : pb.items[ci]) {
-          const bool is_eq = eq_path && it.p0 >= eq_lo && it.p1 <= eq_hi;
-          if (is_eq == (e == 1)) cls.push_back(it);
+  if (hit) {
+    pl.groups = rc_plan.groups;
+    for (ReduceGroup &g : pl.groups) g.norm = norm;
+    pl.partial_size = rc_plan.partial_size;
+    bcounts[0] = rc_plan.counts;
+  } else {
+    rc_plan.nside = 0;  // invalid until the new plan is uploaded
+    if (lsplit > 0) plan_direct(nl, 0, lsplit, npix, F64_NT, F64_TP, F64_MC, 16, 4096, cx, &pl64local);
+    pl.partial_size = pl64local.partial_size;
+    pl.groups = pl64local.groups;
+    // items never straddle the equatorial boundaries, so that each is all
+    // equatorial (healpix_ring_eq_kernel) or all other (healpix_ring_contract_kernel);
+    // per batch the classes are uploaded as [other Q = 1, 2, 4][equatorial Q = 1, 2, 4]
+    const int64_t eq_lo = 2LL * nside * (nside - 1), eq_hi = npix - eq_lo;
+    const std::vector<int64_t> cuts =
+        eq_path ? std::vector<int64_t>{0, eq_lo, eq_hi, npix} : std::vector<int64_t>{0, npix};
+    for (size_t b = 0; b < batches.size(); ++b) {
+      DirectPlan pb;
+      pb.partial_size = pl.partial_size;
+      plan_direct(nl, batches[b].first, batches[b].second, npix, HR_NT, HR_TP, MX_MC, 4,
+                  f64_target_items(npix, true) / 2, cx, &pb, 1, &cuts);
+      pl.partial_size = pb.partial_size;
+      pl.groups.insert(pl.groups.end(), pb.groups.begin(), pb.groups.end());
+      if (!pb.items[3].empty() || !pb.items[4].empty()) return fail(SFBHIP_NOT_SUPPORTED, "ring plan class");
+      for (int e = 0; e < 2; ++e)
+        for (int ci = 0; ci < 3; ++ci) {
+          std::vector<DirectItem> cls;
+          for (const DirectItem &it : pb.items[ci]) {
+            const bool is_eq = eq_path && it.p0 >= eq_lo && it.p1 <= eq_hi;
+            if (is_eq == (e == 1)) cls.push_back(it);
+          }
+          append_xcd(bitems[b], cls, 0, &pb.starts);
+          bcounts[b][3 * e + ci] = (int)cls.size();
         }
-        append_xcd(bitems[b], cls, 0, &pb.starts);
-        bcounts[b][3 * e + ci] = (int)cls.size();
-      }
+    }
   }
+  const DirectPlan &pl64 = hit ? rc_plan.pl64 : pl64local;
   double *partial;
   CHK(get_buf(h, "partial", (size_t)std::max<int64_t>(pl.partial_size, 1), &partial));
 
   // host arrays the asynchronous copies read stay alive until the stream is synchronised
   std::vector<DirectItem> lo;
-  for (auto &v : pl64.items) lo.insert(lo.end(), v.begin(), v.end());
+  int64_t nlo = 0;
+  for (auto &v : pl64.items) nlo += (int64_t)v.size();
   DirectItem *dlo = nullptr;
-  if (!lo.empty()) {
-    CHK(get_buf(h, "hr_lowl_items", lo.size(), &dlo));
-    HIPCHK(hipMemcpyAsync(dlo, lo.data(), sizeof(DirectItem) * lo.size(), hipMemcpyHostToDevice, st));
-  }
+  if (nlo > 0) CHK(get_buf(h, "hr_lowl_items", (size_t)nlo, &dlo));
   std::vector<std::vector<int64_t>> bmoff(batches.size());
   std::vector<std::vector<MapItem>> bmitems(batches.size());
+  MapItem *dmi = nullptr;
+  DirectItem *ditems = nullptr;
+  int64_t nmap0 = hit ? rc_plan.nmap : 0;
+  if (!hit) {
+    for (auto &v : pl64.items) lo.insert(lo.end(), v.begin(), v.end());
+    if (nlo > 0) HIPCHK(hipMemcpyAsync(dlo, lo.data(), sizeof(DirectItem) * lo.size(), hipMemcpyHostToDevice, st));
+    if (single) {
+      // the whole single-batch plan on the device now, and kept
+      map_items(batches[0].first, batches[0].second, bmoff[0], bmitems[0]);
+      nmap0 = (int64_t)bmitems[0].size();
+      CHK(get_buf(h, "hr_mitems", std::max<size_t>(bmitems[0].size(), 1), &dmi));
+      CHK(get_buf(h, "hr_items", std::max<size_t>(bitems[0].size(), 1), &ditems));
+      HIPCHK(hipMemcpyAsync(dmoff, bmoff[0].data(), sizeof(int64_t) * bmoff[0].size(), hipMemcpyHostToDevice, st));
+      if (nmap0)
+        HIPCHK(hipMemcpyAsync(dmi, bmitems[0].data(), sizeof(MapItem) * bmitems[0].size(), hipMemcpyHostToDevice, st));
+      if (!bitems[0].empty())
+        HIPCHK(hipMemcpyAsync(ditems, bitems[0].data(), sizeof(DirectItem) * bitems[0].size(), hipMemcpyHostToDevice,
+                              st));
+      rc_plan.L = L;
+      rc_plan.nmax = p->nmax;
+      rc_plan.lsplit = lsplit;
+      rc_plan.eq = eq_path;
+      rc_plan.nl = nl;
+      rc_plan.pl64 = pl64local;
+      rc_plan.groups = pl.groups;
+      rc_plan.partial_size = pl.partial_size;
+      rc_plan.counts = bcounts[0];
+      rc_plan.nmap = nmap0;
+      rc_plan.nside = nside;  // valid
+    }
+  } else {
+    CHK(get_buf(h, "hr_mitems", (size_t)std::max<int64_t>(nmap0, 1), &dmi));
+    size_t nit = 0;
+    for (int k = 0; k < 6; ++k) nit += (size_t)bcounts[0][k];
+    CHK(get_buf(h, "hr_items", std::max<size_t>(nit, 1), &ditems));
+  }
 
   // Streams: the maps and the largest contraction class run on st; l < lsplit
   // (FP64, exact j_l: some of these C_l sit at cancellation level, where even
@@ -1462,7 +1574,7 @@ sfbhip_status healpix_ring(sfbhip_handle *h, const int64_t *d_offsets, const dou
   CHK(fork_sides(h, st));
   bool used[2] = {false, false};
   sfbhip_status rc = SFBHIP_OK;
-  if (!lo.empty()) {
+  if (nlo > 0) {
     size_t off = 0;
     rc = launch_f64_classes(h, pl64, dlo, &off, st, PointsSoA{}, &pix, dk, p->nmax, lt, JL_EXACT, BesselTableRef{},
                             partial, derr, used, true);
@@ -1470,63 +1582,31 @@ sfbhip_status healpix_ring(sfbhip_handle *h, const int64_t *d_offsets, const dou
   for (size_t b = 0; b < batches.size() && rc == SFBHIP_OK; ++b) {
     rc = [&]() -> sfbhip_status {
       const int lb0 = batches[b].first, lb1 = batches[b].second;
-      // map offsets within the batch's workspace, and the map items
-      std::vector<int64_t> &moff = bmoff[b];
-      std::vector<MapItem> &mitems = bmitems[b];
-      moff.assign(std::max(L, 1), 0);
-      int64_t o = 0, nmaprows = 0;
-      for (int l = lb0; l < lb1; ++l) {
-        moff[l] = o;
-        o += (int64_t)nl[l] * npix;
-        nmaprows += cdiv(nl[l], HR_NB);
+      int64_t nmap = nmap0;
+      if (!single) {
+        // several batches: each plans and uploads its own map items and work items
+        map_items(lb0, lb1, bmoff[b], bmitems[b]);
+        nmap = (int64_t)bmitems[b].size();
+        CHK(get_buf(h, "hr_mitems", std::max<size_t>(bmitems[b].size(), 1), &dmi));
+        CHK(get_buf(h, "hr_items", std::max<size_t>(bitems[b].size(), 1), &ditems));
+        HIPCHK(hipMemcpyAsync(dmoff, bmoff[b].data(), sizeof(int64_t) * bmoff[b].size(), hipMemcpyHostToDevice, st));
+        if (nmap)
+          HIPCHK(hipMemcpyAsync(dmi, bmitems[b].data(), sizeof(MapItem) * bmitems[b].size(), hipMemcpyHostToDevice,
+                                st));
+        if (!bitems[b].empty())
+          HIPCHK(hipMemcpyAsync(ditems, bitems[b].data(), sizeof(DirectItem) * bitems[b].size(),
+                                hipMemcpyHostToDevice, st));
       }
-      const int64_t target = 8192;
-      int64_t S = std::max<int64_t>(1, cdiv(target, std::max<int64_t>(nmaprows, 1)));
-      S = std::min<int64_t>(S, cdiv(npix, 256));
-      const int64_t chunk = cdiv(cdiv(npix, S), 256) * 256;
-      int nodes = 1;
-      for (int l = lb0; l < lb1; ++l) {
-        if (nl[l] == 0) continue;
-        nodes = std::max(nodes, h->kc.cub_nint[l]);
-        for (int n0 = 0; n0 < nl[l]; n0 += HR_NB)
-          for (int64_t p0 = 0; p0 < npix; p0 += chunk) {
-            MapItem mi;
-            mi.l = l;
-            mi.n0 = n0;
-            mi.nrows = std::min(HR_NB, nl[l] - n0);
-            mi.pad = 0;
-            mi.p0 = p0;
-            mi.p1 = std::min(npix, p0 + chunk);
-            mi.moff = moff[l];
-            mi.kofs = (int64_t)l * p->nmax + n0;
-            mitems.push_back(mi);
-          }
-      }
-      nodes = std::min(nodes, HR_LDS_NODES);
-      {
-        std::vector<MapItem> ordered;
-        append_xcd(ordered, mitems, chunk);
-        mitems.swap(ordered);
-      }
-      MapItem *dmi;
-      DirectItem *ditems;
-      CHK(get_buf(h, "hr_mitems", std::max<size_t>(mitems.size(), 1), &dmi));
-      CHK(get_buf(h, "items", std::max<size_t>(bitems[b].size(), 1), &ditems));
-      HIPCHK(hipMemcpyAsync(dmoff, moff.data(), sizeof(int64_t) * moff.size(), hipMemcpyHostToDevice, st));
-      if (!mitems.empty())
-        HIPCHK(hipMemcpyAsync(dmi, mitems.data(), sizeof(MapItem) * mitems.size(), hipMemcpyHostToDevice, st));
-      if (!bitems[b].empty())
-        HIPCHK(hipMemcpyAsync(ditems, bitems[b].data(), sizeof(DirectItem) * bitems[b].size(), hipMemcpyHostToDevice,
-                              st));
+      const int nodes = map_nodes(lb0, lb1);
       const size_t shm = sizeof(float4) * (size_t)nodes;
-      if (!mitems.empty()) {
+      if (nmap) {
         HIPCHK(hipFuncSetAttribute(reinterpret_cast<const void *>(&ring_maps_kernel),
                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
-        ring_maps_kernel<<<(unsigned)mitems.size(), 256, shm, st>>>(dmi, d_offsets, slot_perm, d_radii, npix, dk, ct,
-                                                                   nodes, scale, maps);
+        ring_maps_kernel<<<(unsigned)nmap, 256, shm, st>>>(dmi, d_offsets, slot_perm, d_radii, npix, dk, ct, nodes,
+                                                           scale, maps);
         HIPCHK(hipGetLastError());
       }
-      if (batches.size() == 1) HIPCHK(hipEventRecord(h->ev[1], st));
+      if (single) HIPCHK(hipEventRecord(h->ev[1], st));
       // the contraction classes start once the maps are written: the classes
       // of 4 chunks (the largest) on st, of 2 on side stream 0, of 1 on side
       // stream 1; equatorial classes after the others of their stream
