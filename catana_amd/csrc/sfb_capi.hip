@@ -338,6 +338,22 @@ int qclass(int q) {
   return c;
 }
 
+// The m-chunk ranges [qlo, qhi) of one l's items, at most cap of its Q chunks
+// each.  The mixed-precision kernels number chunks from the top (chunk q holds
+// m = l - 16 q down) and step each point's Legendre recurrence from m = l to
+// the item's first chunk, 16 qlo steps per point of every tile: there the
+// short remainder range goes first (qlo = 0, no steps) and the full ranges
+// follow, which saves the remainder's length in steps for each of them.  The
+// FP64 kernel numbers chunks from m = 0 up and steps from l down to 16 qhi, so
+// there the remainder goes last (from_top false: the highest chunks, no steps).
+std::vector<std::pair<int, int>> chunk_ranges(int Q, int cap, bool from_top) {
+  std::vector<std::pair<int, int>> v;
+  const int r = from_top ? Q % cap : 0;
+  if (r) v.emplace_back(0, r);
+  for (int qlo = r; qlo < Q; qlo += cap) v.emplace_back(qlo, std::min(Q, qlo + cap));
+  return v;
+}
+
 // Items of a row-tiled direct kernel: groups (l, n-tile), split over point
 // ranges, with m chunks split into passes of at most qmax_cap chunks.
 struct DirectPlan {
@@ -381,10 +397,11 @@ struct PlanCtx {
 // point split KS = min(4, ks_cap), of at most 32 rows to KS = min(2, ks_cap).
 // cuts (optional): segment boundaries 0 = c_0 < c_1 < ... = npts, multiples of
 // tp; no point range of an item then crosses one (each segment gets its share
-// of the ranges).
+// of the ranges).  from_top: the kernel numbers m chunks from the top, so the
+// remainder range of chunks comes first (chunk_ranges).
 void plan_direct(const std::vector<int32_t> &nl, int lmin, int lmax, int64_t npts, int nt, int tp, int mc,
                  int qmax_cap, int64_t target_items, const PlanCtx &cx, DirectPlan *pl, int ks_cap = 1,
-                 const std::vector<int64_t> *cuts = nullptr) {
+                 const std::vector<int64_t> *cuts = nullptr, bool from_top = false) {
   int64_t ngroups = 0, passes_total = 0;
   for (int l = lmin; l < lmax; ++l) {
     const int64_t g = cdiv(nl[l], nt);
@@ -431,13 +448,13 @@ void plan_direct(const std::vector<int32_t> &nl, int lmin, int lmax, int64_t npt
       pl->partial_size += S * g.stride;
       pl->groups.push_back(g);
       for (int64_t s = 0; s < S; ++s) {
-        for (int qlo = 0; qlo < Q; qlo += qmax_cap) {
+        for (const auto &qr : chunk_ranges(Q, qmax_cap, from_top)) {
           DirectItem it;
           it.l = l;
           it.n0 = n0;
           it.nrows = g.nrows;
-          it.qlo = qlo;
-          it.qhi = std::min(Q, qlo + qmax_cap);
+          it.qlo = qr.first;
+          it.qhi = qr.second;
           it.cat = cx.cat;
           it.tab = cx.kln ? (int32_t)std::min<double>(
                                 std::floor(cx.kln[(size_t)l * cx.nmax + n0 + g.nrows - 1] * cx.R / MX_H) + 2, 1e9)
@@ -1021,13 +1038,13 @@ void plan_mixed2(const std::vector<int32_t> &nl, int lmin, int lmax, int64_t npt
     pl->partial_size += S * g.stride;
     pl->groups.push_back(g);
     for (int64_t s = 0; s < S; ++s) {
-      for (int qlo = 0; qlo < Q; qlo += qmax_cap) {
+      for (const auto &qr : chunk_ranges(Q, qmax_cap, true)) {
         DirectItem it;
         it.l = l;
         it.n0 = sg.n0;
         it.nrows = sg.rows;
-        it.qlo = qlo;
-        it.qhi = std::min(Q, qlo + qmax_cap);
+        it.qlo = qr.first;
+        it.qhi = qr.second;
         it.cat = cx.cat;
         it.tab = (int32_t)std::min<double>(
             std::floor(cx.kln[(size_t)l * cx.nmax + sg.n0 + sg.rows - 1] * cx.R / MX_H) + 2, 1e9);
@@ -1843,7 +1860,7 @@ sfbhip_status sfbhip_direct(sfbhip_handle *h, const double *d_points, int64_t n_
       plan_mixed2(h->kc.nl, lsplit, p->lmax, n_points, mixed2_qcap(p), target, cx, &plmx);
     else
       plan_direct(h->kc.nl, lsplit, p->lmax, n_points, MX_NT, MX_TP, MX_MC, mixed_qcap(p),
-                  target, cx, &plmx, mixed_ks_cap(p->mixed_operands));
+                  target, cx, &plmx, mixed_ks_cap(p->mixed_operands), nullptr, true);
   }
   std::vector<ReduceGroup> groups = pllw.groups;
   groups.insert(groups.end(), pl64.groups.begin(), pl64.groups.end());
@@ -2140,7 +2157,7 @@ sfbhip_status sfbhip_direct_batched(sfbhip_handle *h, const sfbhip_catalog *cats
                     ctx(c), &plmx);
       else
         plan_direct(nl[c], std::min(lsplit, cats[c].lmax), cats[c].lmax, cats[c].n_points, MX_NT, MX_TP, MX_MC,
-                    mixed_qcap(p), target(c), ctx(c), &plmx, mixed_ks_cap(p->mixed_operands));
+                    mixed_qcap(p), target(c), ctx(c), &plmx, mixed_ks_cap(p->mixed_operands), nullptr, true);
     }
   }
   std::vector<ReduceGroup> groups = pllw.groups;
@@ -2863,13 +2880,13 @@ void plan_grouped(const std::vector<int32_t> &nl, int lmin, int lmax, int64_t np
       const int32_t tab = (int32_t)std::min<double>(
           std::floor(cx.kln[(size_t)l * cx.nmax + n0 + g.nrows - 1] * cx.R / MX_H) + 2, 1e9);
       for (int64_t s = 0; s < S; ++s) {
-        for (int qlo = 0; qlo < Q; qlo += qcap) {
+        for (const auto &qr : chunk_ranges(Q, qcap, true)) {
           DirectItem it;
           it.l = l;
           it.n0 = n0;
           it.nrows = g.nrows;
-          it.qlo = qlo;
-          it.qhi = std::min(Q, qlo + qcap);
+          it.qlo = qr.first;
+          it.qhi = qr.second;
           it.cat = cx.cat;
           it.tab = tab;
           it.flags = ks;

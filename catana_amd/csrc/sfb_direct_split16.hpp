@@ -184,7 +184,6 @@ __device__ __forceinline__ void split16_item(const DirectItem &it, const PointsS
   // rows past nrows take the first row's k (their results are not read)
   const double khr = kln[it.kofs + (rowok ? row : 0)] * (1.0 / MX_H);
   const float khr32 = (float)khr;
-  const double rlo = pts.r[it.p0];
   const double2 sect = leg.sect[l];
   double *cg = sh.cg + S16_GPAD;
   {
@@ -199,12 +198,19 @@ __device__ __forceinline__ void split16_item(const DirectItem &it, const PointsS
 #pragma unroll
     for (int c = 0; c < 2; ++c) sum[q][c] = v4d{0.0, 0.0, 0.0, 0.0};
 
+  // padding points read the item's first point (finite Bessel values; their
+  // Y* columns are zero, so they add nothing)
+  double rn = pts.r[it.p0 + tid < it.p1 ? it.p0 + tid : it.p0];
   for (int64_t base = it.p0; base < it.p1; base += MX_TP) {
     const int64_t gp = base + tid;
     const bool pok = gp < it.p1;
-    // padding points take the item's first radius (finite Bessel values; their
-    // Y* columns are zero, so they add nothing)
-    const double r = pok ? pts.r[gp] : rlo;
+    const int64_t gq = pok ? gp : it.p0;
+    const double r = rn;
+    {
+      // the next tile's radius, loaded now so that it arrives during this tile
+      const int64_t gn = gp + MX_TP;
+      rn = pts.r[gn < it.p1 ? gn : it.p0];
+    }
     // offset from the middle point (lane 4) of this thread's 8-point group and
     // the group's largest |offset|; the group is 8 consecutive lanes of one wave
     const double rmid = __shfl(r, (lane & ~7) | 4);
@@ -221,15 +227,25 @@ __device__ __forceinline__ void split16_item(const DirectItem &it, const PointsS
     // ---- 1. Bessel values of this lane's row into A fragments (pieces 0, 1)
     h8 ah[2][SPW];
     if (wave_rows) {
+      // every step's interval and coefficients first: the SPW table loads are
+      // then in flight together (issued per step, each waited for in turn)
+      float4 cfs[SPW];
+      float fms[SPW];
 #pragma unroll
       for (int st = 0; st < SPW; ++st) {
         const int j = 4 * (kp * SPW + st) + g;  // 8-point group: tile points 8j .. 8j+7
         const double t = sh.rs[8 * j + 4] * khr;
         int i = (int)t;
         i = i < nint - 1 ? i : nint - 1;
-        const float fm = (float)(t - (double)i);
+        fms[st] = (float)(t - (double)i);
+        cfs[st] = T[i];
+      }
+#pragma unroll
+      for (int st = 0; st < SPW; ++st) {
+        const int j = 4 * (kp * SPW + st) + g;
+        const float fm = fms[st];
         const float sp = sh.gs[j] * khr32;
-        const float4 cf = T[i];
+        const float4 cf = cfs[st];
         unsigned hw[4], lw[4];
         if (__all((fm - sp >= S16_FLO) && (fm + sp <= S16_FHI))) {
           const float c0 = cf.x * MX_SCALE, c1 = cf.y * MX_SCALE, c2 = cf.z * MX_SCALE, c3 = cf.w * MX_SCALE;
@@ -258,16 +274,15 @@ __device__ __forceinline__ void split16_item(const DirectItem &it, const PointsS
 
     // ---- 2. Y* generator of this thread's point, positioned at the item's top chunk
     YlmScaled y;
-    if (pok) {
+    {
       const int mstart = l - MX_MC * it.qlo;
-      y.init(l, mstart, pts.x[gp], pts.s[gp], pts.phi[gp], pts.cphi[gp], pts.sphi[gp], sect);
+      y.init(l, mstart, pts.x[gq], pts.s[gq], pts.phi[gq], pts.cphi[gq], pts.sphi[gq], sect);
       for (int m = l; m > mstart; m -= 8) {
 #pragma unroll
         for (int k = 0; k < 8; ++k) y.step_p(cg[m - k]);
         y.rescale_nb();
       }
-    } else {
-      y.zero();
+      if (!pok) y.zero();
     }
     // per chunk: all threads generate it into LDS, then the waves with rows
     // contract it (generating the next chunk while contracting this one, from
@@ -383,8 +398,8 @@ struct GroupedLanes {
 // count (1, 2 or up to 4) selects, so items of every chunk count and
 // catalogue share one launch with no tail per class.  One kernel per KS: a
 // single kernel over all nine (chunks, KS) classes needs more than the 256
-// registers of two blocks per CU and spills (196 bytes per lane), one KS with
-// its three chunk counts does not.
+// registers of two blocks per CU and spills (196 bytes per lane); one KS with
+// its three chunk counts spills at most 12 bytes per lane (KS = 1).
 template <int KS>
 __global__ __launch_bounds__(256, 2) void sfb_direct_split16_grouped_kernel(
     PointsSoA pts, const DirectItem *__restrict__ items, GroupedLanes lanes, int *__restrict__ counters,
