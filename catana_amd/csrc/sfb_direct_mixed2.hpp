@@ -145,23 +145,19 @@ struct YlmScaled {
     const double se = fmax(s, 0x1p-70);
     cot = x / se;
     c2 = cph + cph;
-    int e = 0, be, t;
-    double m = 1.0;
-    double b = frexp(se, &be);
+    // se^l = b^l 2^(l be) with b = frexp(se) in [0.5, 1): b^l >= 2^-l is a
+    // normal FP64 number for l < MX_LMAX, so plain square-and-multiply needs
+    // no renormalisation, and one frexp puts the result in [0.5, 1)
+    static_assert(MX_LMAX < 1000, "b^l must stay a normal FP64 number");
+    int be, t;
+    double b = frexp(se, &be), m = 1.0;
     for (int k = l; k; k >>= 1) {
-      if (k & 1) {
-        m *= b;
-        e += be;
-        m = frexp(m, &t);
-        e += t;
-      }
+      if (k & 1) m *= b;
       b *= b;
-      be *= 2;
-      b = frexp(b, &t);
-      be += t;
     }
+    m = frexp(m, &t);
     v = m * sect.x;
-    E = e + (int)sect.y;
+    E = l * be + t + (int)sect.y;
     vp = 0.0;
     const double hi = (double)m0 * phi;
     const double lo = fma((double)m0, phi, -hi);
