@@ -45,7 +45,8 @@
 namespace sfb {
 
 constexpr int S16_GPAD = 16;  // g_m table padding below m = 0
-constexpr float S16_FLO = -0.2f, S16_FHI = 1.2f;  // range of f served by one interval's cubic
+// range of f served by one interval's cubic: [-0.2, 1.2], i.e. |f - 0.5| <= 0.7
+constexpr float S16_FMID = 0.5f, S16_FHALF = 0.7f;
 
 // Y* chunk in LDS, point-major: plane k (FP16 piece k) holds one row of 32
 // columns (2 mm: real, 2 mm + 1: imaginary part of m = mtop - mm) per tile
@@ -247,7 +248,7 @@ __device__ __forceinline__ void split16_item(const DirectItem &it, const PointsS
         const float sp = sh.gs[j] * khr32;
         const float4 cf = cfs[st];
         unsigned hw[4], lw[4];
-        if (__all((fm - sp >= S16_FLO) && (fm + sp <= S16_FHI))) {
+        if (__all(fabsf(fm - S16_FMID) + sp <= S16_FHALF)) {
           const float c0 = cf.x * MX_SCALE, c1 = cf.y * MX_SCALE, c2 = cf.z * MX_SCALE, c3 = cf.w * MX_SCALE;
           const float4 da = *reinterpret_cast<const float4 *>(sh.dg + 8 * j);
           const float4 db = *reinterpret_cast<const float4 *>(sh.dg + 8 * j + 4);
