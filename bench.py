@@ -70,8 +70,8 @@ def make_batch(seed=0, ncat=64):
 # device-to-device copy (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0).  Counters cannot be read from
 # inside the timed run, so the figure is carried here per workload: (workload, precision,
 # operands, fp64_below, chunks) for the point path, (workload, precision) for the others.
-PROFILED_TRAFFIC_BYTES = {("clustered1m", "mixed", "split16", 2, 4): 7.60e8,   # split16 classes
-                          ("healpix4m", "mixed"): 2.001e10,                    # maps + ring contraction
+PROFILED_TRAFFIC_BYTES = {("clustered1m", "mixed", "split16", 2, 4): 7.59e8,   # split16 classes
+                          ("healpix4m", "mixed"): 2.000e10,                    # maps + ring contraction
                           ("ingest2p25",): 1.337e9}                            #  fp64_below, chunks)
 
 
