@@ -228,8 +228,9 @@ __device__ __forceinline__ void split16_item(const DirectItem &it, const PointsS
     // ---- 1. Bessel values of this lane's row into A fragments (pieces 0, 1)
     h8 ah[2][SPW];
     if (wave_rows) {
-      // every step's interval and coefficients first: the SPW table loads are
-      // then in flight together (issued per step, each waited for in turn)
+      // every step's interval and coefficients first, so that the SPW table
+      // loads are in flight together (loaded inside each step's branch, each
+      // load was waited for in turn)
       float4 cfs[SPW];
       float fms[SPW];
 #pragma unroll
