@@ -947,19 +947,20 @@ sfbhip_status build_cubic(sfbhip_handle *h, int L, const std::vector<double> &xm
     mx = std::max<int32_t>(mx, (int32_t)n);
     if (o > INT32_MAX) return fail(SFBHIP_NOT_SUPPORTED, "mixed path: Bessel tables too large");
   }
-  float4 *tab;
+  float4 *tab, *tab16;
   int32_t *doff, *dni;
   CHK(get_buf(h, tag + "_tab", (size_t)std::max<int64_t>(o, 1), &tab));
+  CHK(get_buf(h, tag + "_tab16", (size_t)std::max<int64_t>(o, 1), &tab16));
   CHK(get_buf(h, tag + "_off", off.size(), &doff));
   CHK(get_buf(h, tag + "_nint", ni.size(), &dni));
   HIPCHK(hipMemcpyAsync(doff, off.data(), sizeof(int32_t) * off.size(), hipMemcpyHostToDevice, st));
   HIPCHK(hipMemcpyAsync(dni, ni.data(), sizeof(int32_t) * ni.size(), hipMemcpyHostToDevice, st));
   if (L > 0) {
-    cubic_table_kernel<<<dim3((unsigned)cdiv(mx, 256), (unsigned)L), 256, 0, st>>>(L, doff, dni, tab);
+    cubic_table_kernel<<<dim3((unsigned)cdiv(mx, 256), (unsigned)L), 256, 0, st>>>(L, doff, dni, tab, tab16);
     HIPCHK(hipGetLastError());
   }
   HIPCHK(hipStreamSynchronize(st));  // host vectors go out of scope
-  *ct = CubicTables{tab, doff, dni};
+  *ct = CubicTables{tab, doff, dni, tab16};
   if (nint_out) *nint_out = ni;
   return SFBHIP_OK;
 }
@@ -979,12 +980,13 @@ sfbhip_status ensure_cubic(sfbhip_handle *h, const sfbhip_params *p, double rdat
     c.cub_valid = true;
     return SFBHIP_OK;
   }
-  float4 *tab;
+  float4 *tab, *tab16;
   int32_t *doff, *dni;
   CHK(get_buf(h, "cub_tab", 1, &tab));
+  CHK(get_buf(h, "cub_tab16", 1, &tab16));
   CHK(get_buf(h, "cub_off", 1, &doff));
   CHK(get_buf(h, "cub_nint", 1, &dni));
-  *ct = CubicTables{tab, doff, dni};
+  *ct = CubicTables{tab, doff, dni, tab16};
   return SFBHIP_OK;
 }
 
@@ -3136,14 +3138,15 @@ sfbhip_status plan_create(sfbhip_handle *h, const sfbhip_catalog *cats, int32_t 
       mx = std::max<int32_t>(mx, (int32_t)k);
       if (co > INT32_MAX) return fail(SFBHIP_NOT_SUPPORTED, "mixed path: Bessel tables too large");
     }
-    float4 *tb;
+    float4 *tb, *tb16;
     int32_t *doff, *dni;
     CHK(plan_alloc(pl, (size_t)co, &tb));
+    CHK(plan_alloc(pl, (size_t)co, &tb16));
     CHK(plan_upload(pl, coff, &doff));
     CHK(plan_upload(pl, cni, &dni));
-    cubic_table_kernel<<<dim3((unsigned)cdiv(mx, 256), (unsigned)L), 256, 0, nullptr>>>(L, doff, dni, tb);
+    cubic_table_kernel<<<dim3((unsigned)cdiv(mx, 256), (unsigned)L), 256, 0, nullptr>>>(L, doff, dni, tb, tb16);
     HIPCHK(hipGetLastError());
-    pl->ct = CubicTables{tb, doff, dni};
+    pl->ct = CubicTables{tb, doff, dni, tb16};
     std::vector<double2> hs;
     std::vector<double> hg, hd;
     std::vector<int32_t> ho;

@@ -44,11 +44,13 @@ constexpr int M2_GPAD = 16;   // g_m table padding below m = 0
 
 // Cubic coefficients per interval: tab[off[l] + i] = (c0, c1, c2, c3) with
 // j_l((i + f) h) ~ c0 + f (c1 + f (c2 + f c3)), 0 <= f < 1 (cubic Hermite on
-// the values and slopes at both ends), i < nint[l].
+// the values and slopes at both ends), i < nint[l]; tab16 holds the same
+// coefficients times 2^12 (MX_SCALE), the scale of the split-FP16 operands.
 struct CubicTables {
   const float4 *tab;
   const int32_t *off;
   const int32_t *nint;
+  const float4 *tab16;
 };
 
 // Legendre constants of the scaled recurrence, per l at off[l] = l (l+1) / 2:
@@ -72,7 +74,7 @@ __device__ inline void jl_value_slope(int l, double x, double &j, double &dj) {
 }
 
 __global__ void cubic_table_kernel(int lmax, const int32_t *__restrict__ off, const int32_t *__restrict__ nint,
-                                   float4 *__restrict__ tab) {
+                                   float4 *__restrict__ tab, float4 *__restrict__ tab16) {
   const int l = blockIdx.y;
   if (l >= lmax) return;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nint[l]; i += gridDim.x * blockDim.x) {
@@ -80,7 +82,9 @@ __global__ void cubic_table_kernel(int lmax, const int32_t *__restrict__ off, co
     jl_value_slope(l, i * MX_H, j0, d0);
     jl_value_slope(l, (i + 1) * MX_H, j1, d1);
     const double s0 = MX_H * d0, s1 = MX_H * d1, dj = j1 - j0;
-    tab[off[l] + i] = make_float4((float)j0, (float)s0, (float)(3.0 * dj - 2.0 * s0 - s1), (float)(s0 + s1 - 2.0 * dj));
+    const float4 c = make_float4((float)j0, (float)s0, (float)(3.0 * dj - 2.0 * s0 - s1), (float)(s0 + s1 - 2.0 * dj));
+    tab[off[l] + i] = c;
+    tab16[off[l] + i] = make_float4(c.x * MX_SCALE, c.y * MX_SCALE, c.z * MX_SCALE, c.w * MX_SCALE);
   }
 }
 
@@ -101,6 +105,17 @@ __device__ __forceinline__ float jl_cubic_lds(const float4 *C, double r, double 
   const int i = (int)t;
   const float f = (float)__builtin_amdgcn_fract(t);
   const float4 c = C[i];
+  return fmaf(fmaf(fmaf(c.w, f, c.z), f, c.y), f, c.x);
+}
+
+// j_l 2^12 at t = r k / h from the scaled table (tab16): the cubic of interval
+// floor(t), clamped to the table.
+__device__ __forceinline__ float jl_cubic_scaled(const float4 *__restrict__ T16, int nint, double t) {
+  int i = (int)t;
+  i = i < nint - 1 ? i : nint - 1;
+  i = i > 0 ? i : 0;
+  const float f = (float)(t - (double)i);
+  const float4 c = T16[i];
   return fmaf(fmaf(fmaf(c.w, f, c.z), f, c.y), f, c.x);
 }
 

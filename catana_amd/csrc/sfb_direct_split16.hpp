@@ -181,7 +181,7 @@ __device__ __forceinline__ void split16_item(const DirectItem &it, const PointsS
   const bool rowok = row < it.nrows;
 
   const int nint = ct.nint[l];
-  const float4 *__restrict__ T = ct.tab + ct.off[l];
+  const float4 *__restrict__ T = ct.tab16 + ct.off[l];  // coefficients of j_l 2^12
   // rows past nrows take the first row's k (their results are not read)
   const double khr = kln[it.kofs + (rowok ? row : 0)] * (1.0 / MX_H);
   const float khr32 = (float)khr;
@@ -250,7 +250,7 @@ __device__ __forceinline__ void split16_item(const DirectItem &it, const PointsS
         const float4 cf = cfs[st];
         unsigned hw[4], lw[4];
         if (__all(fabsf(fm - S16_FMID) + sp <= S16_FHALF)) {
-          const float c0 = cf.x * MX_SCALE, c1 = cf.y * MX_SCALE, c2 = cf.z * MX_SCALE, c3 = cf.w * MX_SCALE;
+          const float c0 = cf.x, c1 = cf.y, c2 = cf.z, c3 = cf.w;
           const float4 da = *reinterpret_cast<const float4 *>(sh.dg + 8 * j);
           const float4 db = *reinterpret_cast<const float4 *>(sh.dg + 8 * j + 4);
           const float dd[8] = {da.x, da.y, da.z, da.w, db.x, db.y, db.z, db.w};
@@ -265,7 +265,7 @@ __device__ __forceinline__ void split16_item(const DirectItem &it, const PointsS
         } else {
           float v[8];
 #pragma unroll
-          for (int e = 0; e < 8; ++e) v[e] = jl_cubic_global(T, nint, sh.rs[8 * j + e] * khr);
+          for (int e = 0; e < 8; ++e) v[e] = jl_cubic_scaled(T, nint, sh.rs[8 * j + e] * khr);
 #pragma unroll
           for (int e = 0; e < 8; e += 2) split2(v[e], v[e + 1], hw[e / 2], lw[e / 2]);
         }
