@@ -99,6 +99,7 @@ struct KlnCache {
 
 namespace {
 struct RingPlanCache;
+struct PointPlanCache;
 }
 
 struct sfbhip_handle {
@@ -120,6 +121,7 @@ struct sfbhip_handle {
   int pix_nside = -1;  // pixel tables on the device are for this nside
   int ring_nside = -1;  // ring tables (cos/sin theta per ring, ring of each pixel) are for this nside
   std::shared_ptr<RingPlanCache> ring_plan;  // the last single-batch ring plan and its shape
+  std::shared_ptr<PointPlanCache> point_plan;  // the last plan of sfbhip_direct and its shape
   int rleg_nside = -1, rleg_lmax = -1;  // ring Legendre table is for this (nside, lmax)
   int ig_L = -1, ig_nz = -1;  // zero rows of the sbessel integrals on the device
   int ncu = 256;  // compute units of the device
@@ -379,6 +381,20 @@ struct RingPlanCache {
   int64_t partial_size = 0;
   std::array<int, 6> counts{};       // ring item classes on the device
   int64_t nmap = 0;                  // map items on the device
+};
+
+// The last plan of sfbhip_direct: its work items are on the device
+// (px_items), so a call of the same shape plans and uploads nothing.  The
+// shape: the point count, the k_ln parameters (lmax, nmax, rmax, k_max, zeros),
+// the precision, the FP64 split, the kernel and chunk options, and the radius
+// the Bessel tables cover (each item's table reach depends on it).
+struct PointPlanCache {
+  int64_t n = 0;  // 0: no plan kept
+  int lmax = 0, nmax = 0, zeros = 0, lsplit = 0, llow = 0, kernel = 0, chunks = 0, operands = 0;
+  bool mixed = false;
+  double rmax = 0.0, kmax = 0.0, R = 0.0;
+  DirectPlan pllw, pl64, plmx;
+  std::vector<ReduceGroup> groups;  // every group of the call (norm set per call)
 };
 
 // Where one catalogue's items and groups point: its k_ln rows (k_{l,n} at
@@ -1841,47 +1857,77 @@ sfbhip_status sfbhip_direct(sfbhip_handle *h, const double *d_points, int64_t n_
   // mixed path: l < min(lsplit, LW_LMAX) in the low-l FP64 kernel, the rest of
   // l < lsplit in the general FP64 kernel (both with the quintic table)
   const int llow = mixed ? std::min(lsplit, LW_LMAX) : 0;
-  DirectPlan pllw, pl64, plmx;
-  PlanCtx cx;
-  cx.nmax = p->nmax;
-  cx.norm = norm;
-  cx.kln = h->kc.kln.data();
-  cx.R = std::max(p->rmax, rdata);
-  // the low-l items: 1024 point ranges per (l, 16-row group) (the partial
-  // reduction sums them in eight chains; 256 ranges: 0.04 ms slower at 1M points)
-  if (llow > 0)
-    plan_direct(h->kc.nl, 0, llow, n_points, LW_ROWS, LW_TILE, F64_MC, 1, 1024 * (int64_t)llow * (64 / LW_ROWS), cx,
-                &pllw);
-  pl64.partial_size = pllw.partial_size;
-  plan_direct(h->kc.nl, llow, lsplit, n_points, F64_NT, F64_TP, F64_MC, 16, f64_target_items(n_points, false), cx,
-              &pl64);
-  plmx.partial_size = pl64.partial_size;
-  if (mixed) {
-    const int64_t target = mixed_target_items(n_points);
-    if (m2)
-      plan_mixed2(h->kc.nl, lsplit, p->lmax, n_points, mixed2_qcap(p), target, cx, &plmx);
-    else
-      plan_direct(h->kc.nl, lsplit, p->lmax, n_points, MX_NT, MX_TP, MX_MC, mixed_qcap(p),
-                  target, cx, &plmx, mixed_ks_cap(p->mixed_operands), nullptr, true);
+  if (!h->point_plan) h->point_plan = std::make_shared<PointPlanCache>();
+  PointPlanCache &pc = *h->point_plan;
+  const double R = std::max(p->rmax, rdata);
+  const bool hit = pc.n == n_points && pc.lmax == p->lmax && pc.nmax == p->nmax && pc.zeros == p->zeros &&
+                   pc.rmax == p->rmax && pc.kmax == p->k_max && pc.mixed == mixed && pc.lsplit == lsplit &&
+                   pc.llow == llow && pc.kernel == p->mixed_kernel && pc.chunks == p->mixed_chunks &&
+                   pc.operands == p->mixed_operands && pc.R == R;
+  DirectPlan &pllw = pc.pllw, &pl64 = pc.pl64, &plmx = pc.plmx;
+  std::vector<ReduceGroup> &groups = pc.groups;
+  std::vector<DirectItem> all;  // the items of a new plan; copied to the device before the call returns
+  DirectItem *ditems;
+  if (!hit) {
+    pc.n = 0;  // no plan kept until this one is on the device
+    pllw = DirectPlan{};
+    pl64 = DirectPlan{};
+    plmx = DirectPlan{};
+    PlanCtx cx;
+    cx.nmax = p->nmax;
+    cx.norm = norm;
+    cx.kln = h->kc.kln.data();
+    cx.R = R;
+    // the low-l items: 1024 point ranges per (l, 16-row group) (the partial
+    // reduction sums them in eight chains; 256 ranges: 0.04 ms slower at 1M points)
+    if (llow > 0)
+      plan_direct(h->kc.nl, 0, llow, n_points, LW_ROWS, LW_TILE, F64_MC, 1, 1024 * (int64_t)llow * (64 / LW_ROWS),
+                  cx, &pllw);
+    pl64.partial_size = pllw.partial_size;
+    plan_direct(h->kc.nl, llow, lsplit, n_points, F64_NT, F64_TP, F64_MC, 16, f64_target_items(n_points, false),
+                cx, &pl64);
+    plmx.partial_size = pl64.partial_size;
+    if (mixed) {
+      const int64_t target = mixed_target_items(n_points);
+      if (m2)
+        plan_mixed2(h->kc.nl, lsplit, p->lmax, n_points, mixed2_qcap(p), target, cx, &plmx);
+      else
+        plan_direct(h->kc.nl, lsplit, p->lmax, n_points, MX_NT, MX_TP, MX_MC, mixed_qcap(p), target, cx, &plmx,
+                    mixed_ks_cap(p->mixed_operands), nullptr, true);
+    }
+    groups = pllw.groups;
+    groups.insert(groups.end(), pl64.groups.begin(), pl64.groups.end());
+    groups.insert(groups.end(), plmx.groups.begin(), plmx.groups.end());
+    for (auto &v : pllw.items) all.insert(all.end(), v.begin(), v.end());
+    for (auto &v : pl64.items) all.insert(all.end(), v.begin(), v.end());
+    // split-FP16 classes in XCD order (each class keeps its place in the list)
+    for (auto &v : plmx.items)
+      if (mixed && s16)
+        append_xcd(all, v, plmx.chunk);
+      else
+        all.insert(all.end(), v.begin(), v.end());
+    CHK(get_buf(h, "px_items", std::max<size_t>(all.size(), 1), &ditems));
+    if (!all.empty())
+      HIPCHK(hipMemcpyAsync(ditems, all.data(), sizeof(DirectItem) * all.size(), hipMemcpyHostToDevice, st));
+    pc.lmax = p->lmax;
+    pc.nmax = p->nmax;
+    pc.zeros = p->zeros;
+    pc.rmax = p->rmax;
+    pc.kmax = p->k_max;
+    pc.mixed = mixed;
+    pc.lsplit = lsplit;
+    pc.llow = llow;
+    pc.kernel = p->mixed_kernel;
+    pc.chunks = p->mixed_chunks;
+    pc.operands = p->mixed_operands;
+    pc.R = R;
+    pc.n = n_points;
+  } else {
+    CHK(get_buf(h, "px_items", 1, &ditems));
   }
-  std::vector<ReduceGroup> groups = pllw.groups;
-  groups.insert(groups.end(), pl64.groups.begin(), pl64.groups.end());
-  groups.insert(groups.end(), plmx.groups.begin(), plmx.groups.end());
+  for (ReduceGroup &g : groups) g.norm = norm;
   double *partial = nullptr;
   CHK(get_buf(h, "partial", (size_t)std::max<int64_t>(plmx.partial_size, 1), &partial));
-  std::vector<DirectItem> all;
-  for (auto &v : pllw.items) all.insert(all.end(), v.begin(), v.end());
-  for (auto &v : pl64.items) all.insert(all.end(), v.begin(), v.end());
-  // split-FP16 classes in XCD order (each class keeps its place in the list)
-  for (auto &v : plmx.items)
-    if (mixed && s16)
-      append_xcd(all, v, plmx.chunk);
-    else
-      all.insert(all.end(), v.begin(), v.end());
-  DirectItem *ditems;
-  CHK(get_buf(h, "items", std::max<size_t>(all.size(), 1), &ditems));
-  if (!all.empty())
-    HIPCHK(hipMemcpyAsync(ditems, all.data(), sizeof(DirectItem) * all.size(), hipMemcpyHostToDevice, st));
   clear_timings(h);
   HIPCHK(hipEventRecord(h->ev[0], st));
   size_t off = 0;

@@ -235,3 +235,29 @@ def test_mixed_lmax256_third_row_group_and_global_table(handle, n, kern):
     assert _rel(cm[:, 128:], c64[:, 128:]).max() < 1e-6
     assert _flmn_err(fm, f64) < 1e-6
     assert np.all((cm == 0) == (c64 == 0))
+
+
+@pytest.mark.parametrize("prec", [capi.PREC_MIXED, capi.PREC_FP64])
+def test_plan_kept_between_calls(handle, prec):
+    # sfbhip_direct keeps its last plan (work items on the device) for calls of the
+    # same shape: the second catalogue of the same size and radius bound reuses it,
+    # other sizes and multipole counts in between replace it; every result must be
+    # bit-identical to what a fresh handle computes for the same call
+    a = workloads.uniform_ball(5000, 500.0, seed=21)
+    b = workloads.uniform_ball(5000, 500.0, seed=22)
+    c = workloads.uniform_ball(4000, 500.0, seed=23)
+    seq = [(a, 40), (b, 40), (c, 40), (b, 40), (a, 24), (b, 40)]
+    print(f"case: plan reuse, precision {prec}, N = 5000/5000/4000, lmax 40/24, nmax 16, rmax 500", flush=True)
+    got = [_run(handle, p, L, 16, 500.0, precision=prec) for p, L in seq]
+    fresh = capi.Handle(0)
+    try:
+        for (p, L), g in zip(seq, got):
+            fresh.close()
+            fresh = capi.Handle(0)
+            ref = _run(fresh, p, L, 16, 500.0, precision=prec)
+            np.testing.assert_array_equal(g[0], ref[0])
+            np.testing.assert_array_equal(g[1], ref[1])
+            for x, y in zip(g[2], ref[2]):
+                np.testing.assert_array_equal(x, y)
+    finally:
+        fresh.close()
