@@ -538,28 +538,29 @@ sfbhip_status prep_points(sfbhip_handle *h, const double *d_points, int64_t n, h
 // The point table reordered by radius (stable), so that consecutive points
 // have close Bessel arguments k r: the table loads of a tile then stay within
 // a few cache lines.  The sum over points does not depend on the order beyond
-// rounding.
+// rounding.  The keys are r rounded to FP32 (four 8-bit radix passes instead of
+// the eight of its FP64 pattern).
 sfbhip_status sort_points_by_radius(sfbhip_handle *h, const PointsSoA &in, hipStream_t st, PointsSoA *out) {
   const int64_t n = in.n;
   if (n < 2 || n > INT32_MAX) {
     *out = in;
     return SFBHIP_OK;
   }
-  unsigned long long *k0, *k1;
+  unsigned *k0, *k1;
   int32_t *i0, *i1;
   double *buf;
-  CHK(get_buf(h, "sort_k0", (size_t)n, &k0));
-  CHK(get_buf(h, "sort_k1", (size_t)n, &k1));
+  CHK(get_buf(h, "sort_k32a", (size_t)n, &k0));
+  CHK(get_buf(h, "sort_k32b", (size_t)n, &k1));
   CHK(get_buf(h, "sort_i0", (size_t)n, &i0));
   CHK(get_buf(h, "sort_i1", (size_t)n, &i1));
   CHK(get_buf(h, "soa_sorted", (size_t)6 * n, &buf));
-  radius_keys_kernel<<<cdiv(n, 256), 256, 0, st>>>(in.r, n, k0, i0);
+  radius_keys32_kernel<<<cdiv(n, 256), 256, 0, st>>>(in.r, n, k0, i0);
   HIPCHK(hipGetLastError());
   size_t tmp_bytes = 0;
-  HIPCHK(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_bytes, k0, k1, i0, i1, (int)n, 0, 64, st));
+  HIPCHK(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_bytes, k0, k1, i0, i1, (int)n, 0, 32, st));
   char *tmp;
   CHK(get_buf(h, "sort_tmp", tmp_bytes, &tmp));
-  HIPCHK(hipcub::DeviceRadixSort::SortPairs(tmp, tmp_bytes, k0, k1, i0, i1, (int)n, 0, 64, st));
+  HIPCHK(hipcub::DeviceRadixSort::SortPairs(tmp, tmp_bytes, k0, k1, i0, i1, (int)n, 0, 32, st));
   double *r = buf, *x = buf + n, *s = buf + 2 * n, *ph = buf + 3 * n, *cp = buf + 4 * n, *sp = buf + 5 * n;
   gather_points_kernel<<<cdiv(n, 256), 256, 0, st>>>(in, i1, n, r, x, s, ph, cp, sp);
   HIPCHK(hipGetLastError());

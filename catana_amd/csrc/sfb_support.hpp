@@ -41,13 +41,13 @@ __global__ void prep_points_kernel(const double *__restrict__ aos, int64_t n, do
   if (!(rr >= 0.0) || !isfinite(rr) || !isfinite(th) || !isfinite(ph)) atomicOr(err, ERR_BADPOINT);
 }
 
-// Sort keys for ordering the catalogue by radius: the bit pattern of r >= 0
-// orders like r.  idx = 0 .. n-1.
-__global__ void radius_keys_kernel(const double *__restrict__ r, int64_t n, unsigned long long *__restrict__ keys,
-                                   int32_t *__restrict__ idx) {
+// 32-bit radius keys of a single call: the FP32 pattern of r (monotone in r >= 0;
+// radii equal to FP32 rounding keep their input order in the stable sort).
+__global__ void radius_keys32_kernel(const double *__restrict__ r, int64_t n, unsigned *__restrict__ keys,
+                                     int32_t *__restrict__ idx) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  keys[i] = (unsigned long long)__double_as_longlong(r[i] > 0.0 ? r[i] : 0.0);  // -0.0 (and NaN) key as +0.0
+  keys[i] = __float_as_uint((float)(r[i] > 0.0 ? r[i] : 0.0));  // -0.0 (and NaN) key as +0.0
   idx[i] = (int32_t)i;
 }
 
