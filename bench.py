@@ -71,6 +71,7 @@ def make_batch(seed=0, ncat=64):
 # inside the timed run, so the figure is carried here per workload: (workload, precision,
 # operands, fp64_below, chunks) for the point path, (workload, precision) for the others.
 PROFILED_TRAFFIC_BYTES = {("clustered1m", "mixed", "split16", 2, 4): 7.59e8,   # split16 classes
+                          ("lognormal2p25", "mixed", "split16", 2, 4): 9.84e11,  # split16 classes
                           ("healpix4m", "mixed"): 2.000e10,                    # maps + ring contraction
                           ("ingest2p25",): 1.337e9}                            #  fp64_below, chunks)
 
